@@ -1,0 +1,182 @@
+"""Throughput of QGN's ARK43_QGLeith time stepper on one MI355X.
+
+Workload: expstrat_1024x16 (1024^2 x 16 layers, exponential stratification,
+surface-intensified shear, quadratic + Ekman drag, ARK43_QGLeith adaptive).
+A step is one accepted TimeStep (rejected attempts included in the time).
+Prints one JSON line; ``value`` is simulated days per wall-second of the
+mixed-precision path, with the FP64 path measured on the same GPU from the
+same spun-up state.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# algorithmic HBM bytes per launch of the profiled kernels
+def kernel_bytes(kernel, g, rs):
+    nz, ny, Kx, Ky = g["nz"], g["ny"], g["Kx"], g["Ky"]
+    ncmp = nz * Ky * Kx
+    nmix = nz * ny * Kx
+    drag = g["drag"]
+    if kernel == "row_jac":
+        return (3 + 2) * nmix * rs + (2 * ny * Kx * rs if drag else 0)
+    if kernel == "col_inv":
+        return ncmp * 16 + ncmp * rs + 3 * nmix * rs
+    if kernel == "col_rhs":
+        return 2 * nmix * rs + (2 * ny * Kx * rs if drag else 0) + ncmp * (16 + 2 * rs)
+    raise ValueError(kernel)
+
+
+def geom(p):
+    ny = p.ny
+    ya, yb = ny // 3, 2 * (ny // 3) + 1
+    return dict(nx=p.nx, ny=ny, nz=p.nz, Kx=p.nx // 3, Ky=ya + (ny - yb), drag=p.C_d != 0.0)
+
+
+def run_gpu(wl, precision, q_spun, t0, dt0, steps, warmup, prof_kernel):
+    import torch
+    from qgn_fortran_amd import QGN
+    from qgn_fortran_amd.model import vertical_modes
+
+    p = wl.params
+    p.precision = precision
+    H, _L, Modes, EVals, qy = vertical_modes(wl.H, wl.S, wl.uBar, p.Htot)
+    m = QGN(p, H, Modes, EVals, wl.uBar, qy)
+    m.set_state(q_spun)
+    t, dt = t0, dt0
+    for _ in range(warmup):
+        t, dt = m.time_step(t, dt)
+    st0 = m.stats()
+    m.prof_enable(prof_kernel)
+    torch.cuda.synchronize()
+    tw0 = time.perf_counter()
+    ts = t
+    for _ in range(steps):
+        t, dt = m.time_step(t, dt)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - tw0
+    nlaunch, kms = m.prof_read()
+    st1 = m.stats()
+    res = dict(
+        precision=precision, wall_s=wall, sim_days=(t - ts) / 86400.0,
+        days_per_s=(t - ts) / 86400.0 / wall, ms_per_step=1e3 * wall / steps,
+        mean_dt=(t - ts) / steps, rejects=st1["rejects"] - st0["rejects"],
+        rhs_evals=st1["rhs_evals"] - st0["rhs_evals"], kernel=prof_kernel,
+        kernel_launches=nlaunch, kernel_ms=kms / max(nlaunch, 1), t_end=t, dt_end=dt,
+    )
+    res["ms_per_rhs"] = 1e3 * wall / max(res["rhs_evals"], 1)
+    q_end = m.get_state()
+    m.close()
+    return res, q_end
+
+
+def spin_up(wl, steps, precision="mixed"):
+    from qgn_fortran_amd import QGN, initial
+    from qgn_fortran_amd.model import vertical_modes
+    p = wl.params
+    p.precision = precision
+    H, _L, Modes, EVals, qy = vertical_modes(wl.H, wl.S, wl.uBar, p.Htot)
+    m = QGN(p, H, Modes, EVals, wl.uBar, qy)
+    q0 = initial.random_phase_pv(p.nx, p.ny, p.nz, wl.k_peak, wl.rms, wl.seed, p.Lx, p.Ly)
+    m.initialize(q0)
+    t, dt = 0.0, wl.dt0
+    tw = time.perf_counter()
+    for i in range(steps):
+        t, dt = m.time_step(t, dt)
+        if (i + 1) % 500 == 0:
+            print(f"# spin-up {i + 1}/{steps}: t = {t / 86400:.2f} days, dt = {dt:.1f} s, "
+                  f"{time.perf_counter() - tw:.1f} s", file=sys.stderr, flush=True)
+    q = m.get_state()
+    m.close()
+    return q, t, dt
+
+
+def cpu_baseline(wl, q_spun, t0, dt0, budget_s=30.0):
+    """The FP64 oracle (scipy pocketfft, multithreaded FFTs) on the host: as
+    many accepted ARK43_QGLeith steps as fit in ~budget_s, at least one."""
+    from oracle import qgn_oracle as orc
+    from qgn_fortran_amd.model import vertical_modes
+    p = wl.params
+    cores = min(os.cpu_count() or 1, 16)
+    H, _L, Modes, EVals, qy = vertical_modes(wl.H, wl.S, wl.uBar, p.Htot)
+    po = orc.Params(nx=p.nx, ny=p.ny, nz=p.nz, Lx=p.Lx, Ly=p.Ly, Htot=p.Htot, beta=p.beta, A2=p.A2, A8=p.A8,
+                    r0=p.r0, C_d=p.C_d, TOL=p.TOL, is_explicit=p.is_explicit, dt_max=p.dt_max)
+    om = orc.Model(po, H, Modes, EVals, wl.uBar, qy, workers=cores)
+    stp = orc.ARK43(om, leith=True)
+    q, t, dt = q_spun, t0, dt0
+    tw = time.perf_counter()
+    n = 0
+    while True:
+        q, t, dt = stp.step(q, t, dt)
+        n += 1
+        if time.perf_counter() - tw > budget_s * 0.5 or n >= 20:
+            break
+    wall = time.perf_counter() - tw
+    return dict(value=(t - t0) / 86400.0 / wall, unit="simulated days/s", cores=cores, kind="port",
+                sample=f"{n} accepted ARK43_QGLeith step(s) of the same 1024^2x16 state, FP64 numpy/scipy "
+                       f"oracle, {wall:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--spinup", type=int, default=2000)
+    ap.add_argument("--workload", default="expstrat_1024x16")
+    ap.add_argument("--kernel", default="row_jac", help="kernel timed for the roofline figure")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=30.0)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (members/slabs would "
+              "shard across GPUs, see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from qgn_fortran_amd import configs
+
+    if not torch.cuda.is_available():
+        print("no GPU", file=sys.stderr)
+        sys.exit(1)
+    wl = configs.WORKLOADS[a.workload]()
+    q, t0, dt0 = spin_up(wl, a.spinup)
+    mixed, _ = run_gpu(configs.WORKLOADS[a.workload](), "mixed", q, t0, dt0, a.steps, a.warmup, a.kernel)
+    fp64, _ = run_gpu(configs.WORKLOADS[a.workload](), "fp64", q, t0, dt0, a.steps, a.warmup, a.kernel)
+    g = geom(wl.params)
+    rs = 8  # mixed path: FP32 complex intermediates
+    kb = kernel_bytes(a.kernel, g, rs)
+    achieved = kb / (mixed["kernel_ms"] * 1e-3) / 1e9
+    roof = dict(bound="hbm", achieved=round(achieved, 1), peak=8000.0, unit="GB/s",
+                frac=round(achieved / 8000.0, 4), traffic=None, kernel=a.kernel,
+                bytes_per_launch=kb, avg_launch_ms=mixed["kernel_ms"])
+    cpu = None
+    if not a.no_cpu:
+        cpu = cpu_baseline(configs.WORKLOADS[a.workload](), q, t0, dt0, a.cpu_budget)
+    out = {
+        "metric": "simulated days/wall-s, 1024^2x16-layer ARK43_QGLeith, mixed precision",
+        "value": round(mixed["days_per_s"], 4),
+        "unit": "simulated days/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(mixed["ms_per_step"], 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 FFT/Jacobian, f64 state/stages/error",
+        "data": "synthetic (seeded random-phase PV, spun up on the GPU)",
+        "config": {"workload": a.workload, "nx": wl.params.nx, "ny": wl.params.ny, "nz": wl.params.nz,
+                   "scheme": "ARK43_QGLeith", "spinup_steps": a.spinup, "parallelism": "single GPU"},
+        "mixed": mixed, "fp64": fp64,
+        "mixed_vs_fp64": round(mixed["days_per_s"] / fp64["days_per_s"], 3),
+        "roofline": roof, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

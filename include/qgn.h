@@ -1,0 +1,154 @@
+/* C-ABI of the MI355X (gfx950) QGN spectral right-hand side and time stepper.
+ *
+ * Drop-in replacement for the parts of iangrooms/QGN_Fortran that do the
+ * arithmetic: the MKL DFTI plans and transforms, and the QGN_Module
+ * procedures Initialize / TimeStep / GetRHS / WriteQP.  A Fortran host binds
+ * these with iso_c_binding (qgn_fortran_amd/host/qgn_c.f90).
+ *
+ * Conventions
+ *   - Arrays use the reference's Fortran memory layout: physical fields
+ *     real(nx,ny,nz), spectra complex(nx/2+1,ny,nz), x fastest, complex
+ *     interleaved (re,im) as C double[2] / Fortran complex(c_double_complex).
+ *   - Every function returns 0 on success or a negative QGN_E* code; the
+ *     reference ignored every DFTI status (QGN_Module.f90:180,452-455).
+ *     qgn_last_error() gives a message for the calling thread.
+ *   - The caller owns host arrays; a context owns its device buffers, plans
+ *     and HIP stream.  A context is not thread-safe; use one host thread per
+ *     context.  The prognostic state stays in device memory between steps.
+ */
+#ifndef QGN_H
+#define QGN_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  QGN_OK = 0,
+  QGN_EINVAL = -1,    /* bad argument or unsupported size */
+  QGN_EHIP = -2,      /* HIP runtime error */
+  QGN_ENOMEM = -3,    /* device allocation failed */
+  QGN_ENONFINITE = -4,/* NaN/Inf in the error norm of an adaptive step */
+  QGN_EREJECT = -5    /* adaptive step rejected more than max_rejects times */
+};
+
+/* TimeStep include fragments (QGN_Module.f90:306) */
+enum { QGN_AB3 = 0, QGN_RK4 = 1, QGN_BSRK32 = 2, QGN_ARK43 = 3, QGN_ARK43_QGLEITH = 4 };
+/* arithmetic of the RHS: FP64 throughout, or FP32 FFT/Jacobian/N_j with
+ * FP64 state, stage sums and error norm */
+enum { QGN_FP64 = 0, QGN_MIXED = 1 };
+
+/* Mirrors parameters.f90:1-31 (sizes become run-time values). */
+typedef struct qgn_params {
+  int nx, ny, nz;      /* nx even; nx/2 and ny products of 2, 3, 5 */
+  double Lx, Ly, Htot; /* m */
+  double beta;         /* 1/(m s) */
+  double A2, A8;       /* m^2/s, m^8/s */
+  double r0;           /* Ekman: drag on layer nz is r0*Htot/H(nz) */
+  double C_d;          /* quadratic drag */
+  double TOL;          /* adaptive-step error tolerance (max-norm, PV units) */
+  double dt_max;       /* s; parameters.f90:31 uses pi/(beta*Lx) */
+  int is_explicit;     /* parameters.f90:28 */
+  int scheme;          /* QGN_AB3 .. QGN_ARK43_QGLEITH */
+  int precision;       /* QGN_FP64 or QGN_MIXED */
+  int max_rejects;     /* bound on the reject loop; <= 0 means 1000 */
+} qgn_params;
+
+typedef struct qgn_ctx qgn_ctx;
+
+/* Module set-up.  Replaces the vertical-structure and plan parts of
+ * Initialize (QGN_Module.f90:53-175): H (nz, already rescaled to Htot),
+ * Modes (nz x nz, column-major, Modes(:,k) = mode k), EVals (nz), uBar (nz),
+ * qyBar (nz) are the module variables of QGN_Module.f90:22-26. */
+int qgn_create(qgn_ctx** ctx, const qgn_params* p, const double* H, const double* Modes,
+               const double* EVals, const double* uBar, const double* qyBar);
+int qgn_destroy(qgn_ctx* ctx);
+
+/* Initialize's transform of the initial condition (QGN_Module.f90:177-183):
+ * forward r2c of qp real(nx,ny,nz), DeAlias, store as the device state.
+ * q_hat (may be NULL) receives the dealiased spectrum. */
+int qgn_initialize(qgn_ctx* ctx, const double* qp, double* q_hat);
+
+/* Set / read the prognostic spectrum q_hat complex(nx/2+1,ny,nz).
+ * qgn_set_state applies DeAlias (the state is always dealiased). */
+int qgn_set_state(qgn_ctx* ctx, const double* q_hat);
+int qgn_get_state(qgn_ctx* ctx, double* q_hat);
+
+/* TimeStep (QGN_Module.f90:302-309) with the scheme chosen at create:
+ * advances the device state by one accepted step, updates t and dt. */
+int qgn_time_step(qgn_ctx* ctx, double* t, double* dt);
+
+/* GetRHS (QGN_Module.f90:258-296): rhs = RHS(q_hat), both
+ * complex(nx/2+1,ny,nz) on the host.  The input is dealiased first, the
+ * precondition the reference states for Jacobian (QGN_Module.f90:213). */
+int qgn_rhs(qgn_ctx* ctx, const double* q_hat, double* rhs);
+
+/* GetPsi (QGN_Module.f90:315-341) on the host: psi_hat from q_hat. */
+int qgn_get_psi(qgn_ctx* ctx, const double* q_hat, double* psi_hat);
+
+/* WriteQP's fields (QGN_Module.f90:348-361) of the current state:
+ * q_phys and psi_phys real(nx,ny,nz) = c2r(.)/(nx*ny).  Either may be NULL. */
+int qgn_fields(qgn_ctx* ctx, double* q_phys, double* psi_phys);
+
+/* QG-Leith coefficients A8L(nz) of the current state
+ * (ARK43_QGLeith.f90:62-73). */
+int qgn_leith(qgn_ctx* ctx, double* a8l);
+
+typedef struct qgn_stats {
+  long long steps;      /* accepted steps */
+  long long attempts;   /* accepted + rejected */
+  long long rejects;
+  long long rhs_evals;
+  double last_err;      /* err1 of the last attempt */
+  double err0;          /* controller memory */
+} qgn_stats;
+int qgn_get_stats(qgn_ctx* ctx, qgn_stats* s);
+
+/* The HIP stream (hipStream_t) every kernel of the context runs on. */
+int qgn_get_stream(qgn_ctx* ctx, void** stream);
+
+/* Kernel timing with HIP events on the context stream: after
+ * qgn_prof_enable(ctx, id), every launch of kernel `id` is bracketed by
+ * events; qgn_prof_read returns the launch count and summed milliseconds
+ * since enable (it synchronises the stream).  id: QGN_K_* below. */
+enum { QGN_K_ROW_JAC = 0, QGN_K_COL_INV = 1, QGN_K_COL_RHS = 2, QGN_K_PROJECT = 3,
+       QGN_K_STAGE = 4, QGN_K_UPDATE = 5, QGN_K_ERRNORM = 6, QGN_K_COUNT = 7 };
+int qgn_prof_enable(qgn_ctx* ctx, int kernel_id);
+int qgn_prof_read(qgn_ctx* ctx, int* count, double* total_ms);
+
+/* ---- MKL DFTI replacement (MKL_DFTI_Setup.f90:1-52) --------------------
+ * Batched 2-D real transforms with the reference's descriptor:
+ * DFTI_REAL, lengths [nx,ny], NUMBER_OF_TRANSFORMS = howmany,
+ * CCE half spectrum along x, out of place, unnormalised both ways,
+ * input distance nx*ny / output distance (nx/2+1)*ny.
+ * Backward transforms use only the real part of the kx=0 and kx=nx/2 bins
+ * after the y inverse, as DFTI does for non-Hermitian input. */
+typedef struct qgn_dft qgn_dft;
+int qgn_dft_create(qgn_dft** h, int precision, int nx, int ny, int howmany);
+/* DftiComputeForward(g2s_q, gridE, specE), host arrays */
+int qgn_dft_forward(qgn_dft* h, const double* grid, double* spec);
+/* DftiComputeBackward(s2g_q, specE, gridE), host arrays */
+int qgn_dft_backward(qgn_dft* h, const double* spec, double* grid);
+/* device-pointer variants on a caller stream (NULL = the handle's stream);
+ * precision QGN_MIXED handles take float data */
+int qgn_dft_forward_device(qgn_dft* h, const void* grid, void* spec, void* stream);
+int qgn_dft_backward_device(qgn_dft* h, const void* spec, void* grid, void* stream);
+int qgn_dft_free(qgn_dft* h);
+
+/* ---- LAPACK DGEEV replacement for the vertical structure ---------------
+ * QGN_Module.f90:84-148: H <- Htot*H/sum(H); tridiagonal stretching matrix
+ * L from S(0..nz) (only S(1..nz-1) used); eigenpairs of L (real, since L is
+ * similar to a symmetric matrix), each mode normalised so that
+ * sum(H*Modes(:,k)**2)/Htot = 1 with Modes(1,k) > 0; qyBar = -L uBar.
+ * Modes are returned in descending eigenvalue order (barotropic first);
+ * DGEEV's order is unspecified and only changes WriteQPmodal's mode order. */
+int qgn_vertical_modes(int nz, double Htot, const double* H_in, const double* S, const double* uBar,
+                       double* H_out, double* L, double* Modes, double* EVals, double* qyBar);
+
+const char* qgn_last_error(void);
+const char* qgn_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QGN_H */

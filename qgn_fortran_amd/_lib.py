@@ -1,0 +1,109 @@
+"""ctypes binding of libqgn.so (declarations in include/qgn.h).
+
+Loading fails loudly: there is no CPU fallback for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libqgn.so")
+
+QGN_OK, QGN_EINVAL, QGN_EHIP, QGN_ENOMEM, QGN_ENONFINITE, QGN_EREJECT = 0, -1, -2, -3, -4, -5
+SCHEMES = {"AB3": 0, "RK4": 1, "BSRK32": 2, "ARK43": 3, "ARK43_QGLeith": 4}
+PRECISIONS = {"fp64": 0, "mixed": 1}
+KERNELS = {"row_jac": 0, "col_inv": 1, "col_rhs": 2, "project": 3, "stage": 4, "update": 5, "errnorm": 6}
+
+# Every symbol include/qgn.h declares.
+EXPORTS = [
+    "qgn_create", "qgn_destroy", "qgn_initialize", "qgn_set_state", "qgn_get_state", "qgn_time_step",
+    "qgn_rhs", "qgn_get_psi", "qgn_fields", "qgn_leith", "qgn_get_stats", "qgn_get_stream",
+    "qgn_prof_enable", "qgn_prof_read", "qgn_dft_create", "qgn_dft_forward", "qgn_dft_backward",
+    "qgn_dft_forward_device", "qgn_dft_backward_device", "qgn_dft_free", "qgn_vertical_modes",
+    "qgn_last_error", "qgn_version",
+]
+
+
+class QgnParams(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("Lx", C.c_double), ("Ly", C.c_double), ("Htot", C.c_double),
+        ("beta", C.c_double), ("A2", C.c_double), ("A8", C.c_double),
+        ("r0", C.c_double), ("C_d", C.c_double), ("TOL", C.c_double), ("dt_max", C.c_double),
+        ("is_explicit", C.c_int), ("scheme", C.c_int), ("precision", C.c_int), ("max_rejects", C.c_int),
+    ]
+
+
+class QgnStats(C.Structure):
+    _fields_ = [
+        ("steps", C.c_longlong), ("attempts", C.c_longlong), ("rejects", C.c_longlong),
+        ("rhs_evals", C.c_longlong), ("last_err", C.c_double), ("err0", C.c_double),
+    ]
+
+
+class QgnError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"qgn error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load():
+    """Load libqgn.so from the package directory (raises if it is missing)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m qgn_fortran_amd.build`")
+    lib = C.CDLL(LIB_PATH)
+    P, D, I = C.c_void_p, C.POINTER(C.c_double), C.c_int
+    sig = {
+        "qgn_create": (I, [C.POINTER(P), C.POINTER(QgnParams), D, D, D, D, D]),
+        "qgn_destroy": (I, [P]),
+        "qgn_initialize": (I, [P, D, D]),
+        "qgn_set_state": (I, [P, D]),
+        "qgn_get_state": (I, [P, D]),
+        "qgn_time_step": (I, [P, D, D]),
+        "qgn_rhs": (I, [P, D, D]),
+        "qgn_get_psi": (I, [P, D, D]),
+        "qgn_fields": (I, [P, D, D]),
+        "qgn_leith": (I, [P, D]),
+        "qgn_get_stats": (I, [P, C.POINTER(QgnStats)]),
+        "qgn_get_stream": (I, [P, C.POINTER(P)]),
+        "qgn_prof_enable": (I, [P, I]),
+        "qgn_prof_read": (I, [P, C.POINTER(I), D]),
+        "qgn_dft_create": (I, [C.POINTER(P), I, I, I, I]),
+        "qgn_dft_forward": (I, [P, D, D]),
+        "qgn_dft_backward": (I, [P, D, D]),
+        "qgn_dft_forward_device": (I, [P, P, P, P]),
+        "qgn_dft_backward_device": (I, [P, P, P, P]),
+        "qgn_dft_free": (I, [P]),
+        "qgn_vertical_modes": (I, [I, C.c_double, D, D, D, D, D, D, D, D]),
+        "qgn_last_error": (C.c_char_p, []),
+        "qgn_version": (C.c_char_p, []),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(code):
+    if code != 0:
+        msg = _lib.qgn_last_error().decode() if _lib is not None else ""
+        raise QgnError(code, msg)
+    return code
+
+
+def dptr(a):
+    """double* view of a C-contiguous float64/complex128 numpy array."""
+    if a is None:
+        return None
+    if not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("array must be C-contiguous")
+    return a.ctypes.data_as(C.POINTER(C.c_double))

@@ -1,0 +1,207 @@
+// Batched complex FFTs staged in LDS for gfx950 (64-wide wavefronts).
+//
+// Several sequences of one length n live in one LDS buffer; a workgroup
+// transforms all of them together with mixed-radix Stockham passes
+// (radices 2, 3, 4, 5, 8).  Each pass is register-staged: every thread
+// loads the R inputs of up to MAXB butterflies, the workgroup barriers,
+// each thread writes its R outputs to their self-sorting positions, and the
+// workgroup barriers again, so a single LDS buffer suffices.
+//
+// Transform definition: X[m] = sum_j x[j] exp(sign * 2*pi*i * j*m / n), no
+// scaling, which is what MKL DFTI computes for DftiComputeForward
+// (sign = -1) and DftiComputeBackward (sign = +1) along each axis
+// (MKL_DFTI_Setup.f90:1-52).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace qgn {
+
+template <typename R> struct cplx_t;
+template <> struct cplx_t<double> { using type = double2; };
+template <> struct cplx_t<float> { using type = float2; };
+template <typename R> using cplx = typename cplx_t<R>::type;
+
+template <typename C> __device__ __forceinline__ C cmk(decltype(C::x) a, decltype(C::x) b) {
+  C r; r.x = a; r.y = b; return r;
+}
+template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return cmk<C>(a.x + b.x, a.y + b.y); }
+template <typename C> __device__ __forceinline__ C csub(C a, C b) { return cmk<C>(a.x - b.x, a.y - b.y); }
+template <typename C> __device__ __forceinline__ C cmul(C a, C b) {
+  return cmk<C>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename C> __device__ __forceinline__ C cconj(C a) { return cmk<C>(a.x, -a.y); }
+template <typename C> __device__ __forceinline__ C cscale(C a, decltype(C::x) s) { return cmk<C>(a.x * s, a.y * s); }
+// multiply by sign*i  (sign = +1 or -1)
+template <typename C> __device__ __forceinline__ C cmul_si(C a, int sign) {
+  return sign > 0 ? cmk<C>(-a.y, a.x) : cmk<C>(a.y, -a.x);
+}
+
+// ---- small DFT kernels on registers (in place), X[m] = sum_j a[j] w^(j m),
+//      w = exp(sign*2*pi*i/R) ------------------------------------------------
+template <typename C, int RAD> struct Dft;
+
+template <typename C> struct Dft<C, 2> {
+  static __device__ __forceinline__ void run(C* a, int) {
+    C t = a[0];
+    a[0] = cadd(t, a[1]);
+    a[1] = csub(t, a[1]);
+  }
+};
+
+template <typename C> struct Dft<C, 4> {
+  static __device__ __forceinline__ void run(C* a, int s) {
+    C t0 = cadd(a[0], a[2]), t1 = csub(a[0], a[2]);
+    C t2 = cadd(a[1], a[3]), t3 = cmul_si(csub(a[1], a[3]), s);
+    a[0] = cadd(t0, t2);
+    a[2] = csub(t0, t2);
+    a[1] = cadd(t1, t3);
+    a[3] = csub(t1, t3);
+  }
+};
+
+template <typename C> struct Dft<C, 8> {
+  static __device__ __forceinline__ void run(C* a, int s) {
+    using R = decltype(C::x);
+    const R h = (R)0.70710678118654752440084436210484903928;
+    C e[4] = {a[0], a[2], a[4], a[6]};
+    C o[4] = {a[1], a[3], a[5], a[7]};
+    Dft<C, 4>::run(e, s);
+    Dft<C, 4>::run(o, s);
+    // o[m] *= w8^m, w8 = exp(s*i*pi/4) = (1 + s i)/sqrt(2)
+    C o1 = cmk<C>(h * (o[1].x - s * o[1].y), h * (o[1].y + s * o[1].x));
+    C o2 = cmul_si(o[2], s);
+    C o3 = cmk<C>(h * (-o[3].x - s * o[3].y), h * (-o[3].y + s * o[3].x));
+    a[0] = cadd(e[0], o[0]);
+    a[4] = csub(e[0], o[0]);
+    a[1] = cadd(e[1], o1);
+    a[5] = csub(e[1], o1);
+    a[2] = cadd(e[2], o2);
+    a[6] = csub(e[2], o2);
+    a[3] = cadd(e[3], o3);
+    a[7] = csub(e[3], o3);
+  }
+};
+
+template <typename C> struct Dft<C, 3> {
+  static __device__ __forceinline__ void run(C* a, int s) {
+    using R = decltype(C::x);
+    const R c3 = (R)0.86602540378443864676372317075293618347;  // sqrt(3)/2
+    C t = cadd(a[1], a[2]);
+    C d = csub(a[1], a[2]);
+    C m = cmk<C>(a[0].x - (R)0.5 * t.x, a[0].y - (R)0.5 * t.y);
+    C r = cmul_si(cscale(d, c3), s);
+    a[0] = cadd(a[0], t);
+    a[1] = cadd(m, r);
+    a[2] = csub(m, r);
+  }
+};
+
+template <typename C> struct Dft<C, 5> {
+  static __device__ __forceinline__ void run(C* a, int s) {
+    using R = decltype(C::x);
+    const R c1 = (R)0.30901699437494742410229341718281905886;   // cos(2pi/5)
+    const R c2 = (R)-0.80901699437494742410229341718281905886;  // cos(4pi/5)
+    const R s1 = (R)0.95105651629515357211643933337938214340;   // sin(2pi/5)
+    const R s2 = (R)0.58778525229247312916870595463907276860;   // sin(4pi/5)
+    C t1 = cadd(a[1], a[4]), t2 = cadd(a[2], a[3]);
+    C d1 = csub(a[1], a[4]), d2 = csub(a[2], a[3]);
+    C m1 = cmk<C>(a[0].x + c1 * t1.x + c2 * t2.x, a[0].y + c1 * t1.y + c2 * t2.y);
+    C m2 = cmk<C>(a[0].x + c2 * t1.x + c1 * t2.x, a[0].y + c2 * t1.y + c1 * t2.y);
+    C r1 = cmul_si(cmk<C>(s1 * d1.x + s2 * d2.x, s1 * d1.y + s2 * d2.y), s);
+    C r2 = cmul_si(cmk<C>(s2 * d1.x - s1 * d2.x, s2 * d1.y - s1 * d2.y), s);
+    a[0] = cadd(a[0], cadd(t1, t2));
+    a[1] = cadd(m1, r1);
+    a[4] = csub(m1, r1);
+    a[2] = cadd(m2, r2);
+    a[3] = csub(m2, r2);
+  }
+};
+
+// ---- FFT plan: list of radix passes for one length --------------------------
+constexpr int kMaxPasses = 16;
+struct FftPlan {
+  int n;
+  int npass;
+  int radix[kMaxPasses];
+  int p[kMaxPasses];        // product of the radices of the earlier passes
+  int twstride[kMaxPasses]; // n / (p * radix): table stride of the pass twiddle
+};
+
+// Layout of a batch of sequences in LDS: element t of sequence s sits at
+// buf[s * sstride + t * estride].  seq_fast selects how consecutive threads
+// are spread: over sequences first (column tiles, where sequences are adjacent
+// in memory) or over elements first (rows).
+struct LdsBatch {
+  int nseq;
+  int sstride;
+  int estride;
+  bool seq_fast;
+};
+
+// One radix-RAD Stockham pass over every sequence of the batch.
+// tw: table of exp(-2*pi*i*m/n), m in [0, n).  EPT: elements per thread bound.
+template <typename Rt, int RAD, int EPT>
+__device__ __forceinline__ void stockham_pass(cplx<Rt>* buf, const LdsBatch& B, int n, int p, int twstride,
+                                              const cplx<Rt>* __restrict__ tw, int sign) {
+  using C = cplx<Rt>;
+  constexpr int MAXB = (EPT + RAD - 1) / RAD;
+  const int nb = n / RAD;
+  const int G = nb * B.nseq;
+  const int nthr = blockDim.x;
+  const int tid = threadIdx.x;
+  C v[MAXB][RAD];
+  int obase[MAXB];
+#pragma unroll
+  for (int b = 0; b < MAXB; ++b) {
+    const int g = tid + b * nthr;
+    obase[b] = -1;
+    if (g < G) {
+      int s, t;
+      if (B.seq_fast) { s = g % B.nseq; t = g / B.nseq; }
+      else { t = g % nb; s = g / nb; }
+      const int k = t % p;
+      C* base = buf + s * B.sstride;
+#pragma unroll
+      for (int j = 0; j < RAD; ++j) v[b][j] = base[(t + j * nb) * B.estride];
+      if (p > 1) {
+#pragma unroll
+        for (int j = 1; j < RAD; ++j) {
+          C w = tw[j * k * twstride];
+          if (sign > 0) w.y = -w.y;
+          v[b][j] = cmul(v[b][j], w);
+        }
+      }
+      Dft<C, RAD>::run(v[b], sign);
+      obase[b] = s * B.sstride + ((t - k) * RAD + k) * B.estride;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < MAXB; ++b) {
+    if (obase[b] >= 0) {
+#pragma unroll
+      for (int m = 0; m < RAD; ++m) buf[obase[b] + m * p * B.estride] = v[b][m];
+    }
+  }
+  __syncthreads();
+}
+
+// Full transform of every sequence in the batch.  Caller must have
+// synchronised after filling buf; on return buf holds the result and the
+// workgroup is synchronised.
+template <typename Rt, int EPT>
+__device__ __forceinline__ void lds_fft(cplx<Rt>* buf, const LdsBatch& B, const FftPlan& P,
+                                        const cplx<Rt>* __restrict__ tw, int sign) {
+  for (int ps = 0; ps < P.npass; ++ps) {
+    const int r = P.radix[ps], p = P.p[ps], ts = P.twstride[ps];
+    switch (r) {
+      case 8: stockham_pass<Rt, 8, EPT>(buf, B, P.n, p, ts, tw, sign); break;
+      case 4: stockham_pass<Rt, 4, EPT>(buf, B, P.n, p, ts, tw, sign); break;
+      case 2: stockham_pass<Rt, 2, EPT>(buf, B, P.n, p, ts, tw, sign); break;
+      case 3: stockham_pass<Rt, 3, EPT>(buf, B, P.n, p, ts, tw, sign); break;
+      default: stockham_pass<Rt, 5, EPT>(buf, B, P.n, p, ts, tw, sign); break;
+    }
+  }
+}
+
+}  // namespace qgn

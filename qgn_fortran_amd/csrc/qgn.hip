@@ -1,0 +1,1104 @@
+// Host side of the QGN HIP library: contexts, FFT plans, RHS pipeline,
+// time steppers and the C-ABI declared in include/qgn.h.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <type_traits>
+#include <utility>
+#include <vector>
+
+#include "kernels.hpp"
+#include "qgn.h"
+
+using namespace qgn;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                        \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return fail(QGN_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));             \
+  } while (0)
+
+#define CHK(expr)            \
+  do {                       \
+    int r_ = (expr);         \
+    if (r_ != 0) return r_;  \
+  } while (0)
+
+constexpr int kEPT = 16;        // FFT elements per thread bound (template)
+constexpr int kColThreads = 256;
+constexpr long double kPi = 3.141592653589793238462643383279502884L;
+
+// ---------------------------------------------------------------------------
+// FFT plans and twiddle tables
+// ---------------------------------------------------------------------------
+bool make_plan(int n, FftPlan* P) {
+  std::vector<int> rad;
+  int m = n, e2 = 0;
+  while (m % 2 == 0) { m /= 2; ++e2; }
+  int n8 = e2 / 3, rem = e2 % 3;
+  std::vector<int> twos;
+  for (int i = 0; i < n8; ++i) twos.push_back(8);
+  if (rem == 1) {
+    if (n8 > 0) { twos.pop_back(); twos.push_back(4); twos.push_back(4); }
+    else twos.push_back(2);
+  } else if (rem == 2) {
+    twos.push_back(4);
+  }
+  while (m % 3 == 0) { rad.push_back(3); m /= 3; }
+  while (m % 5 == 0) { rad.push_back(5); m /= 5; }
+  if (m != 1) return false;
+  rad.insert(rad.end(), twos.begin(), twos.end());
+  if ((int)rad.size() > kMaxPasses) return false;
+  P->n = n;
+  P->npass = (int)rad.size();
+  int p = 1;
+  for (int i = 0; i < P->npass; ++i) {
+    P->radix[i] = rad[i];
+    P->p[i] = p;
+    P->twstride[i] = n / (p * rad[i]);
+    p *= rad[i];
+  }
+  if (n == 1) { P->npass = 0; }
+  return true;
+}
+
+// exp(-2 pi i m / n), m in [0, count)
+std::vector<double> twiddle_table(int n, int count) {
+  std::vector<double> t(2 * (size_t)count);
+  for (int m = 0; m < count; ++m) {
+    long double a = -2.0L * kPi * (long double)m / (long double)n;
+    t[2 * m] = (double)cosl(a);
+    t[2 * m + 1] = (double)sinl(a);
+  }
+  return t;
+}
+
+template <typename R> struct Tw {
+  cplx<R>* y = nullptr;  // length ny
+  cplx<R>* x = nullptr;  // length nx/2
+  cplx<R>* r = nullptr;  // length nx/2+1, exp(-2 pi i k / nx)
+};
+
+template <typename R>
+int upload_tw(const std::vector<double>& src, cplx<R>** dst) {
+  size_t n = src.size() / 2;
+  std::vector<R> h(src.size());
+  for (size_t i = 0; i < src.size(); ++i) h[i] = (R)src[i];
+  HIPCHK(hipMalloc((void**)dst, n * sizeof(cplx<R>)));
+  HIPCHK(hipMemcpy(*dst, h.data(), n * sizeof(cplx<R>), hipMemcpyHostToDevice));
+  return 0;
+}
+
+template <typename R>
+int make_tw(int nx, int ny, Tw<R>* tw) {
+  CHK(upload_tw<R>(twiddle_table(ny, ny), &tw->y));
+  CHK(upload_tw<R>(twiddle_table(nx / 2, nx / 2), &tw->x));
+  CHK(upload_tw<R>(twiddle_table(nx, nx / 2 + 1), &tw->r));
+  return 0;
+}
+
+template <typename R>
+void free_tw(Tw<R>* tw) {
+  if (tw->y) (void)hipFree(tw->y);
+  if (tw->x) (void)hipFree(tw->x);
+  if (tw->r) (void)hipFree(tw->r);
+  tw->y = tw->x = tw->r = nullptr;
+}
+
+int col_tile(int ny) {
+  int c = 1;
+  while (c < 16 && 2 * c * ny <= kEPT * kColThreads) c *= 2;
+  return c;
+}
+
+int row_threads(int nseq, int N2) {
+  int t = (nseq * N2 + kEPT - 1) / kEPT;
+  t = ((t + 63) / 64) * 64;
+  if (t < 64) t = 64;
+  return t;
+}
+
+Geom make_geom(int nx, int ny, int nz, double Lx, double Ly) {
+  Geom g;
+  g.nx = nx; g.ny = ny; g.nz = nz;
+  g.nxh = nx / 2 + 1;
+  g.N2 = nx / 2;
+  g.Kx = nx / 3;
+  g.ya = ny / 3;
+  g.yb = 2 * (ny / 3) + 1;
+  if (g.yb > ny) g.yb = ny;
+  g.Ky = g.ya + (ny - g.yb);
+  g.dkx = (2.0 * 3.14159265358979323846 / Lx);
+  g.dky = (2.0 * 3.14159265358979323846 / Ly);
+  return g;
+}
+
+bool size_ok(int nx, int ny, std::string* why) {
+  FftPlan P;
+  if (nx < 6 || ny < 6 || nx % 2) { *why = "nx must be even and nx, ny >= 6"; return false; }
+  if (!make_plan(nx / 2, &P) || !make_plan(ny, &P)) { *why = "nx/2 and ny must factor into 2, 3, 5"; return false; }
+  if (nx > 2048 || ny > 4096) { *why = "nx must be <= 2048 and ny <= 4096"; return false; }
+  return true;
+}
+
+// ARK4(3)6L[2]SA coefficients, ARK43.f90:17-57 (each a._dp/b._dp in double)
+struct ArkTab {
+  double ae[7][7] = {}, ai[7][7] = {}, b[7] = {}, be[7] = {};
+  ArkTab() {
+    ae[2][1] = 0.5;
+    ae[3][1] = 13861.0 / 62500.0;
+    ae[3][2] = 6889.0 / 62500.0;
+    ae[4][1] = -116923316275.0 / 2393684061468.0;
+    ae[4][2] = -2731218467317.0 / 15368042101831.0;
+    ae[4][3] = 9408046702089.0 / 11113171139209.0;
+    ae[5][1] = -451086348788.0 / 2902428689909.0;
+    ae[5][2] = -2682348792572.0 / 7519795681897.0;
+    ae[5][3] = 12662868775082.0 / 11960479115383.0;
+    ae[5][4] = 3355817975965.0 / 11060851509271.0;
+    ae[6][1] = 647845179188.0 / 3216320057751.0;
+    ae[6][2] = 73281519250.0 / 8382639484533.0;
+    ae[6][3] = 552539513391.0 / 3454668386233.0;
+    ae[6][4] = 3354512671639.0 / 8306763924573.0;
+    ae[6][5] = 4040.0 / 17871.0;
+    b[1] = 82889.0 / 524892.0;
+    b[2] = 0.0;
+    b[3] = 15625.0 / 83664.0;
+    b[4] = 69875.0 / 102672.0;
+    b[5] = -2260.0 / 8211.0;
+    b[6] = 0.25;
+    ai[2][1] = 0.25;
+    ai[3][1] = 8611.0 / 62500.0;
+    ai[3][2] = -1743.0 / 31250.0;
+    ai[4][1] = 5012029.0 / 34652500.0;
+    ai[4][2] = -654441.0 / 2922500.0;
+    ai[4][3] = 174375.0 / 388108.0;
+    ai[5][1] = 15267082809.0 / 155376265600.0;
+    ai[5][2] = -71443401.0 / 120774400.0;
+    ai[5][3] = 730878875.0 / 902184768.0;
+    ai[5][4] = 2285395.0 / 8070912.0;
+    for (int j = 1; j <= 6; ++j) ai[6][j] = b[j];
+    be[1] = 31666707.0 / 9881966720.0;
+    be[2] = 0.0;
+    be[3] = -256875.0 / 105007616.0;
+    be[4] = -2768025.0 / 128864768.0;
+    be[5] = 169839.0 / 3864644.0;
+    be[6] = -5247.0 / 225920.0;
+  }
+};
+const ArkTab kArk;
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Context
+// ---------------------------------------------------------------------------
+struct qgn_ctx {
+  qgn_params p;
+  Geom g;
+  hipStream_t stream = nullptr;
+  FftPlan py, px;
+  Tw<double> twd;
+  Tw<float> twf;
+  int Ccol = 4;          // column tile width for the RHS type
+  int Ccol64 = 4;        // column tile width for FP64 passes
+  size_t rsize = 16;     // sizeof(cplx<R>) of the RHS pipeline
+  // constants
+  double *HM = nullptr, *MD = nullptr, *evals = nullptr, *ubar = nullptr, *qybar = nullptr;
+  double *a8l = nullptr, *partial = nullptr;
+  int leith_nblk = 64;
+  double c12 = 0.0;
+  double Hnz = 1.0;
+  // compact arrays
+  long ncmp = 0;         // complex elements of a compact array
+  long nmix = 0;         // complex elements of a mixed (nz, ny, Kx) array
+  double2 *q = nullptr, *qn = nullptr, *E = nullptr;
+  double2* psi64 = nullptr;
+  void* psi = nullptr;   // cplx<R>
+  void* N[6] = {};       // cplx<R>
+  double2* Y[6] = {};    // Y[0] unused
+  void* hist[3] = {};    // AB3 / BSRK3 saved RHS, cplx<R>
+  // mixed arrays, cplx<R>
+  void *Qm = nullptr, *Um = nullptr, *Vm = nullptr, *UQm = nullptr, *VQm = nullptr, *DUm = nullptr, *DVm = nullptr;
+  double2* Em = nullptr; // FP64 mixed for the error norm
+  unsigned long long* errslot = nullptr;
+  unsigned long long* h_errslot = nullptr;  // pinned
+  // full-layout scratch for the boundary
+  double2 *fa = nullptr, *fb = nullptr;
+  double* phys = nullptr;
+  // stepper state (the reference's `save` variables)
+  bool first = true, second = false;
+  double err0 = 0.0, last_err = 0.0;
+  qgn_stats st = {};
+  // profiling
+  int prof_id = -1;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+  int ev_used = 0;
+  double prof_ms = 0.0;
+  int prof_count = 0;
+};
+
+namespace {
+
+template <typename F>
+int timed(qgn_ctx* c, int id, F&& launch) {
+  bool on = (c->prof_id == id);
+  hipEvent_t a = nullptr, b = nullptr;
+  if (on) {
+    if (c->ev_used == (int)c->ev.size()) {
+      if (c->ev.size() >= 4096) {
+        // fold what has been recorded so far
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (int i = 0; i < c->ev_used; ++i) {
+          float ms = 0.f;
+          HIPCHK(hipEventElapsedTime(&ms, c->ev[i].first, c->ev[i].second));
+          c->prof_ms += ms;
+          c->prof_count++;
+        }
+        c->ev_used = 0;
+      } else {
+        hipEvent_t x, y;
+        HIPCHK(hipEventCreate(&x));
+        HIPCHK(hipEventCreate(&y));
+        c->ev.push_back({x, y});
+      }
+    }
+    a = c->ev[c->ev_used].first;
+    b = c->ev[c->ev_used].second;
+    HIPCHK(hipEventRecord(a, c->stream));
+  }
+  launch();
+  HIPCHK(hipGetLastError());
+  if (on) {
+    HIPCHK(hipEventRecord(b, c->stream));
+    c->ev_used++;
+  }
+  return 0;
+}
+
+int grid1d(long n, int threads = 256) {
+  long b = (n + threads - 1) / threads;
+  if (b > 65536) b = 65536;
+  if (b < 1) b = 1;
+  return (int)b;
+}
+
+template <typename R> const Tw<R>& tw_of(qgn_ctx* c);
+template <> const Tw<double>& tw_of<double>(qgn_ctx* c) { return c->twd; }
+template <> const Tw<float>& tw_of<float>(qgn_ctx* c) { return c->twf; }
+
+// psi = P(q): GetPsi on compact arrays
+template <typename R>
+int launch_project(qgn_ctx* c, const double2* q, void* psi) {
+  const int nz = c->g.nz;
+  int pp = 256 / nz;
+  if (pp < 1) pp = 1;
+  const int threads = ((pp * nz + 63) / 64) * 64;
+  const long plane = (long)c->g.Ky * c->g.Kx;
+  const int blocks = (int)((plane + pp - 1) / pp);
+  const size_t lds = 2 * (size_t)nz * nz * sizeof(double) + 2 * (size_t)nz * pp * sizeof(double2);
+  return timed(c, QGN_K_PROJECT, [&] {
+    hipLaunchKernelGGL((k_project<R>), dim3(blocks), dim3(threads), lds, c->stream, q,
+                       reinterpret_cast<cplx<R>*>(psi), c->HM, c->MD, c->evals, 1.0 / c->p.Htot, pp, c->g);
+  });
+}
+
+// y-inverse FFT of up to 3 compact fields into mixed arrays
+template <typename R>
+int launch_col_inv(qgn_ctx* c, const ColInvArgs& a) {
+  const int C = col_tile(c->g.ny);
+  dim3 grid((c->g.Kx + C - 1) / C, c->g.nz);
+  const size_t lds = (size_t)C * c->g.ny * sizeof(cplx<R>);
+  const Tw<R>& tw = tw_of<R>(c);
+  return timed(c, QGN_K_COL_INV, [&] {
+    hipLaunchKernelGGL((k_col_inv<R, kEPT>), grid, dim3(kColThreads), lds, c->stream, a, c->g, c->py, tw.y, C);
+  });
+}
+
+template <typename R>
+int rhs_impl(qgn_ctx* c, const double2* qin, void* out) {
+  const Geom& g = c->g;
+  const Tw<R>& tw = tw_of<R>(c);
+  void* psi = std::is_same<R, double>::value ? (void*)c->psi64 : c->psi;
+  CHK(launch_project<R>(c, qin, psi));
+  ColInvArgs ca{};
+  ca.nf = 3;
+  ca.src[0] = qin; ca.src_is_double[0] = 1; ca.mult[0] = 0; ca.dst[0] = c->Qm;
+  ca.src[1] = psi; ca.src_is_double[1] = std::is_same<R, double>::value; ca.mult[1] = 1; ca.dst[1] = c->Um;
+  ca.src[2] = psi; ca.src_is_double[2] = std::is_same<R, double>::value; ca.mult[2] = 2; ca.dst[2] = c->Vm;
+  CHK(launch_col_inv<R>(c, ca));
+  RowJacArgs ra{};
+  ra.Q = c->Qm; ra.U = c->Um; ra.V = c->Vm; ra.UQ = c->UQm; ra.VQ = c->VQm; ra.DU = c->DUm; ra.DV = c->DVm;
+  ra.drag = c->p.C_d != 0.0;
+  {
+    const int nseq = ra.drag ? 4 : 3;
+    const int threads = row_threads(4, g.N2);
+    const size_t lds = (size_t)nseq * g.N2 * sizeof(cplx<R>);
+    const R scale = (R)(1.0 / ((double)g.nx * (double)g.ny));
+    CHK(timed(c, QGN_K_ROW_JAC, [&] {
+      hipLaunchKernelGGL((k_row_jac<R, kEPT>), dim3(g.ny * g.nz), dim3(threads), lds, c->stream, ra, g, c->px,
+                         tw.x, tw.r, scale);
+    }));
+  }
+  ColRhsArgs ka{};
+  ka.UQ = c->UQm; ka.VQ = c->VQm; ka.DU = c->DUm; ka.DV = c->DVm;
+  ka.q = qin; ka.psi = psi; ka.out = out;
+  ka.ubar = c->ubar; ka.qybar = c->qybar;
+  ka.beta = c->p.beta;
+  ka.ekman = c->p.r0 * c->p.Htot / c->Hnz;
+  ka.cdrag = c->p.C_d * c->p.Htot / c->Hnz;
+  ka.A2 = c->p.A2; ka.A8 = c->p.A8;
+  ka.explicit_visc = c->p.is_explicit;
+  ka.drag = ra.drag;
+  {
+    const int C = col_tile(g.ny);
+    dim3 grid((g.Kx + C - 1) / C, g.nz);
+    const size_t lds = (size_t)C * g.ny * sizeof(cplx<R>);
+    CHK(timed(c, QGN_K_COL_RHS, [&] {
+      hipLaunchKernelGGL((k_col_rhs<R, R, kEPT>), grid, dim3(kColThreads), lds, c->stream, ka, g, c->py, tw.y, C);
+    }));
+  }
+  c->st.rhs_evals++;
+  return 0;
+}
+
+int rhs(qgn_ctx* c, const double2* qin, void* out) {
+  if (c->p.precision == QGN_MIXED) return rhs_impl<float>(c, qin, out);
+  return rhs_impl<double>(c, qin, out);
+}
+
+// max over the physical grid of |c2r(E)| (unnormalised), FP64
+int errnorm(qgn_ctx* c, const double2* Ecmp, double* maxv) {
+  const Geom& g = c->g;
+  ColInvArgs ca{};
+  ca.nf = 1;
+  ca.src[0] = Ecmp; ca.src_is_double[0] = 1; ca.mult[0] = 0; ca.dst[0] = c->Em;
+  CHK(launch_col_inv<double>(c, ca));
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, c->stream, c->errslot, 1ull * 0);
+  HIPCHK(hipGetLastError());
+  const int threads = row_threads(1, g.N2);
+  const size_t lds = (size_t)g.N2 * sizeof(double2);
+  CHK(timed(c, QGN_K_ERRNORM, [&] {
+    hipLaunchKernelGGL((k_row_maxabs<double, kEPT>), dim3(g.ny * g.nz), dim3(threads), lds, c->stream, c->Em,
+                       g.Kx, g, c->px, c->twd.x, c->twd.r, c->errslot);
+  }));
+  HIPCHK(hipMemcpyAsync(c->h_errslot, c->errslot, sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  unsigned long long bits = *c->h_errslot;
+  double v;
+  std::memcpy(&v, &bits, sizeof(v));
+  *maxv = v;
+  return 0;
+}
+
+int leith(qgn_ctx* c, const double2* q) {
+  const Geom& g = c->g;
+  hipLaunchKernelGGL(k_leith_partial, dim3(c->leith_nblk, g.nz), dim3(256), 0, c->stream, q, g, c->partial,
+                     c->leith_nblk);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_leith_final, dim3(g.nz), dim3(256), 0, c->stream, q, g, c->partial, c->leith_nblk, c->c12,
+                     c->a8l);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+template <typename S>
+int launch_stage(qgn_ctx* c, const StageArgs& a) {
+  const long n = c->ncmp;
+  return timed(c, QGN_K_STAGE, [&] {
+    hipLaunchKernelGGL((k_stage<S>), dim3(grid1d(n)), dim3(256), 0, c->stream, a, c->g);
+  });
+}
+int stage(qgn_ctx* c, const StageArgs& a) {
+  return c->p.precision == QGN_MIXED ? launch_stage<float>(c, a) : launch_stage<double>(c, a);
+}
+
+int update(qgn_ctx* c, const UpdateArgs& a) {
+  const long n = c->ncmp;
+  return timed(c, QGN_K_UPDATE, [&] {
+    if (c->p.precision == QGN_MIXED)
+      hipLaunchKernelGGL((k_update<float>), dim3(grid1d(n)), dim3(256), 0, c->stream, a, c->g);
+    else
+      hipLaunchKernelGGL((k_update<double>), dim3(grid1d(n)), dim3(256), 0, c->stream, a, c->g);
+  });
+}
+
+int axpy(qgn_ctx* c, const AxpyArgs& a) {
+  const long n = c->ncmp;
+  if (c->p.precision == QGN_MIXED)
+    hipLaunchKernelGGL((k_axpy<float>), dim3(grid1d(n)), dim3(256), 0, c->stream, a, c->g);
+  else
+    hipLaunchKernelGGL((k_axpy<double>), dim3(grid1d(n)), dim3(256), 0, c->stream, a, c->g);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int lincomb(qgn_ctx* c, const void* X0, const void* X1, const void* X2, const void* X3, double c0, double c1,
+            double c2, double c3, int nxs, void* out) {
+  const long n = c->ncmp;
+  if (c->p.precision == QGN_MIXED)
+    hipLaunchKernelGGL((k_lincomb<float>), dim3(grid1d(n)), dim3(256), 0, c->stream, X0, X1, X2, X3, c0, c1, c2,
+                       c3, nxs, out, c->g);
+  else
+    hipLaunchKernelGGL((k_lincomb<double>), dim3(grid1d(n)), dim3(256), 0, c->stream, X0, X1, X2, X3, c0, c1, c2,
+                       c3, nxs, out, c->g);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Steppers (the TimeStep include fragments)
+// ---------------------------------------------------------------------------
+int step_ark43(qgn_ctx* c, double* t, double* dt, bool qgleith) {
+  const qgn_params& p = c->p;
+  const Geom& g = c->g;
+  if (c->first) {              // ARK43*.f90 FirstCall
+    c->err0 = p.TOL;
+    c->first = false;
+  }
+  const double* a8l = nullptr;
+  if (qgleith) {
+    CHK(leith(c, c->q));
+    a8l = c->a8l;
+  }
+  CHK(rhs(c, c->q, c->N[0]));
+  const double reject_factor = qgleith ? 0.5 : 0.75;
+  const double safety = qgleith ? 0.2 : 0.75;
+  const int maxrej = p.max_rejects > 0 ? p.max_rejects : 1000;
+  int tries = 0;
+  while (true) {
+    for (int i = 2; i <= 6; ++i) {
+      StageArgs a{};
+      a.q = c->q;
+      a.ns = i - 1;
+      for (int j = 1; j < i; ++j) {
+        a.N[j - 1] = c->N[j - 1];
+        a.Y[j - 1] = c->Y[j - 1];
+        a.ae[j - 1] = kArk.ae[i][j];
+        a.ai[j - 1] = kArk.ai[i][j];
+      }
+      a.dt = *dt;
+      a.a8l = a8l;
+      a.A8 = p.A8;
+      a.A2 = p.A2;
+      a.implicit_visc = 1;
+      a.out = c->Y[i - 1];
+      CHK(stage(c, a));
+      CHK(rhs(c, c->Y[i - 1], c->N[i - 1]));
+    }
+    UpdateArgs u{};
+    u.q = c->q;
+    u.ns = 6;
+    for (int j = 1; j <= 6; ++j) {
+      u.N[j - 1] = c->N[j - 1];
+      u.Y[j - 1] = c->Y[j - 1];
+      u.b[j - 1] = kArk.b[j];
+      u.be[j - 1] = kArk.be[j];
+    }
+    u.dt = *dt;
+    u.a8l = a8l;
+    u.A8 = p.A8;
+    u.A2 = p.A2;
+    u.implicit_visc = 1;
+    u.E = c->E;
+    u.qn = c->qn;
+    CHK(update(c, u));
+    double mx;
+    CHK(errnorm(c, c->E, &mx));
+    const double err1 = *dt * mx / (double)((long)g.nx * g.ny);
+    c->last_err = err1;
+    c->st.attempts++;
+    if (!std::isfinite(err1)) return fail(QGN_ENONFINITE, "ARK43: non-finite error norm");
+    if (err1 > p.TOL) {        // reject
+      *dt = reject_factor * *dt;
+      c->st.rejects++;
+      if (++tries > maxrej) return fail(QGN_EREJECT, "ARK43: too many rejected steps");
+      continue;
+    }
+    std::swap(c->q, c->qn);
+    *t = *t + *dt;
+    *dt = std::fmin(*dt * std::pow(safety * p.TOL / err1, 0.075) * std::pow(c->err0 / err1, 0.1), p.dt_max);
+    c->err0 = err1;
+    c->st.steps++;
+    return 0;
+  }
+}
+
+int step_rk4(qgn_ctx* c, double* t, double* dt) {
+  const double h = *dt;
+  void* r[4] = {c->N[0], c->N[1], c->N[2], c->N[3]};
+  CHK(rhs(c, c->q, r[0]));
+  const double cs[3] = {0.5 * h, 0.5 * h, h};
+  for (int s = 0; s < 3; ++s) {
+    AxpyArgs a{};
+    a.q = c->q; a.X[0] = r[s]; a.c[0] = cs[s]; a.nx = 1; a.out = c->Y[1];
+    CHK(axpy(c, a));
+    CHK(rhs(c, c->Y[1], r[s + 1]));
+  }
+  AxpyArgs a{};
+  a.q = c->q;
+  a.X[0] = r[0]; a.X[1] = r[1]; a.X[2] = r[2]; a.X[3] = r[3];
+  a.c[0] = h / 6.0; a.c[1] = 2.0 * h / 6.0; a.c[2] = 2.0 * h / 6.0; a.c[3] = h / 6.0;
+  a.nx = 4;
+  a.out = c->qn;
+  CHK(axpy(c, a));
+  std::swap(c->q, c->qn);
+  *t += h;
+  c->st.steps++;
+  c->st.attempts++;
+  return 0;
+}
+
+int step_ab3(qgn_ctx* c, double* t, double* dt) {
+  const double h = *dt;
+  AxpyArgs a{};
+  a.q = c->q;
+  a.out = c->qn;
+  if (c->first) {
+    CHK(rhs(c, c->q, c->hist[0]));
+    a.X[0] = c->hist[0]; a.c[0] = h; a.nx = 1;
+    c->first = false;
+    c->second = true;
+  } else if (c->second) {
+    CHK(rhs(c, c->q, c->hist[1]));
+    a.X[0] = c->hist[1]; a.c[0] = 1.5 * h;
+    a.X[1] = c->hist[0]; a.c[1] = -0.5 * h;
+    a.nx = 2;
+    c->second = false;
+  } else {
+    CHK(rhs(c, c->q, c->hist[2]));
+    a.X[0] = c->hist[2]; a.c[0] = 23.0 * h / 12.0;
+    a.X[1] = c->hist[1]; a.c[1] = -16.0 * h / 12.0;
+    a.X[2] = c->hist[0]; a.c[2] = 5.0 * h / 12.0;
+    a.nx = 3;
+  }
+  CHK(axpy(c, a));
+  if (a.nx == 3) {  // RHS0 = RHS1; RHS1 = RHS2 (AB3.f90:17-18)
+    void* t0 = c->hist[0];
+    c->hist[0] = c->hist[1];
+    c->hist[1] = c->hist[2];
+    c->hist[2] = t0;
+  }
+  std::swap(c->q, c->qn);
+  *t += h;
+  c->st.steps++;
+  c->st.attempts++;
+  return 0;
+}
+
+int step_bsrk3(qgn_ctx* c, double* t, double* dt) {
+  const qgn_params& p = c->p;
+  const Geom& g = c->g;
+  void*& rhs0 = c->hist[0];
+  if (c->first) {
+    CHK(rhs(c, c->q, rhs0));
+    c->err0 = p.TOL;
+    c->first = false;
+  }
+  const int maxrej = p.max_rejects > 0 ? p.max_rejects : 1000;
+  int tries = 0;
+  while (true) {
+    const double h = *dt;
+    AxpyArgs a{};
+    a.q = c->q; a.X[0] = rhs0; a.c[0] = 0.5 * h; a.nx = 1; a.out = c->Y[1];
+    CHK(axpy(c, a));
+    CHK(rhs(c, c->Y[1], c->N[1]));
+    a.X[0] = c->N[1]; a.c[0] = 0.75 * h;
+    CHK(axpy(c, a));
+    CHK(rhs(c, c->Y[1], c->N[2]));
+    AxpyArgs b{};
+    b.q = c->q;
+    b.X[0] = rhs0; b.c[0] = 2.0 * h / 9.0;
+    b.X[1] = c->N[1]; b.c[1] = 1.0 * h / 3.0;
+    b.X[2] = c->N[2]; b.c[2] = 4.0 * h / 9.0;
+    b.nx = 3;
+    b.out = c->Y[2];
+    CHK(axpy(c, b));
+    CHK(rhs(c, c->Y[2], c->N[3]));
+    CHK(lincomb(c, rhs0, c->N[1], c->N[2], c->N[3], 5.0 / 72.0, -1.0 / 12.0, -1.0 / 9.0, 1.0 / 8.0, 4, c->E));
+    double mx;
+    CHK(errnorm(c, c->E, &mx));
+    const double r1 = h * (mx / (double)((long)g.nx * g.ny));
+    c->last_err = r1;
+    c->st.attempts++;
+    if (!std::isfinite(r1)) return fail(QGN_ENONFINITE, "BSRK32: non-finite error norm");
+    if (r1 > p.TOL) {
+      *dt = 0.75 * h;
+      c->st.rejects++;
+      if (++tries > maxrej) return fail(QGN_EREJECT, "BSRK32: too many rejected steps");
+      continue;
+    }
+    std::swap(c->q, c->Y[2]);
+    *t += h;
+    std::swap(rhs0, c->N[3]);   // FSAL
+    *dt = std::fmin(h * std::pow(0.75 * p.TOL / r1, 0.1) * std::pow(c->err0 / r1, 0.4 / 3.0), p.dt_max);
+    c->err0 = r1;
+    c->st.steps++;
+    return 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// General batched 2-D FFTs (DFTI layout)
+// ---------------------------------------------------------------------------
+template <typename R>
+int dft_fwd(hipStream_t s, const Geom& g, const FftPlan& px, const FftPlan& py, const Tw<R>& tw, int howmany,
+            const R* grid, cplx<R>* spec, cplx<R>* tmp) {
+  const int nrows = g.ny * howmany;
+  hipLaunchKernelGGL((k_row_r2c<R, kEPT>), dim3(nrows), dim3(row_threads(1, g.N2)),
+                     (size_t)g.N2 * sizeof(cplx<R>), s, grid, tmp, g.nxh, g, px, tw.x, tw.r);
+  HIPCHK(hipGetLastError());
+  const int C = col_tile(g.ny);
+  hipLaunchKernelGGL((k_col_fft<R, kEPT>), dim3((g.nxh + C - 1) / C, howmany), dim3(kColThreads),
+                     (size_t)C * g.ny * sizeof(cplx<R>), s, tmp, spec, g.nxh, -1, g, py, tw.y, C);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+template <typename R>
+int dft_bwd(hipStream_t s, const Geom& g, const FftPlan& px, const FftPlan& py, const Tw<R>& tw, int howmany,
+            const cplx<R>* spec, R* grid, cplx<R>* tmp, R scale) {
+  const int C = col_tile(g.ny);
+  hipLaunchKernelGGL((k_col_fft<R, kEPT>), dim3((g.nxh + C - 1) / C, howmany), dim3(kColThreads),
+                     (size_t)C * g.ny * sizeof(cplx<R>), s, spec, tmp, g.nxh, +1, g, py, tw.y, C);
+  HIPCHK(hipGetLastError());
+  const int nrows = g.ny * howmany;
+  hipLaunchKernelGGL((k_row_c2r<R, kEPT>), dim3(nrows), dim3(row_threads(1, g.N2)),
+                     (size_t)g.N2 * sizeof(cplx<R>), s, tmp, g.nxh, grid, scale, g, px, tw.x, tw.r);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int alloc(void** p, size_t bytes) {
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) return fail(QGN_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  return 0;
+}
+
+void destroy_ctx(qgn_ctx* c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  void* ptrs[] = {c->HM, c->MD, c->evals, c->ubar, c->qybar, c->a8l, c->partial, c->q, c->qn, c->E, c->psi64,
+                  c->psi, c->Qm, c->Um, c->Vm, c->UQm, c->VQm, c->DUm, c->DVm, c->Em, c->errslot, c->fa, c->fb,
+                  c->phys};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  for (int i = 0; i < 6; ++i) {
+    if (c->N[i]) (void)hipFree(c->N[i]);
+    if (c->Y[i]) (void)hipFree(c->Y[i]);
+  }
+  for (int i = 0; i < 3; ++i)
+    if (c->hist[i]) (void)hipFree(c->hist[i]);
+  if (c->h_errslot) (void)hipHostFree(c->h_errslot);
+  for (auto& e : c->ev) {
+    (void)hipEventDestroy(e.first);
+    (void)hipEventDestroy(e.second);
+  }
+  free_tw(&c->twd);
+  free_tw(&c->twf);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int upload(double** dst, const double* src, size_t n) {
+  CHK(alloc((void**)dst, n * sizeof(double)));
+  HIPCHK(hipMemcpy(*dst, src, n * sizeof(double), hipMemcpyHostToDevice));
+  return 0;
+}
+
+int create_impl(qgn_ctx* c, const qgn_params* p, const double* H, const double* Modes, const double* EVals,
+                const double* uBar, const double* qyBar) {
+  c->p = *p;
+  std::string why;
+  if (!size_ok(p->nx, p->ny, &why)) return fail(QGN_EINVAL, why);
+  if (p->nz < 1 || p->nz > 64) return fail(QGN_EINVAL, "nz must be in [1, 64]");
+  if (p->scheme < QGN_AB3 || p->scheme > QGN_ARK43_QGLEITH) return fail(QGN_EINVAL, "unknown scheme");
+  if (p->precision != QGN_FP64 && p->precision != QGN_MIXED) return fail(QGN_EINVAL, "unknown precision");
+  if (!(p->Lx > 0) || !(p->Ly > 0) || !(p->Htot > 0)) return fail(QGN_EINVAL, "Lx, Ly, Htot must be positive");
+  const int nz = p->nz;
+  c->g = make_geom(p->nx, p->ny, nz, p->Lx, p->Ly);
+  make_plan(p->ny, &c->py);
+  make_plan(p->nx / 2, &c->px);
+  HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  CHK(make_tw<double>(p->nx, p->ny, &c->twd));
+  CHK(make_tw<float>(p->nx, p->ny, &c->twf));
+  // constants
+  std::vector<double> hm((size_t)nz * nz), md((size_t)nz * nz);
+  for (int k = 0; k < nz; ++k)
+    for (int kk = 0; kk < nz; ++kk) {
+      hm[(size_t)k * nz + kk] = H[kk] * Modes[(size_t)k * nz + kk];  // H(kk)*Modes(kk,k), column-major
+      md[(size_t)k * nz + kk] = Modes[(size_t)kk * nz + k];          // Modes(k,kk)
+    }
+  CHK(upload(&c->HM, hm.data(), hm.size()));
+  CHK(upload(&c->MD, md.data(), md.size()));
+  CHK(upload(&c->evals, EVals, nz));
+  CHK(upload(&c->ubar, uBar, nz));
+  CHK(upload(&c->qybar, qyBar, nz));
+  c->Hnz = H[nz - 1];
+  CHK(alloc((void**)&c->a8l, nz * sizeof(double)));
+  {
+    std::vector<double> a8(nz, p->A8);
+    HIPCHK(hipMemcpy(c->a8l, a8.data(), nz * sizeof(double), hipMemcpyHostToDevice));
+  }
+  CHK(alloc((void**)&c->partial, (size_t)nz * c->leith_nblk * sizeof(double)));
+  {
+    // (1.3*Lx/(nx*pi))**12 with 1.3 a default-real literal (ARK43_QGLeith.f90:72)
+    const double c13 = (double)1.3f;
+    const double base = c13 * p->Lx / ((double)p->nx * 3.14159265358979323846);
+    double b2 = base * base, b4 = b2 * b2, b8 = b4 * b4;
+    c->c12 = b8 * b4;
+  }
+  const Geom& g = c->g;
+  c->ncmp = (long)nz * g.Ky * g.Kx;
+  c->nmix = (long)nz * g.ny * g.Kx;
+  const size_t rs = p->precision == QGN_MIXED ? sizeof(float2) : sizeof(double2);
+  c->rsize = rs;
+  const size_t cb = (size_t)c->ncmp * sizeof(double2);
+  CHK(alloc((void**)&c->q, cb));
+  CHK(alloc((void**)&c->qn, cb));
+  CHK(alloc((void**)&c->E, cb));
+  CHK(alloc((void**)&c->psi64, cb));
+  CHK(alloc(&c->psi, (size_t)c->ncmp * rs));
+  HIPCHK(hipMemset(c->q, 0, cb));
+  for (int i = 0; i < 6; ++i) CHK(alloc(&c->N[i], (size_t)c->ncmp * rs));
+  for (int i = 1; i < 6; ++i) CHK(alloc((void**)&c->Y[i], cb));
+  for (int i = 0; i < 3; ++i) CHK(alloc(&c->hist[i], (size_t)c->ncmp * rs));
+  const size_t mb = (size_t)c->nmix * rs;
+  CHK(alloc(&c->Qm, mb));
+  CHK(alloc(&c->Um, mb));
+  CHK(alloc(&c->Vm, mb));
+  CHK(alloc(&c->UQm, mb));
+  CHK(alloc(&c->VQm, mb));
+  CHK(alloc(&c->DUm, (size_t)g.ny * g.Kx * rs));
+  CHK(alloc(&c->DVm, (size_t)g.ny * g.Kx * rs));
+  CHK(alloc((void**)&c->Em, (size_t)c->nmix * sizeof(double2)));
+  CHK(alloc((void**)&c->errslot, sizeof(unsigned long long)));
+  HIPCHK(hipHostMalloc((void**)&c->h_errslot, sizeof(unsigned long long), hipHostMallocDefault));
+  const size_t fbytes = (size_t)nz * g.ny * g.nxh * sizeof(double2);
+  CHK(alloc((void**)&c->fa, fbytes));
+  CHK(alloc((void**)&c->fb, fbytes));
+  CHK(alloc((void**)&c->phys, (size_t)nz * g.ny * g.nx * sizeof(double)));
+  HIPCHK(hipDeviceSynchronize());
+  return 0;
+}
+
+int pack_from_host(qgn_ctx* c, const double* full, double2* cmp) {
+  const Geom& g = c->g;
+  HIPCHK(hipMemcpyAsync(c->fa, full, (size_t)g.nz * g.ny * g.nxh * sizeof(double2), hipMemcpyHostToDevice,
+                        c->stream));
+  hipLaunchKernelGGL(k_pack, dim3(grid1d(c->ncmp)), dim3(256), 0, c->stream, c->fa, cmp, g);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int unpack_to_host(qgn_ctx* c, const double2* cmp, double* full) {
+  const Geom& g = c->g;
+  const long nf = (long)g.nz * g.ny * g.nxh;
+  hipLaunchKernelGGL(k_unpack, dim3(grid1d(nf)), dim3(256), 0, c->stream, cmp, c->fa, g);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(full, c->fa, nf * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// compact S-typed -> compact double2 (in E) for export
+int widen(qgn_ctx* c, const void* src, double2* dst) {
+  if (c->p.precision == QGN_MIXED)
+    return lincomb(c, src, nullptr, nullptr, nullptr, 1.0, 0, 0, 0, 1, dst);
+  HIPCHK(hipMemcpyAsync(dst, src, (size_t)c->ncmp * sizeof(double2), hipMemcpyDeviceToDevice, c->stream));
+  return 0;
+}
+
+// c2r of a compact FP64 field to physical, scaled by 1/(nx ny), into host
+int cmp_to_phys_host(qgn_ctx* c, const double2* cmp, double* out) {
+  const Geom& g = c->g;
+  const long nf = (long)g.nz * g.ny * g.nxh;
+  hipLaunchKernelGGL(k_unpack, dim3(grid1d(nf)), dim3(256), 0, c->stream, cmp, c->fa, g);
+  HIPCHK(hipGetLastError());
+  CHK(dft_bwd<double>(c->stream, g, c->px, c->py, c->twd, g.nz, c->fa, c->phys, c->fb,
+                      1.0 / ((double)g.nx * (double)g.ny)));
+  HIPCHK(hipMemcpyAsync(out, c->phys, (size_t)g.nz * g.ny * g.nx * sizeof(double), hipMemcpyDeviceToHost,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* qgn_last_error(void) { return g_err.c_str(); }
+const char* qgn_version(void) { return "qgn-mi355x 0.1 (gfx950)"; }
+
+int qgn_create(qgn_ctx** ctx, const qgn_params* p, const double* H, const double* Modes, const double* EVals,
+               const double* uBar, const double* qyBar) {
+  if (!ctx || !p || !H || !Modes || !EVals || !uBar || !qyBar) return fail(QGN_EINVAL, "null argument");
+  *ctx = nullptr;
+  qgn_ctx* c = new qgn_ctx();
+  int r = create_impl(c, p, H, Modes, EVals, uBar, qyBar);
+  if (r != 0) {
+    std::string msg = g_err;
+    destroy_ctx(c);
+    g_err = msg;
+    return r;
+  }
+  *ctx = c;
+  return 0;
+}
+
+int qgn_destroy(qgn_ctx* ctx) {
+  destroy_ctx(ctx);
+  return 0;
+}
+
+int qgn_initialize(qgn_ctx* c, const double* qp, double* q_hat) {
+  if (!c || !qp) return fail(QGN_EINVAL, "null argument");
+  const Geom& g = c->g;
+  HIPCHK(hipMemcpyAsync(c->phys, qp, (size_t)g.nz * g.ny * g.nx * sizeof(double), hipMemcpyHostToDevice,
+                        c->stream));
+  CHK(dft_fwd<double>(c->stream, g, c->px, c->py, c->twd, g.nz, c->phys, c->fa, c->fb));
+  hipLaunchKernelGGL(k_pack, dim3(grid1d(c->ncmp)), dim3(256), 0, c->stream, c->fa, c->q, g);
+  HIPCHK(hipGetLastError());
+  if (q_hat) CHK(unpack_to_host(c, c->q, q_hat));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int qgn_set_state(qgn_ctx* c, const double* q_hat) {
+  if (!c || !q_hat) return fail(QGN_EINVAL, "null argument");
+  CHK(pack_from_host(c, q_hat, c->q));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int qgn_get_state(qgn_ctx* c, double* q_hat) {
+  if (!c || !q_hat) return fail(QGN_EINVAL, "null argument");
+  return unpack_to_host(c, c->q, q_hat);
+}
+
+int qgn_time_step(qgn_ctx* c, double* t, double* dt) {
+  if (!c || !t || !dt) return fail(QGN_EINVAL, "null argument");
+  switch (c->p.scheme) {
+    case QGN_AB3: return step_ab3(c, t, dt);
+    case QGN_RK4: return step_rk4(c, t, dt);
+    case QGN_BSRK32: return step_bsrk3(c, t, dt);
+    case QGN_ARK43: return step_ark43(c, t, dt, false);
+    default: return step_ark43(c, t, dt, true);
+  }
+}
+
+int qgn_rhs(qgn_ctx* c, const double* q_hat, double* out) {
+  if (!c || !q_hat || !out) return fail(QGN_EINVAL, "null argument");
+  CHK(pack_from_host(c, q_hat, c->Y[5]));
+  CHK(rhs(c, c->Y[5], c->N[5]));
+  CHK(widen(c, c->N[5], c->E));
+  return unpack_to_host(c, c->E, out);
+}
+
+int qgn_get_psi(qgn_ctx* c, const double* q_hat, double* psi_hat) {
+  if (!c || !q_hat || !psi_hat) return fail(QGN_EINVAL, "null argument");
+  CHK(pack_from_host(c, q_hat, c->Y[5]));
+  CHK(launch_project<double>(c, c->Y[5], c->psi64));
+  return unpack_to_host(c, c->psi64, psi_hat);
+}
+
+int qgn_fields(qgn_ctx* c, double* q_phys, double* psi_phys) {
+  if (!c) return fail(QGN_EINVAL, "null argument");
+  if (q_phys) CHK(cmp_to_phys_host(c, c->q, q_phys));
+  if (psi_phys) {
+    CHK(launch_project<double>(c, c->q, c->psi64));
+    CHK(cmp_to_phys_host(c, c->psi64, psi_phys));
+  }
+  return 0;
+}
+
+int qgn_leith(qgn_ctx* c, double* a8l) {
+  if (!c || !a8l) return fail(QGN_EINVAL, "null argument");
+  CHK(leith(c, c->q));
+  HIPCHK(hipMemcpyAsync(a8l, c->a8l, c->g.nz * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int qgn_get_stats(qgn_ctx* c, qgn_stats* s) {
+  if (!c || !s) return fail(QGN_EINVAL, "null argument");
+  *s = c->st;
+  s->last_err = c->last_err;
+  s->err0 = c->err0;
+  return 0;
+}
+
+int qgn_get_stream(qgn_ctx* c, void** stream) {
+  if (!c || !stream) return fail(QGN_EINVAL, "null argument");
+  *stream = (void*)c->stream;
+  return 0;
+}
+
+int qgn_prof_enable(qgn_ctx* c, int id) {
+  if (!c || id < -1 || id >= QGN_K_COUNT) return fail(QGN_EINVAL, "bad kernel id");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->prof_id = id;
+  c->ev_used = 0;
+  c->prof_ms = 0.0;
+  c->prof_count = 0;
+  return 0;
+}
+
+int qgn_prof_read(qgn_ctx* c, int* count, double* total_ms) {
+  if (!c || !count || !total_ms) return fail(QGN_EINVAL, "null argument");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int i = 0; i < c->ev_used; ++i) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev[i].first, c->ev[i].second));
+    c->prof_ms += ms;
+    c->prof_count++;
+  }
+  c->ev_used = 0;
+  *count = c->prof_count;
+  *total_ms = c->prof_ms;
+  return 0;
+}
+
+// ---- DFTI replacement ------------------------------------------------------
+}  // extern "C"
+
+struct qgn_dft {
+  int precision, howmany;
+  Geom g;
+  FftPlan px, py;
+  Tw<double> twd;
+  Tw<float> twf;
+  hipStream_t stream = nullptr;
+  void *dgrid = nullptr, *dspec = nullptr, *tmp = nullptr;
+};
+
+namespace {
+void dft_destroy(qgn_dft* h) {
+  if (!h) return;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  if (h->dgrid) (void)hipFree(h->dgrid);
+  if (h->dspec) (void)hipFree(h->dspec);
+  if (h->tmp) (void)hipFree(h->tmp);
+  free_tw(&h->twd);
+  free_tw(&h->twf);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+}
+
+int dft_create_impl(qgn_dft* h, int precision, int nx, int ny, int howmany) {
+  std::string why;
+  if (!size_ok(nx, ny, &why)) return fail(QGN_EINVAL, why);
+  if (howmany < 1) return fail(QGN_EINVAL, "howmany must be >= 1");
+  if (precision != QGN_FP64 && precision != QGN_MIXED) return fail(QGN_EINVAL, "unknown precision");
+  h->precision = precision;
+  h->howmany = howmany;
+  h->g = make_geom(nx, ny, howmany, 1.0, 1.0);
+  make_plan(ny, &h->py);
+  make_plan(nx / 2, &h->px);
+  HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  if (precision == QGN_FP64) CHK(make_tw<double>(nx, ny, &h->twd));
+  else CHK(make_tw<float>(nx, ny, &h->twf));
+  const size_t es = precision == QGN_FP64 ? sizeof(double) : sizeof(float);
+  CHK(alloc(&h->tmp, (size_t)howmany * ny * h->g.nxh * 2 * es));
+  return 0;
+}
+
+int dft_ensure_host_bufs(qgn_dft* h) {
+  const size_t es = h->precision == QGN_FP64 ? sizeof(double) : sizeof(float);
+  if (!h->dgrid) CHK(alloc(&h->dgrid, (size_t)h->howmany * h->g.ny * h->g.nx * es));
+  if (!h->dspec) CHK(alloc(&h->dspec, (size_t)h->howmany * h->g.ny * h->g.nxh * 2 * es));
+  return 0;
+}
+
+int dft_run(qgn_dft* h, const void* in, void* out, hipStream_t s, bool fwd) {
+  if (h->precision == QGN_FP64) {
+    if (fwd)
+      return dft_fwd<double>(s, h->g, h->px, h->py, h->twd, h->howmany, (const double*)in, (double2*)out,
+                             (double2*)h->tmp);
+    return dft_bwd<double>(s, h->g, h->px, h->py, h->twd, h->howmany, (const double2*)in, (double*)out,
+                           (double2*)h->tmp, 1.0);
+  }
+  if (fwd)
+    return dft_fwd<float>(s, h->g, h->px, h->py, h->twf, h->howmany, (const float*)in, (float2*)out,
+                          (float2*)h->tmp);
+  return dft_bwd<float>(s, h->g, h->px, h->py, h->twf, h->howmany, (const float2*)in, (float*)out,
+                        (float2*)h->tmp, 1.0f);
+}
+}  // namespace
+
+extern "C" {
+
+int qgn_dft_create(qgn_dft** h, int precision, int nx, int ny, int howmany) {
+  if (!h) return fail(QGN_EINVAL, "null argument");
+  *h = nullptr;
+  qgn_dft* d = new qgn_dft();
+  int r = dft_create_impl(d, precision, nx, ny, howmany);
+  if (r != 0) {
+    std::string msg = g_err;
+    dft_destroy(d);
+    g_err = msg;
+    return r;
+  }
+  *h = d;
+  return 0;
+}
+
+int qgn_dft_free(qgn_dft* h) {
+  dft_destroy(h);
+  return 0;
+}
+
+int qgn_dft_forward(qgn_dft* h, const double* grid, double* spec) {
+  if (!h || !grid || !spec) return fail(QGN_EINVAL, "null argument");
+  if (h->precision != QGN_FP64) return fail(QGN_EINVAL, "host entry points take FP64 data");
+  CHK(dft_ensure_host_bufs(h));
+  const Geom& g = h->g;
+  HIPCHK(hipMemcpyAsync(h->dgrid, grid, (size_t)h->howmany * g.ny * g.nx * sizeof(double), hipMemcpyHostToDevice,
+                        h->stream));
+  CHK(dft_run(h, h->dgrid, h->dspec, h->stream, true));
+  HIPCHK(hipMemcpyAsync(spec, h->dspec, (size_t)h->howmany * g.ny * g.nxh * sizeof(double2),
+                        hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int qgn_dft_backward(qgn_dft* h, const double* spec, double* grid) {
+  if (!h || !grid || !spec) return fail(QGN_EINVAL, "null argument");
+  if (h->precision != QGN_FP64) return fail(QGN_EINVAL, "host entry points take FP64 data");
+  CHK(dft_ensure_host_bufs(h));
+  const Geom& g = h->g;
+  HIPCHK(hipMemcpyAsync(h->dspec, spec, (size_t)h->howmany * g.ny * g.nxh * sizeof(double2),
+                        hipMemcpyHostToDevice, h->stream));
+  CHK(dft_run(h, h->dspec, h->dgrid, h->stream, false));
+  HIPCHK(hipMemcpyAsync(grid, h->dgrid, (size_t)h->howmany * g.ny * g.nx * sizeof(double), hipMemcpyDeviceToHost,
+                        h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int qgn_dft_forward_device(qgn_dft* h, const void* grid, void* spec, void* stream) {
+  if (!h || !grid || !spec) return fail(QGN_EINVAL, "null argument");
+  return dft_run(h, grid, spec, stream ? (hipStream_t)stream : h->stream, true);
+}
+
+int qgn_dft_backward_device(qgn_dft* h, const void* spec, void* grid, void* stream) {
+  if (!h || !grid || !spec) return fail(QGN_EINVAL, "null argument");
+  return dft_run(h, spec, grid, stream ? (hipStream_t)stream : h->stream, false);
+}
+
+}  // extern "C"

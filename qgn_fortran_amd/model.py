@@ -1,0 +1,233 @@
+"""Host-side mirror of QGN_Module's public interface over the HIP library.
+
+``QGN`` follows QGN_Module.f90:7 (Initialize, TimeStep, WriteQP, Cleanup) plus
+the internal GetRHS/GetPsi used by the parity tests; ``Dfti`` follows the two
+MKL DFTI descriptors of MKL_DFTI_Setup.f90.  Arrays are numpy, in the
+reference's memory layout: spectra complex128 of shape (nz, ny, nx//2+1)
+(= Fortran complex(nx/2+1,ny,nz)), physical fields float64 (nz, ny, nx).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+import math
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, dptr
+
+PI = 4.0 * math.atan(1.0)
+
+
+@dataclasses.dataclass
+class Params:
+    """parameters.f90:1-31, sizes at run time."""
+    nx: int
+    ny: int
+    nz: int
+    Lx: float
+    Ly: float
+    Htot: float
+    beta: float
+    A2: float = 0.0
+    A8: float = 0.0
+    r0: float = 0.0
+    C_d: float = 0.0
+    TOL: float = 1e-10
+    is_explicit: bool = False
+    dt_max: float | None = None
+    scheme: str = "ARK43_QGLeith"
+    precision: str = "fp64"
+    max_rejects: int = 1000
+
+    def __post_init__(self):
+        if self.dt_max is None:
+            self.dt_max = PI / (self.beta * self.Lx)   # parameters.f90:31
+
+    def to_c(self) -> _lib.QgnParams:
+        return _lib.QgnParams(
+            self.nx, self.ny, self.nz, self.Lx, self.Ly, self.Htot, self.beta, self.A2, self.A8,
+            self.r0, self.C_d, self.TOL, self.dt_max, int(bool(self.is_explicit)),
+            _lib.SCHEMES[self.scheme], _lib.PRECISIONS[self.precision], int(self.max_rejects))
+
+
+def vertical_modes(H_raw, S, uBar, Htot):
+    """Vertical structure of Initialize (QGN_Module.f90:84-148) via the
+    library's symmetric eigensolver.  Returns (H, L, Modes, EVals, qyBar);
+    Modes[:, k] is mode k (numpy view of Fortran Modes(:,k))."""
+    lib = _lib.load()
+    H_raw = np.ascontiguousarray(H_raw, dtype=np.float64)
+    S = np.ascontiguousarray(S, dtype=np.float64)
+    uBar = np.ascontiguousarray(uBar, dtype=np.float64)
+    nz = H_raw.size
+    if S.size != nz + 1 or uBar.size != nz:
+        raise ValueError("S needs nz+1 entries and uBar nz entries")
+    H = np.empty(nz)
+    Lf = np.empty(nz * nz)
+    Mf = np.empty(nz * nz)
+    ev = np.empty(nz)
+    qy = np.empty(nz)
+    check(lib.qgn_vertical_modes(nz, float(Htot), dptr(H_raw), dptr(S), dptr(uBar), dptr(H), dptr(Lf),
+                                 dptr(Mf), dptr(ev), dptr(qy)))
+    # column-major (Fortran) -> numpy [row, col]
+    L = Lf.reshape(nz, nz).T.copy()
+    Modes = Mf.reshape(nz, nz).T.copy()
+    return H, L, Modes, ev, qy
+
+
+class QGN:
+    """One QGN model instance on the GPU (module state of QGN_Module.f90:11-41)."""
+
+    def __init__(self, p: Params, H, Modes, EVals, uBar, qyBar):
+        self.lib = _lib.load()
+        self.p = p
+        self.nxh = p.nx // 2 + 1
+        self.H = np.ascontiguousarray(H, dtype=np.float64)
+        modes_f = np.ascontiguousarray(np.asarray(Modes, dtype=np.float64).T)  # Fortran order
+        self._ctx = C.c_void_p()
+        cp = p.to_c()
+        check(self.lib.qgn_create(C.byref(self._ctx), C.byref(cp), dptr(self.H), dptr(modes_f),
+                                  dptr(np.ascontiguousarray(EVals, dtype=np.float64)),
+                                  dptr(np.ascontiguousarray(uBar, dtype=np.float64)),
+                                  dptr(np.ascontiguousarray(qyBar, dtype=np.float64))))
+
+    @classmethod
+    def from_profiles(cls, p: Params, H_raw, S, uBar):
+        H, _L, Modes, EVals, qyBar = vertical_modes(H_raw, S, uBar, p.Htot)
+        return cls(p, H, Modes, EVals, uBar, qyBar)
+
+    # ---- shapes ----------------------------------------------------------
+    def spec_shape(self):
+        return (self.p.nz, self.p.ny, self.nxh)
+
+    def phys_shape(self):
+        return (self.p.nz, self.p.ny, self.p.nx)
+
+    def _spec_in(self, a):
+        a = np.ascontiguousarray(a, dtype=np.complex128)
+        if a.shape != self.spec_shape():
+            raise ValueError(f"spectrum shape {a.shape} != {self.spec_shape()}")
+        return a
+
+    # ---- QGN_Module procedures ------------------------------------------
+    def initialize(self, q0):
+        """Initialize's IC transform (QGN_Module.f90:177-183); returns q_hat."""
+        q0 = np.ascontiguousarray(q0, dtype=np.float64)
+        if q0.shape != self.phys_shape():
+            raise ValueError("q0 shape")
+        out = np.empty(self.spec_shape(), dtype=np.complex128)
+        check(self.lib.qgn_initialize(self._ctx, dptr(q0), dptr(out)))
+        return out
+
+    def set_state(self, q_hat):
+        check(self.lib.qgn_set_state(self._ctx, dptr(self._spec_in(q_hat))))
+
+    def get_state(self):
+        out = np.empty(self.spec_shape(), dtype=np.complex128)
+        check(self.lib.qgn_get_state(self._ctx, dptr(out)))
+        return out
+
+    def time_step(self, t, dt):
+        """TimeStep (QGN_Module.f90:302-309); returns the new (t, dt)."""
+        tt, dd = C.c_double(t), C.c_double(dt)
+        check(self.lib.qgn_time_step(self._ctx, C.byref(tt), C.byref(dd)))
+        return tt.value, dd.value
+
+    def get_rhs(self, q_hat):
+        """GetRHS (QGN_Module.f90:258-296)."""
+        out = np.empty(self.spec_shape(), dtype=np.complex128)
+        check(self.lib.qgn_rhs(self._ctx, dptr(self._spec_in(q_hat)), dptr(out)))
+        return out
+
+    def get_psi(self, q_hat):
+        """GetPsi (QGN_Module.f90:315-341)."""
+        out = np.empty(self.spec_shape(), dtype=np.complex128)
+        check(self.lib.qgn_get_psi(self._ctx, dptr(self._spec_in(q_hat)), dptr(out)))
+        return out
+
+    def fields(self):
+        """WriteQP's q_phys and psi_phys of the current state (QGN_Module.f90:353-361)."""
+        q = np.empty(self.phys_shape())
+        psi = np.empty(self.phys_shape())
+        check(self.lib.qgn_fields(self._ctx, dptr(q), dptr(psi)))
+        return q, psi
+
+    def leith(self):
+        """A8L(nz) of the current state (ARK43_QGLeith.f90:62-73)."""
+        out = np.empty(self.p.nz)
+        check(self.lib.qgn_leith(self._ctx, dptr(out)))
+        return out
+
+    def stats(self):
+        s = _lib.QgnStats()
+        check(self.lib.qgn_get_stats(self._ctx, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in s._fields_}
+
+    def stream(self):
+        p = C.c_void_p()
+        check(self.lib.qgn_get_stream(self._ctx, C.byref(p)))
+        return p.value
+
+    def prof_enable(self, kernel):
+        check(self.lib.qgn_prof_enable(self._ctx, _lib.KERNELS[kernel] if isinstance(kernel, str) else kernel))
+
+    def prof_read(self):
+        n = C.c_int()
+        ms = C.c_double()
+        check(self.lib.qgn_prof_read(self._ctx, C.byref(n), C.byref(ms)))
+        return n.value, ms.value
+
+    def close(self):
+        """Cleanup (QGN_Module.f90:448-458)."""
+        if self._ctx:
+            self.lib.qgn_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Dfti:
+    """Batched 2-D real FFT with the descriptor of MKL_DFTI_Setup.f90:11-50.
+
+    forward(grid) = DftiComputeForward(g2s_q, ...), backward(spec) =
+    DftiComputeBackward(s2g_q, ...); both unnormalised."""
+
+    def __init__(self, nx, ny, howmany, precision="fp64"):
+        self.lib = _lib.load()
+        self.nx, self.ny, self.howmany = nx, ny, howmany
+        self._h = C.c_void_p()
+        check(self.lib.qgn_dft_create(C.byref(self._h), _lib.PRECISIONS[precision], nx, ny, howmany))
+
+    def forward(self, grid):
+        grid = np.ascontiguousarray(grid, dtype=np.float64).reshape(self.howmany, self.ny, self.nx)
+        out = np.empty((self.howmany, self.ny, self.nx // 2 + 1), dtype=np.complex128)
+        check(self.lib.qgn_dft_forward(self._h, dptr(grid), dptr(out)))
+        return out
+
+    def backward(self, spec):
+        spec = np.ascontiguousarray(spec, dtype=np.complex128).reshape(self.howmany, self.ny, self.nx // 2 + 1)
+        out = np.empty((self.howmany, self.ny, self.nx))
+        check(self.lib.qgn_dft_backward(self._h, dptr(spec), dptr(out)))
+        return out
+
+    def forward_device(self, grid_ptr, spec_ptr, stream=None):
+        check(self.lib.qgn_dft_forward_device(self._h, grid_ptr, spec_ptr, stream))
+
+    def backward_device(self, spec_ptr, grid_ptr, stream=None):
+        check(self.lib.qgn_dft_backward_device(self._h, spec_ptr, grid_ptr, stream))
+
+    def close(self):
+        if self._h:
+            self.lib.qgn_dft_free(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,151 @@
+"""GPU parity: the HIP library through its C-ABI against the FP64 oracle on the
+same seeded inputs.  Tolerances: FP64 path per-layer relative L2 <= 1e-12 for
+one RHS and for transforms; trajectories <= 1e-10 after a few steps; mixed
+precision RHS relative L2 <= 1e-5 against FP64."""
+import numpy as np
+import pytest
+
+from conftest import make_case, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu_model(case):
+    from qgn_fortran_amd import QGN
+    return QGN(case["pp"], case["H"], case["Modes"], case["EVals"], case["uBar"], case["qyBar"])
+
+
+def _oracle_model(case):
+    from oracle import qgn_oracle as orc
+    return orc.Model(case["po"], case["H"], case["Modes"], case["EVals"], case["uBar"], case["qyBar"])
+
+
+@pytest.mark.parametrize("nx,ny,howmany", [(24, 18, 1), (96, 96, 3), (128, 128, 2), (384, 384, 2),
+                                            (1024, 1024, 2), (160, 120, 2), (2048, 64, 1)])
+def test_dft_forward_backward(nx, ny, howmany):
+    """DFTI-equivalent r2c / c2r (MKL_DFTI_Setup.f90) vs pocketfft; backward
+    on non-Hermitian input follows the real-part rule for kx = 0, nx/2."""
+    from oracle import qgn_oracle as orc
+    from qgn_fortran_amd import Dfti
+    print(f"dft {nx}x{ny}x{howmany}", flush=True)
+    rng = np.random.default_rng(nx * 7 + ny)
+    g = rng.standard_normal((howmany, ny, nx))
+    d = Dfti(nx, ny, howmany)
+    s = d.forward(g)
+    ref = orc.rfft2(g)
+    assert rel_l2(s, ref) < 1e-13
+    spec = rng.standard_normal(ref.shape) + 1j * rng.standard_normal(ref.shape)
+    back = d.backward(spec)
+    refb = orc.irfft2(spec, nx)
+    assert rel_l2(back, refb) < 1e-13
+    d.close()
+
+
+RHS_CASES = [
+    dict(nx=128, ny=128, nz=2, profile="two", A8=1e23, is_explicit=True, C_d=0.0, k_peak=8),
+    dict(nx=96, ny=96, nz=4, C_d=1e-5),
+    dict(nx=192, ny=128, nz=3, C_d=1e-5, Ly=0.7e6),
+    dict(nx=384, ny=384, nz=8, C_d=1e-5, k_peak=16),
+    dict(nx=256, ny=256, nz=16, C_d=0.0, k_peak=16),
+]
+
+
+@pytest.mark.parametrize("kw", RHS_CASES, ids=lambda k: f"{k['nx']}x{k['ny']}x{k['nz']}")
+def test_rhs_fp64(kw):
+    """GetRHS (QGN_Module.f90:258-296): per-layer relative L2 <= 1e-12."""
+    print("rhs", kw, flush=True)
+    case = make_case(**kw)
+    om = _oracle_model(case)
+    gm = _gpu_model(case)
+    q_hat = om.initialize(case["q0"])
+    ref = om.get_rhs(q_hat)
+    out = gm.get_rhs(q_hat)
+    err = rel_l2(out, ref, axis=(1, 2))
+    print("per-layer rel L2", err)
+    assert np.all(err < 1e-12)
+    psi = gm.get_psi(q_hat)
+    assert np.all(rel_l2(psi, om.get_psi(q_hat), axis=(1, 2)) < 1e-13)
+    gm.close()
+
+
+def test_initialize_and_fields():
+    """Initialize (QGN_Module.f90:177-183) and WriteQP's fields (:353-361)."""
+    case = make_case(192, 192, 4, k_peak=10)
+    om = _oracle_model(case)
+    gm = _gpu_model(case)
+    qh = gm.initialize(case["q0"])
+    ref = om.initialize(case["q0"])
+    assert rel_l2(qh, ref) < 1e-13
+    q, psi = gm.fields()
+    qr, pr = om.get_psi_phys(ref)
+    assert rel_l2(q, qr) < 1e-13
+    assert rel_l2(psi, pr) < 1e-13
+    st = gm.get_state()
+    assert np.array_equal(st, qh)
+    gm.close()
+
+
+def test_leith_coefficients():
+    """A8L (ARK43_QGLeith.f90:62-73): Parseval on the device vs physical-space
+    sums in the oracle, including the non-Hermitian kx=0 column at n=128."""
+    case = make_case(128, 128, 4, k_peak=12)
+    om = _oracle_model(case)
+    gm = _gpu_model(case)
+    qh = om.initialize(case["q0"])
+    gm.set_state(qh)
+    a = gm.leith()
+    r = om.leith_a8(qh)
+    assert np.max(np.abs(a / r - 1)) < 1e-12
+    gm.close()
+
+
+STEP_CASES = [
+    ("RK4", dict(nx=64, ny=64, nz=2, profile="two", A8=1e23, is_explicit=True, C_d=0.0), 1800.0, 6),
+    ("AB3", dict(nx=64, ny=64, nz=2, profile="two", A8=1e23, is_explicit=True, C_d=0.0), 1800.0, 6),
+    ("BSRK32", dict(nx=96, ny=96, nz=3, C_d=1e-5, TOL=1e-12), 750.0, 6),
+    ("ARK43", dict(nx=96, ny=96, nz=3, C_d=1e-5, A8=1e22, TOL=1e-11), 750.0, 6),
+    ("ARK43_QGLeith", dict(nx=96, ny=96, nz=4, C_d=1e-5, TOL=1e-11), 750.0, 6),
+    ("ARK43_QGLeith", dict(nx=128, ny=128, nz=4, C_d=1e-5, TOL=1e-12, rms=1e-5), 20000.0, 4),
+]
+
+
+@pytest.mark.parametrize("scheme,kw,dt0,nsteps", STEP_CASES, ids=lambda x: str(x)[:20])
+def test_stepper_trajectory(scheme, kw, dt0, nsteps):
+    """TimeStep with each include fragment: state, t and dt after each step
+    agree with the oracle (accept/reject sequence included)."""
+    from oracle import qgn_oracle as orc
+    print("step", scheme, kw, flush=True)
+    case = make_case(scheme=scheme, **kw)
+    om = _oracle_model(case)
+    gm = _gpu_model(case)
+    stepper = orc.SCHEMES[scheme](om)
+    q = om.initialize(case["q0"])
+    gm.set_state(q)
+    t_o = t_g = 0.0
+    dt_o = dt_g = dt0
+    for n in range(nsteps):
+        q, t_o, dt_o = stepper.step(q, t_o, dt_o)
+        t_g, dt_g = gm.time_step(t_g, dt_g)
+        qg = gm.get_state()
+        e = rel_l2(qg, q)
+        assert e < 1e-10, (n, e)
+        assert abs(t_g - t_o) <= 1e-12 * max(1.0, abs(t_o)), (n, t_g, t_o)
+        assert abs(dt_g - dt_o) <= 1e-9 * dt_o, (n, dt_g, dt_o)
+    rej = getattr(stepper, "rejects", 0)
+    assert gm.stats()["rejects"] == rej
+    gm.close()
+
+
+def test_mixed_precision_rhs():
+    """FP32 FFT/Jacobian path: RHS relative L2 vs the FP64 RHS <= 1e-5."""
+    case = make_case(256, 256, 8, k_peak=16, C_d=1e-5)
+    om = _oracle_model(case)
+    q_hat = om.initialize(case["q0"])
+    ref = om.get_rhs(q_hat)
+    case["pp"].precision = "mixed"
+    gm = _gpu_model(case)
+    out = gm.get_rhs(q_hat)
+    err = rel_l2(out, ref, axis=(1, 2))
+    print("mixed per-layer rel L2", err)
+    assert np.all(err < 1e-5)
+    gm.close()
