@@ -22,21 +22,20 @@ something other than its own formula):
   * rfft2/irfft2      - brute-force O(n^4) DFT with explicit symmetrisation,
                         closed-form single modes, Parseval.
   * dealias           - closed-form kept index sets for n = 96..2048.
-  * get_psi           - per-wavenumber dense solve (k^2 I - L) psi = -q ...,
+  * get_psi           - per-wavenumber dense solve (-k^2 I + L) psi = q,
                         H-orthonormality of the modes.
   * jacobian          - closed form for two single Fourier modes,
                         antisymmetry, energy/enstrophy conservation at n%3==0.
   * get_rhs linear    - Rossby-wave dispersion relation, Ekman closed form.
-  * quadratic drag    - parity unpinned beyond restatement: no closed form
-                        exists for the band-limited |u|u product; it is checked
-                        against a direct physical-space evaluation only.
+  * quadratic drag    - closed form for a single barotropic Fourier mode
+                        (|u| then has a closed-form product).
   * vertical_modes    - the deformation radii stored in
                         set_background_profiles.ipynb:180-185, 2-layer closed
                         form, L Modes = Modes diag(EVals).
   * steppers          - Butcher-coefficient identities, linear-test-equation
                         amplification factors, observed convergence order.
-  * leith / PI.3.4    - Parseval identity for the Leith sum; controller
-                        formula checked on hand-computed scalars.
+  * leith / pi34      - Leith sum against a single-mode closed form and the
+                        binary32 literal 1.3; controller on hand-computed scalars.
 """
 from __future__ import annotations
 
@@ -76,7 +75,7 @@ class Params:
     def __post_init__(self):
         if self.dt_max is None:
             # parameters.f90:31  CFL for the fastest Rossby wave
-            self.dt_max = PI / (self.beta * self.Lx)
+            self.dt_max = PI / (self.beta * self.Lx) if self.beta != 0 else float("inf")
 
 
 def stretching_matrix(H, S):
@@ -302,6 +301,13 @@ B = [_fd(v) for v in ARK_B]
 BE = [_fd(v) for v in ARK_BE]
 
 
+def pi34(dt, err0, err1, target, a, b, dt_max):
+    """PI step-size controller: dt (target/err1)^a (err0/err1)^b, capped at
+    dt_max.  ARK43*.f90 use a = 0.075, b = 0.1 (ARK43_QGLeith.f90:137,
+    ARK43.f90:122); BSRK3.f90:87 uses a = 0.1, b = 0.4/3."""
+    return min(dt * ((target / err1) ** a) * ((err0 / err1) ** b), dt_max)
+
+
 class ARK43:
     """ARK4(3)6L[2]SA with fixed A8 (ARK43.f90) or QG-Leith A8L (ARK43_QGLeith.f90).
 
@@ -362,7 +368,7 @@ class ARK43:
                                   + B[5] * (N[6] + L[6]))
             q_new[:, 0, 0] = 0.0
             t = t + dt
-            dt = min(dt * ((safety * p.TOL / err1) ** 0.075) * ((self.err0 / err1) ** 0.1), p.dt_max)
+            dt = pi34(dt, self.err0, err1, safety * p.TOL, 0.075, 0.1, p.dt_max)
             self.err0 = err1
             return dealias(q_new, p.nx, p.ny), t, dt
 
@@ -449,7 +455,7 @@ class BSRK3:
             self.log.append((dt, r1, True))
             t = t + dt
             self.rhs0 = rhs3
-            dt = min(dt * ((0.75 * p.TOL / r1) ** 0.1) * ((self.r0 / r1) ** (0.4 / 3.0)), p.dt_max)
+            dt = pi34(dt, self.r0, r1, 0.75 * p.TOL, 0.1, 0.4 / 3.0, p.dt_max)
             self.r0 = r1
             return dealias(q_tmp, p.nx, p.ny), t, dt
 
@@ -457,28 +463,3 @@ class BSRK3:
 SCHEMES = {"AB3": AB3, "RK4": RK4, "BSRK32": BSRK3,
            "ARK43": lambda m: ARK43(m, leith=False),
            "ARK43_QGLeith": lambda m: ARK43(m, leith=True)}
-
-
-# ---------------------------------------------------------------------------
-# Seeded initial condition (builder's generator, not in the reference)
-# ---------------------------------------------------------------------------
-def random_phase_pv(nx, ny, nz, Lx, Ly, k_peak, rms, seed):
-    """Seeded random-phase PV with an isotropic amplitude spectrum peaked at
-    wavenumber index ``k_peak``; each layer scaled to RMS ``rms``.  Physical
-    field (nz, ny, nx), float64.  numpy PCG64(seed)."""
-    rng = np.random.Generator(np.random.PCG64(seed))
-    nxh = nx // 2 + 1
-    i = np.arange(nxh)[None, :]
-    j = np.arange(ny)[:, None]
-    jj = np.where(j <= ny // 2, j, j - ny)
-    kk = np.sqrt((i * (Lx / Lx)) ** 2 + (jj * (Lx / Ly)) ** 2)
-    amp = kk ** 2 * np.exp(-(kk / k_peak) ** 2)      # peak at kk = k_peak
-    q = np.empty((nz, ny, nx))
-    for z in range(nz):
-        ph = rng.uniform(0.0, 2.0 * PI, size=(ny, nxh))
-        spec = amp * np.exp(1j * ph)
-        f = sfft.irfft2(spec, s=(ny, nx))
-        f -= f.mean()
-        f *= rms / math.sqrt(np.mean(f ** 2))
-        q[z] = f
-    return q

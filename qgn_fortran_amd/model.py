@@ -43,7 +43,7 @@ class Params:
 
     def __post_init__(self):
         if self.dt_max is None:
-            self.dt_max = PI / (self.beta * self.Lx)   # parameters.f90:31
+            self.dt_max = PI / (self.beta * self.Lx) if self.beta != 0 else float("inf")   # parameters.f90:31
 
     def to_c(self) -> _lib.QgnParams:
         return _lib.QgnParams(
