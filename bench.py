@@ -20,18 +20,21 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# algorithmic HBM bytes per launch of the profiled kernels
+# Algorithmic HBM bytes per launch of the RHS passes, counting only the kept
+# (unpadded) modes: rs = bytes of one complex of the pass's arithmetic type,
+# 16 for the FP64 compact state q.  ncmp = nz*Ky*Kx compact modes,
+# nmix = nz*ny*Kx mixed values.
 def kernel_bytes(kernel, g, rs):
     nz, ny, Kx, Ky = g["nz"], g["ny"], g["Kx"], g["Ky"]
     ncmp = nz * Ky * Kx
     nmix = nz * ny * Kx
-    drag = g["drag"]
-    if kernel == "row_jac":
-        return (3 + 2) * nmix * rs + (2 * ny * Kx * rs if drag else 0)
-    if kernel == "col_inv":
-        return ncmp * 16 + ncmp * rs + 3 * nmix * rs
-    if kernel == "col_rhs":
-        return 2 * nmix * rs + (2 * ny * Kx * rs if drag else 0) + ncmp * (16 + 2 * rs)
+    drag = 2 * ny * Kx * rs if g["drag"] else 0
+    if kernel == "row_jac":     # read u, v, q_x, q_y; write J (+ |u|u, |u|v on layer nz)
+        return 5 * nmix * rs + drag
+    if kernel == "col_inv":     # read q, psi; write u, v, q_x, q_y
+        return ncmp * (16 + rs) + 4 * nmix * rs
+    if kernel == "col_rhs":     # read J, q, psi; write N
+        return nmix * rs + ncmp * (16 + 2 * rs)
     raise ValueError(kernel)
 
 
@@ -135,6 +138,8 @@ def main():
     ap.add_argument("--kernel", default="row_jac", help="kernel timed for the roofline figure")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=30.0)
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="first copy 1 GiB three times (reference kernel for PMC byte counters)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (members/slabs would "
@@ -146,6 +151,13 @@ def main():
     if not torch.cuda.is_available():
         print("no GPU", file=sys.stderr)
         sys.exit(1)
+    if a.calib_copy:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     wl = configs.WORKLOADS[a.workload]()
     q, t0, dt0 = spin_up(wl, a.spinup)
     mixed, _ = run_gpu(configs.WORKLOADS[a.workload](), "mixed", q, t0, dt0, a.steps, a.warmup, a.kernel)

@@ -22,6 +22,7 @@ struct Geom {
   int nxh;   // nx/2 + 1
   int N2;    // nx/2
   int Kx;    // kept kx indices: 0 .. nx/3-1
+  int Kxp;   // row stride of compact and mixed arrays: Kx padded to 16
   int Ky;    // kept ky rows
   int ya;    // rows [0, ya) kept
   int yb;    // rows [yb, ny) kept
@@ -51,15 +52,15 @@ __global__ void k_fill_u64(unsigned long long* p, unsigned long long v) {
 // Layout conversion: full <-> compact (DeAlias, QGN_Module.f90:194-204)
 // ---------------------------------------------------------------------------
 __global__ void k_pack(const double2* __restrict__ full, double2* __restrict__ cmp, Geom g) {
-  const long n = (long)g.nz * g.Ky * g.Kx;
+  const long n = (long)g.nz * g.Ky * g.Kxp;
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
-    const int i = (int)(e % g.Kx);
-    const long r = e / g.Kx;
+    const int i = (int)(e % g.Kxp);
+    const long r = e / g.Kxp;
     const int jc = (int)(r % g.Ky);
     const int z = (int)(r / g.Ky);
     const int j = compact_row_to_full(jc, g);
-    double2 v = full[((long)z * g.ny + j) * g.nxh + i];
-    if (i == 0 && j == 0) v = make_double2(0.0, 0.0);
+    double2 v = make_double2(0.0, 0.0);
+    if (i < g.Kx && !(i == 0 && j == 0)) v = full[((long)z * g.ny + j) * g.nxh + i];
     cmp[e] = v;
   }
 }
@@ -73,7 +74,7 @@ __global__ void k_unpack(const double2* __restrict__ cmp, double2* __restrict__ 
     const int z = (int)(r / g.ny);
     const int jc = full_row_to_compact(j, g);
     double2 v = make_double2(0.0, 0.0);
-    if (jc >= 0 && i < g.Kx && !(i == 0 && j == 0)) v = cmp[((long)z * g.Ky + jc) * g.Kx + i];
+    if (jc >= 0 && i < g.Kx && !(i == 0 && j == 0)) v = cmp[((long)z * g.Ky + jc) * g.Kxp + i];
     full[e] = v;
   }
 }
@@ -89,7 +90,7 @@ template <typename Tout>
 __global__ void k_project(const double2* __restrict__ q, cplx<Tout>* __restrict__ psi,
                           const double* __restrict__ HM, const double* __restrict__ MD,
                           const double* __restrict__ evals, double inv_htot, int pp_per_block, Geom g) {
-  extern __shared__ double lds_d[];
+  extern __shared__ __align__(16) double lds_d[];
   const int nz = g.nz;
   double* sHM = lds_d;
   double* sMD = sHM + nz * nz;
@@ -97,7 +98,7 @@ __global__ void k_project(const double2* __restrict__ q, cplx<Tout>* __restrict_
   double2* sm = sq + nz * pp_per_block;                      // [nz][pp]
   const int tid = threadIdx.x;
   for (int e = tid; e < nz * nz; e += blockDim.x) { sHM[e] = HM[e]; sMD[e] = MD[e]; }
-  const long plane = (long)g.Ky * g.Kx;
+  const long plane = (long)g.Ky * g.Kxp;
   const int pp = tid % pp_per_block;
   const int k = tid / pp_per_block;
   const long pt = (long)blockIdx.x * pp_per_block + pp;
@@ -112,12 +113,12 @@ __global__ void k_project(const double2* __restrict__ q, cplx<Tout>* __restrict_
       acc.x += w * v.x;
       acc.y += w * v.y;
     }
-    const int i = (int)(pt % g.Kx);
-    const int j = compact_row_to_full((int)(pt / g.Kx), g);
+    const int i = (int)(pt % g.Kxp);
+    const int j = compact_row_to_full((int)(pt / g.Kxp), g);
     const double kx = kx_of(i, g), ky = ky_of(j, g);
     const double den = -(kx * kx + ky * ky) + evals[k];
     double2 pm = make_double2(acc.x * inv_htot / den, acc.y * inv_htot / den);
-    if (i == 0 && j == 0) pm = make_double2(0.0, 0.0);
+    if ((i == 0 && j == 0) || i >= g.Kx) pm = make_double2(0.0, 0.0);
     sm[k * pp_per_block + pp] = pm;
   }
   __syncthreads();
@@ -134,58 +135,47 @@ __global__ void k_project(const double2* __restrict__ q, cplx<Tout>* __restrict_
 }
 
 // ---------------------------------------------------------------------------
-// Column pass, spectral -> mixed: y-inverse FFT of up to three fields built
-// from compact inputs (Jacobian's q, u = -i ky psi, v = i kx psi;
-// QGN_Module.f90:222-235).  Grid (tiles of C kx columns, nz).
-//   mult[f]: 0 -> x1, 1 -> -i ky, 2 -> +i kx
+// Argument blocks of the RHS passes (kernels_ct.hpp).
+//
+// Column pass P1, spectral -> mixed: y-inverse FFT of fields built from
+// compact inputs.  For the Jacobian these are u = -i ky psi, v = i kx psi,
+// q_x = i kx q, q_y = i ky q (QGN_Module.f90:222-225 builds u, v the same
+// way).  mult: 0 -> x1, 1 -> -i ky, 2 -> +i kx, 3 -> +i ky.
 // ---------------------------------------------------------------------------
 struct ColInvArgs {
-  const void* src[3];   // compact inputs (double2 for kind 0, else Tin)
-  int src_is_double[3];
-  int mult[3];
-  void* dst[3];         // mixed outputs (cplx<R>)
+  const void* src[4];   // compact inputs (double2 when src_is_double, else cplx<R>)
+  int src_is_double[4];
+  int mult[4];
+  void* dst[4];         // mixed outputs (cplx<R>)
   int nf;
 };
 
-template <typename R, int EPT>
-__global__ void __launch_bounds__(256) k_col_inv(ColInvArgs a, Geom g, FftPlan py, const cplx<R>* __restrict__ twy, int C) {
-  using Cx = cplx<R>;
-  extern __shared__ unsigned char lds_raw[];
-  Cx* buf = reinterpret_cast<Cx*>(lds_raw);
-  const int i0 = blockIdx.x * C;
-  const int z = blockIdx.y;
-  const int nthr = blockDim.x;
-  const int tot = C * g.ny;
-  const LdsBatch B{C, 1, C, true};
-  for (int f = 0; f < a.nf; ++f) {
-    for (int e = threadIdx.x; e < tot; e += nthr) {
-      const int c = e % C, j = e / C, i = i0 + c;
-      const int jc = full_row_to_compact(j, g);
-      Cx v = cmk<Cx>((R)0, (R)0);
-      if (jc >= 0 && i < g.Kx) {
-        const long idx = ((long)z * g.Ky + jc) * g.Kx + i;
-        if (a.src_is_double[f]) v = to_c<R>(reinterpret_cast<const double2*>(a.src[f])[idx]);
-        else v = reinterpret_cast<const Cx*>(a.src[f])[idx];
-        if (a.mult[f] == 1) {          // -i ky
-          const R ky = (R)ky_of(j, g);
-          v = cmk<Cx>(ky * v.y, -ky * v.x);
-        } else if (a.mult[f] == 2) {   // +i kx
-          const R kx = (R)kx_of(i, g);
-          v = cmk<Cx>(-kx * v.y, kx * v.x);
-        }
-      }
-      buf[e] = v;
-    }
-    __syncthreads();
-    lds_fft<R, EPT>(buf, B, py, twy, +1);
-    Cx* dst = reinterpret_cast<Cx*>(a.dst[f]);
-    for (int e = threadIdx.x; e < tot; e += nthr) {
-      const int c = e % C, j = e / C, i = i0 + c;
-      if (i < g.Kx) dst[((long)z * g.ny + j) * g.Kx + i] = buf[e];
-    }
-    __syncthreads();
-  }
-}
+// Row pass P2: x-c2r of u, v, q_x, q_y; J = u q_x + v q_y (and |u| u,
+// |u| v on the bottom layer); x-r2c keeping the Kx dealiased bins.
+struct RowJacArgs {
+  const void* U; const void* V; const void* QX; const void* QY;  // mixed (nz, ny, Kxp)
+  void* J;                                                       // mixed (nz, ny, Kxp)
+  void* DU; void* DV;                                            // (ny, Kxp), bottom layer
+  int drag;
+};
+
+// Column pass P3: y-forward FFT of J and the rest of GetRHS.
+struct ColRhsArgs {
+  const void* J;         // mixed cplx<R>
+  const double2* q;      // compact
+  const void* psi;       // compact cplx<R>
+  void* out;             // compact cplx<S>
+  const double* ubar; const double* qybar;
+  double beta, ekman, A2, A8;
+  int explicit_visc;
+};
+
+// Bottom-layer quadratic drag: out(nz) += cdrag (i ky du_hat - i kx dv_hat).
+struct DragArgs {
+  const void* DU; const void* DV;   // (ny, Kxp) cplx<R>
+  void* out;                        // compact cplx<S>, layer nz-1 updated in place
+  double cdrag;
+};
 
 // ---------------------------------------------------------------------------
 // x-c2r pre-processing: Z[k] = (X[k] + conj X[N2-k]) + i e^{+2 pi i k/nx}
@@ -193,6 +183,20 @@ __global__ void __launch_bounds__(256) k_col_inv(ColInvArgs a, Geom g, FftPlan p
 // and kx = nx/2 bins are ignored, as DFTI does.  z = IFFT_N2(Z) then holds
 // x[2m] + i x[2m+1].
 // ---------------------------------------------------------------------------
+// From the already loaded bins a = X[k], b = X[N2-k].
+template <typename R>
+__device__ __forceinline__ cplx<R> c2r_pre_ab(cplx<R> a, cplx<R> b, int k, const cplx<R>* __restrict__ twr,
+                                              R scale) {
+  using Cx = cplx<R>;
+  if (k == 0) { a.y = (R)0; b.y = (R)0; }
+  const Cx s = cmk<Cx>(a.x + b.x, a.y - b.y);   // X[k] + conj X[N2-k]
+  const Cx d = cmk<Cx>(a.x - b.x, a.y + b.y);   // X[k] - conj X[N2-k]
+  Cx w = twr[k];
+  w.y = -w.y;                                   // e^{+2 pi i k / nx}
+  const Cx wd = cmul(w, d);
+  return cmk<Cx>((s.x - wd.y) * scale, (s.y + wd.x) * scale);  // s + i*wd
+}
+
 template <typename R>
 __device__ __forceinline__ cplx<R> c2r_pre(const cplx<R>* __restrict__ row, int Kin, int k, int N2,
                                            const cplx<R>* __restrict__ twr, R scale) {
@@ -224,161 +228,6 @@ __device__ __forceinline__ cplx<R> r2c_post(const cplx<R>* __restrict__ zs, int 
 }
 
 // ---------------------------------------------------------------------------
-// Row pass of the Jacobian (QGN_Module.f90:227-242 and the quadratic drag
-// products :285-290): x-c2r of q, u, v; products u q, v q (and |u| u,
-// |u| v on the bottom layer); x-r2c keeping the Kx dealiased bins.
-// One workgroup per row (z, y).
-// ---------------------------------------------------------------------------
-struct RowJacArgs {
-  const void* Q; const void* U; const void* V;   // mixed (nz, ny, Kx)
-  void* UQ; void* VQ;                            // mixed (nz, ny, Kx)
-  void* DU; void* DV;                            // (ny, Kx), bottom layer
-  int drag;                                      // quadratic drag active
-};
-
-template <typename R, int EPT>
-__global__ void __launch_bounds__(256) k_row_jac(RowJacArgs a, Geom g, FftPlan px, const cplx<R>* __restrict__ twx,
-                          const cplx<R>* __restrict__ twr, R scale) {
-  using Cx = cplx<R>;
-  extern __shared__ unsigned char lds_raw[];
-  Cx* buf = reinterpret_cast<Cx*>(lds_raw);
-  const int N2 = g.N2;
-  const int row = blockIdx.x;
-  const int z = row / g.ny;
-  const int j = row % g.ny;
-  const bool bottom = a.drag && (z == g.nz - 1);
-  const long roff = (long)row * g.Kx;
-  const Cx* rows[3] = {reinterpret_cast<const Cx*>(a.Q) + roff, reinterpret_cast<const Cx*>(a.U) + roff,
-                       reinterpret_cast<const Cx*>(a.V) + roff};
-  for (int e = threadIdx.x; e < 3 * N2; e += blockDim.x) {
-    const int s = e / N2, k = e % N2;
-    buf[s * N2 + k] = c2r_pre<R>(rows[s], g.Kx, k, N2, twr, scale);
-  }
-  __syncthreads();
-  lds_fft<R, EPT>(buf, LdsBatch{3, N2, 1, false}, px, twx, +1);
-  for (int m = threadIdx.x; m < N2; m += blockDim.x) {
-    const Cx q = buf[m], u = buf[N2 + m], v = buf[2 * N2 + m];
-    buf[m] = cmk<Cx>(u.x * q.x, u.y * q.y);
-    buf[N2 + m] = cmk<Cx>(v.x * q.x, v.y * q.y);
-    if (bottom) {
-      const R r0 = sqrt(u.x * u.x + v.x * v.x), r1 = sqrt(u.y * u.y + v.y * v.y);
-      buf[2 * N2 + m] = cmk<Cx>(r0 * u.x, r1 * u.y);
-      buf[3 * N2 + m] = cmk<Cx>(r0 * v.x, r1 * v.y);
-    }
-  }
-  __syncthreads();
-  const int nout = bottom ? 4 : 2;
-  lds_fft<R, EPT>(buf, LdsBatch{nout, N2, 1, false}, px, twx, -1);
-  Cx* outs[4] = {reinterpret_cast<Cx*>(a.UQ) + roff, reinterpret_cast<Cx*>(a.VQ) + roff,
-                 reinterpret_cast<Cx*>(a.DU) + (long)j * g.Kx, reinterpret_cast<Cx*>(a.DV) + (long)j * g.Kx};
-  for (int e = threadIdx.x; e < nout * g.Kx; e += blockDim.x) {
-    const int s = e / g.Kx, k = e % g.Kx;
-    outs[s][k] = r2c_post<R>(buf + s * N2, k, N2, twr);
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Column pass, mixed -> spectral, and the rest of GetRHS
-// (QGN_Module.f90:244-247, 268-294): y-forward FFT of u q, v q (and the drag
-// products on the bottom layer), then
-//   RHS = -(i kx uq + i ky vq) - Ubar_k i kx q - (beta + Qy_k) i kx psi
-//         [- (A2 k2 + A8 k8) q  if is_explicit]
-//         [+ r0 Htot/H_nz k2 psi + C_d Htot/H_nz (i ky du - i kx dv)  on layer nz]
-// evaluated on the kept modes only (DeAlias of :294), (0,0) set to zero.
-// ---------------------------------------------------------------------------
-struct ColRhsArgs {
-  const void* UQ; const void* VQ; const void* DU; const void* DV;  // mixed
-  const double2* q;                                                // compact
-  const void* psi;                                                 // compact cplx<R>
-  void* out;                                                       // compact cplx<S>
-  const double* ubar; const double* qybar;
-  double beta, ekman, cdrag, A2, A8;
-  int explicit_visc, drag;
-};
-
-template <typename R, typename S, int EPT>
-__global__ void __launch_bounds__(256) k_col_rhs(ColRhsArgs a, Geom g, FftPlan py, const cplx<R>* __restrict__ twy, int C) {
-  using Cx = cplx<R>;
-  extern __shared__ unsigned char lds_raw[];
-  Cx* buf = reinterpret_cast<Cx*>(lds_raw);
-  const int i0 = blockIdx.x * C;
-  const int z = blockIdx.y;
-  const int nthr = blockDim.x;
-  const int tot = C * g.ny;
-  const int nown = C * g.Ky;
-  const bool bottom = (z == g.nz - 1);
-  const int nf = (bottom && a.drag) ? 4 : 2;
-  const LdsBatch B{C, 1, C, true};
-  Cx acc[EPT];
-#pragma unroll
-  for (int b = 0; b < EPT; ++b) acc[b] = cmk<Cx>((R)0, (R)0);
-  const void* srcs[4] = {a.UQ, a.VQ, a.DU, a.DV};
-  for (int f = 0; f < nf; ++f) {
-    const Cx* src = reinterpret_cast<const Cx*>(srcs[f]);
-    const long zoff = (f < 2) ? (long)z * g.ny : 0;
-    for (int e = threadIdx.x; e < tot; e += nthr) {
-      const int c = e % C, j = e / C, i = i0 + c;
-      buf[e] = (i < g.Kx) ? src[(zoff + j) * g.Kx + i] : cmk<Cx>((R)0, (R)0);
-    }
-    __syncthreads();
-    lds_fft<R, EPT>(buf, B, py, twy, -1);
-#pragma unroll
-    for (int b = 0; b < EPT; ++b) {
-      const int o = threadIdx.x + b * nthr;
-      if (o < nown) {
-        const int c = o % C, jc = o / C;
-        const int j = compact_row_to_full(jc, g);
-        const Cx v = buf[j * C + c];
-        const R kx = (R)kx_of(i0 + c, g), ky = (R)ky_of(j, g);
-        R cr;  // v * (coef * i): coef real
-        if (f == 0) cr = -kx;                 // -i kx uq
-        else if (f == 1) cr = -ky;            // -i ky vq
-        else if (f == 2) cr = (R)a.cdrag * ky;  // +Cd i ky du
-        else cr = -(R)a.cdrag * kx;           // -Cd i kx dv
-        acc[b] = cmk<Cx>(acc[b].x - cr * v.y, acc[b].y + cr * v.x);
-      }
-    }
-    __syncthreads();
-  }
-  const Cx* psi = reinterpret_cast<const Cx*>(a.psi);
-  cplx<S>* out = reinterpret_cast<cplx<S>*>(a.out);
-  const R ub = (R)a.ubar[z];
-  const R bq = (R)(a.beta + a.qybar[z]);
-#pragma unroll
-  for (int b = 0; b < EPT; ++b) {
-    const int o = threadIdx.x + b * nthr;
-    if (o < nown) {
-      const int c = o % C, jc = o / C, i = i0 + c;
-      if (i < g.Kx) {
-        const int j = compact_row_to_full(jc, g);
-        const long idx = ((long)z * g.Ky + jc) * g.Kx + i;
-        const Cx q = to_c<R>(a.q[idx]);
-        const Cx ps = psi[idx];
-        const R kx = (R)kx_of(i, g), ky = (R)ky_of(j, g);
-        Cx r = acc[b];
-        // - Ubar i kx q - (beta + Qy) (i kx psi)
-        r.x += ub * kx * q.y + bq * kx * ps.y;
-        r.y -= ub * kx * q.x + bq * kx * ps.x;
-        if (a.explicit_visc) {
-          const R k2 = kx * kx + ky * ky;
-          const R k4 = k2 * k2;
-          const R lam = (R)a.A2 * k2 + (R)a.A8 * (k4 * k4);
-          r.x -= lam * q.x;
-          r.y -= lam * q.y;
-        }
-        if (bottom) {
-          const R k2 = kx * kx + ky * ky;
-          r.x += (R)a.ekman * k2 * ps.x;
-          r.y += (R)a.ekman * k2 * ps.y;
-        }
-        if (i == 0 && j == 0) r = cmk<Cx>((R)0, (R)0);
-        out[idx] = cmk<cplx<S>>((S)r.x, (S)r.y);
-      }
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
 // IMEX stage combination (ARK43*.f90 stage blocks):
 //   q_tmp = Mq (q + dt * (sum_j ae_j N_j + sum_j ai_j L_j)),
 //   L_j = -(A2 k2 + A8L_k k8) Y_j, Mq = 1 / (1 + dt/4 (A2 k2 + A8L_k k8)),
@@ -399,7 +248,7 @@ struct StageArgs {
 
 template <typename S>
 __global__ void k_stage(StageArgs a, Geom g) {
-  const long plane = (long)g.Ky * g.Kx;
+  const long plane = (long)g.Ky * g.Kxp;
   const long n = plane * g.nz;
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
     const double2 q = a.q[e];
@@ -413,8 +262,8 @@ __global__ void k_stage(StageArgs a, Geom g) {
     if (a.implicit_visc) {
       const int z = (int)(e / plane);
       const long pt = e % plane;
-      const int i = (int)(pt % g.Kx);
-      const int j = compact_row_to_full((int)(pt / g.Kx), g);
+      const int i = (int)(pt % g.Kxp);
+      const int j = compact_row_to_full((int)(pt / g.Kxp), g);
       const double kx = kx_of(i, g), ky = ky_of(j, g);
       const double k2 = kx * kx + ky * ky;
       const double k4 = k2 * k2;
@@ -428,6 +277,85 @@ __global__ void k_stage(StageArgs a, Geom g) {
       mq = 1.0 / (1.0 + 0.25 * a.dt * lam);
     }
     a.out[e] = make_double2(mq * (q.x + a.dt * sx), mq * (q.y + a.dt * sy));
+  }
+}
+
+// Stage combination fused with GetPsi: a workgroup holds PP spectral points
+// x nz layers, one thread per (point, layer).  Each thread forms q_tmp of its
+// layer (as k_stage) and writes it; the workgroup then applies the
+// layer -> mode -> layer projection of k_project through LDS and writes psi.
+template <typename S, typename Tp>
+__global__ void __launch_bounds__(256) k_stage_proj(StageArgs a, Geom g, const double* __restrict__ HM,
+                                                    const double* __restrict__ MD, const double* __restrict__ evals,
+                                                    double inv_htot, int pp_per_block, cplx<Tp>* __restrict__ psi) {
+  extern __shared__ __align__(16) double lds_d[];
+  const int nz = g.nz;
+  double* sHM = lds_d;
+  double* sMD = sHM + nz * nz;
+  double2* sq = reinterpret_cast<double2*>(sMD + nz * nz);  // [nz][pp]
+  double2* sm = sq + nz * pp_per_block;                      // [nz][pp]
+  const int tid = threadIdx.x;
+  for (int e = tid; e < nz * nz; e += blockDim.x) { sHM[e] = HM[e]; sMD[e] = MD[e]; }
+  const long plane = (long)g.Ky * g.Kxp;
+  const int pp = tid % pp_per_block;
+  const int k = tid / pp_per_block;
+  const long pt = (long)blockIdx.x * pp_per_block + pp;
+  const bool active = (k < nz) && (pt < plane);
+  int i = 0, j = 0;
+  double k2 = 0.0;
+  if (active) {
+    i = (int)(pt % g.Kxp);
+    j = compact_row_to_full((int)(pt / g.Kxp), g);
+    const double kx = kx_of(i, g), ky = ky_of(j, g);
+    k2 = kx * kx + ky * ky;
+    const long e = k * plane + pt;
+    const double2 q = a.q[e];
+    double sx = 0.0, sy = 0.0;
+    for (int jj = 0; jj < a.ns; ++jj) {
+      const cplx<S> v = reinterpret_cast<const cplx<S>*>(a.N[jj])[e];
+      sx += a.ae[jj] * (double)v.x;
+      sy += a.ae[jj] * (double)v.y;
+    }
+    double mq = 1.0;
+    if (a.implicit_visc) {
+      const double k4 = k2 * k2;
+      const double a8 = a.a8l ? a.a8l[k] : a.A8;
+      const double lam = a.A2 * k2 + a8 * (k4 * k4);
+      for (int jj = 0; jj < a.ns; ++jj) {
+        const double2 y = jj == 0 ? q : a.Y[jj][e];
+        sx += a.ai[jj] * (-lam * y.x);
+        sy += a.ai[jj] * (-lam * y.y);
+      }
+      mq = 1.0 / (1.0 + 0.25 * a.dt * lam);
+    }
+    const double2 qt = make_double2(mq * (q.x + a.dt * sx), mq * (q.y + a.dt * sy));
+    a.out[e] = qt;
+    sq[k * pp_per_block + pp] = qt;
+  }
+  __syncthreads();
+  if (active) {
+    double2 acc = make_double2(0.0, 0.0);
+    for (int kk = 0; kk < nz; ++kk) {
+      const double w = sHM[k * nz + kk];
+      const double2 v = sq[kk * pp_per_block + pp];
+      acc.x += w * v.x;
+      acc.y += w * v.y;
+    }
+    const double den = -k2 + evals[k];
+    double2 pm = make_double2(acc.x * inv_htot / den, acc.y * inv_htot / den);
+    if ((i == 0 && j == 0) || i >= g.Kx) pm = make_double2(0.0, 0.0);
+    sm[k * pp_per_block + pp] = pm;
+  }
+  __syncthreads();
+  if (active) {
+    double2 acc = make_double2(0.0, 0.0);
+    for (int kk = 0; kk < nz; ++kk) {
+      const double w = sMD[k * nz + kk];
+      const double2 v = sm[kk * pp_per_block + pp];
+      acc.x += v.x * w;
+      acc.y += v.y * w;
+    }
+    psi[k * plane + pt] = cmk<cplx<Tp>>((Tp)acc.x, (Tp)acc.y);
   }
 }
 
@@ -450,7 +378,7 @@ struct UpdateArgs {
 
 template <typename S>
 __global__ void k_update(UpdateArgs a, Geom g) {
-  const long plane = (long)g.Ky * g.Kx;
+  const long plane = (long)g.Ky * g.Kxp;
   const long n = plane * g.nz;
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
     const double2 q = a.q[e];
@@ -458,8 +386,8 @@ __global__ void k_update(UpdateArgs a, Geom g) {
     const long pt = e % plane;
     double lam = 0.0;
     if (a.implicit_visc) {
-      const int i = (int)(pt % g.Kx);
-      const int j = compact_row_to_full((int)(pt / g.Kx), g);
+      const int i = (int)(pt % g.Kxp);
+      const int j = compact_row_to_full((int)(pt / g.Kxp), g);
       const double kx = kx_of(i, g), ky = ky_of(j, g);
       const double k2 = kx * kx + ky * ky;
       const double k4 = k2 * k2;
@@ -499,7 +427,7 @@ struct AxpyArgs {
 };
 template <typename S>
 __global__ void k_axpy(AxpyArgs a, Geom g) {
-  const long n = (long)g.Ky * g.Kx * g.nz;
+  const long n = (long)g.Ky * g.Kxp * g.nz;
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
     double2 r = a.q[e];
     for (int jj = 0; jj < a.nx; ++jj) {
@@ -515,7 +443,7 @@ __global__ void k_axpy(AxpyArgs a, Geom g) {
 template <typename S>
 __global__ void k_lincomb(const void* X0, const void* X1, const void* X2, const void* X3, double c0, double c1,
                           double c2, double c3, int nxs, void* out, Geom g) {
-  const long n = (long)g.Ky * g.Kx * g.nz;
+  const long n = (long)g.Ky * g.Kxp * g.nz;
   const cplx<S>* xs[4] = {reinterpret_cast<const cplx<S>*>(X0), reinterpret_cast<const cplx<S>*>(X1),
                           reinterpret_cast<const cplx<S>*>(X2), reinterpret_cast<const cplx<S>*>(X3)};
   const double cs[4] = {c0, c1, c2, c3};
@@ -527,40 +455,6 @@ __global__ void k_lincomb(const void* X0, const void* X1, const void* X2, const 
       ry += cs[jj] * (double)v.y;
     }
     reinterpret_cast<double2*>(out)[e] = make_double2(rx, ry);
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Max-norm of the physical error field: x-c2r of each mixed row, then
-// max |x| into one 64-bit slot (bit pattern of a non-negative double orders
-// like the value).  ARK43_QGLeith.f90:121-124 takes the max over all layers.
-// ---------------------------------------------------------------------------
-template <typename R, int EPT>
-__global__ void __launch_bounds__(256) k_row_maxabs(const cplx<R>* __restrict__ mixed, int Kin, Geom g, FftPlan px,
-                             const cplx<R>* __restrict__ twx, const cplx<R>* __restrict__ twr,
-                             unsigned long long* __restrict__ slot) {
-  using Cx = cplx<R>;
-  extern __shared__ unsigned char lds_raw[];
-  Cx* buf = reinterpret_cast<Cx*>(lds_raw);
-  __shared__ double red[16];
-  const int N2 = g.N2;
-  const Cx* row = mixed + (long)blockIdx.x * Kin;
-  for (int k = threadIdx.x; k < N2; k += blockDim.x) buf[k] = c2r_pre<R>(row, Kin, k, N2, twr, (R)1);
-  __syncthreads();
-  lds_fft<R, EPT>(buf, LdsBatch{1, N2, 1, false}, px, twx, +1);
-  double m = 0.0;
-  for (int k = threadIdx.x; k < N2; k += blockDim.x) {
-    const Cx v = buf[k];
-    m = fmax(m, fmax(fabs((double)v.x), fabs((double)v.y)));
-  }
-  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  if (lane == 0) red[wid] = m;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double mm = 0.0;
-    for (int w = 0; w < (int)(blockDim.x + 63) / 64; ++w) mm = fmax(mm, red[w]);
-    atomicMax(slot, (unsigned long long)__double_as_longlong(mm));
   }
 }
 
@@ -577,12 +471,12 @@ __global__ void __launch_bounds__(256) k_row_maxabs(const cplx<R>* __restrict__ 
 __global__ void k_leith_partial(const double2* __restrict__ q, Geom g, double* __restrict__ partial, int nblk) {
   __shared__ double red[16];
   const int z = blockIdx.y;
-  const long plane = (long)g.Ky * g.Kx;
+  const long plane = (long)g.Ky * g.Kxp;
   double s = 0.0;
   for (long pt = blockIdx.x * (long)blockDim.x + threadIdx.x; pt < plane; pt += (long)nblk * blockDim.x) {
-    const int i = (int)(pt % g.Kx);
-    if (i == 0) continue;
-    const int j = compact_row_to_full((int)(pt / g.Kx), g);
+    const int i = (int)(pt % g.Kxp);
+    if (i == 0 || i >= g.Kx) continue;
+    const int j = compact_row_to_full((int)(pt / g.Kxp), g);
     const double kx = kx_of(i, g), ky = ky_of(j, g);
     const double k2 = kx * kx + ky * ky;
     const double k4 = k2 * k2;
@@ -604,15 +498,15 @@ __global__ void k_leith_final(const double2* __restrict__ q, Geom g, const doubl
                               double c12, double* __restrict__ a8l) {
   __shared__ double red[16];
   const int z = blockIdx.x;
-  const long plane = (long)g.Ky * g.Kx;
+  const long plane = (long)g.Ky * g.Kxp;
   double s = 0.0;
   for (int j = threadIdx.x; j < g.ny; j += blockDim.x) {
     const int jc = full_row_to_compact(j, g);
     const int jm = (g.ny - j) % g.ny;
     const int jmc = full_row_to_compact(jm, g);
     double2 a = make_double2(0.0, 0.0), b = make_double2(0.0, 0.0);
-    if (jc >= 0) a = q[z * plane + (long)jc * g.Kx];
-    if (jmc >= 0) b = q[z * plane + (long)jmc * g.Kx];
+    if (jc >= 0) a = q[z * plane + (long)jc * g.Kxp];
+    if (jmc >= 0) b = q[z * plane + (long)jmc * g.Kxp];
     if (j == 0) a = b = make_double2(0.0, 0.0);  // (0,0) is always zero
     const double sx = 0.5 * (a.x + b.x), sy = 0.5 * (a.y - b.y);
     const double ky = ky_of(j, g);
@@ -642,7 +536,7 @@ template <typename R, int EPT>
 __global__ void __launch_bounds__(256) k_row_r2c(const R* __restrict__ in, cplx<R>* __restrict__ out, int Kout, Geom g, FftPlan px,
                           const cplx<R>* __restrict__ twx, const cplx<R>* __restrict__ twr) {
   using Cx = cplx<R>;
-  extern __shared__ unsigned char lds_raw[];
+  extern __shared__ __align__(16) unsigned char lds_raw[];
   Cx* buf = reinterpret_cast<Cx*>(lds_raw);
   const int N2 = g.N2;
   const Cx* row = reinterpret_cast<const Cx*>(in + (long)blockIdx.x * g.nx);
@@ -658,7 +552,7 @@ template <typename R, int EPT>
 __global__ void __launch_bounds__(256) k_row_c2r(const cplx<R>* __restrict__ in, int Kin, R* __restrict__ out, R scale, Geom g,
                           FftPlan px, const cplx<R>* __restrict__ twx, const cplx<R>* __restrict__ twr) {
   using Cx = cplx<R>;
-  extern __shared__ unsigned char lds_raw[];
+  extern __shared__ __align__(16) unsigned char lds_raw[];
   Cx* buf = reinterpret_cast<Cx*>(lds_raw);
   const int N2 = g.N2;
   const Cx* row = in + (long)blockIdx.x * Kin;
@@ -674,7 +568,7 @@ template <typename R, int EPT>
 __global__ void __launch_bounds__(256) k_col_fft(const cplx<R>* __restrict__ in, cplx<R>* __restrict__ out, int K, int sign, Geom g,
                           FftPlan py, const cplx<R>* __restrict__ twy, int C) {
   using Cx = cplx<R>;
-  extern __shared__ unsigned char lds_raw[];
+  extern __shared__ __align__(16) unsigned char lds_raw[];
   Cx* buf = reinterpret_cast<Cx*>(lds_raw);
   const int i0 = blockIdx.x * C;
   const long zoff = (long)blockIdx.y * g.ny;

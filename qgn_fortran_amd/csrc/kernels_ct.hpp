@@ -1,0 +1,349 @@
+// The FFT passes of GetRHS with compile-time sizes (array layouts in
+// kernels.hpp).  NY = ny for the column passes, N2 = nx/2 for the row
+// passes; C columns per workgroup, NTHR threads.  Twiddle tables are copied
+// into LDS beside the data.
+//
+// The Jacobian is formed in advective form, J = u q_x + v q_y, where the
+// reference uses the flux form i kx (u q)^ + i ky (v q)^
+// (QGN_Module.f90:237-247).  Both are the 2/3-truncated Galerkin product of
+// band-limited fields; they differ by the truncation of q (u_x + v_y), and
+// u = -i ky psi, v = i kx psi make that divergence zero exactly in spectral
+// space, so the two agree to round-off (checked against the oracle's flux
+// form in the GPU parity tests).  The advective form leaves one product
+// field per row instead of two, so the column pass needs no accumulators.
+#pragma once
+#include "fft_ct.hpp"
+#include "kernels.hpp"
+
+namespace qgn {
+
+// P1: y-inverse FFT of up to four fields built from compact inputs.
+template <typename R, int NY, int C, int NTHR>
+__global__ void __launch_bounds__(NTHR) k_col_inv_ct(ColInvArgs a, Geom g, const cplx<R>* __restrict__ twy) {
+  using Cx = cplx<R>;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  Cx* buf = reinterpret_cast<Cx*>(lds_raw);
+  Cx* tw = buf + C * NY;
+  using L = CtLayout<C, 1, C, true>;
+  constexpr int TOT = C * NY;
+  const int i0 = blockIdx.x * C;
+  const int z = blockIdx.y;
+  load_table<R>(tw, twy, ct_tw_size(NY));
+  {
+    const int f = blockIdx.z;   // one field per grid slice
+    const int mult = a.mult[f];
+    const bool isd = a.src_is_double[f];
+    const void* src = a.src[f];
+    for (int e = threadIdx.x; e < TOT; e += NTHR) {
+      const int c = e % C, j = e / C, i = i0 + c;
+      const int jc = full_row_to_compact(j, g);
+      Cx v = cmk<Cx>((R)0, (R)0);
+      if (jc >= 0) {
+        const long idx = ((long)z * g.Ky + jc) * g.Kxp + i;
+        if (isd) v = to_c<R>(reinterpret_cast<const double2*>(src)[idx]);
+        else v = reinterpret_cast<const Cx*>(src)[idx];
+        if (mult == 1) {          // -i ky
+          const R ky = (R)ky_of(j, g);
+          v = cmk<Cx>(ky * v.y, -ky * v.x);
+        } else if (mult == 2) {   // +i kx
+          const R kx = (R)kx_of(i, g);
+          v = cmk<Cx>(-kx * v.y, kx * v.x);
+        } else if (mult == 3) {   // +i ky
+          const R ky = (R)ky_of(j, g);
+          v = cmk<Cx>(-ky * v.y, ky * v.x);
+        }
+      }
+      buf[e] = v;
+    }
+    __syncthreads();
+    ct_fft<R, NY, L, NTHR>(buf, tw, +1);
+    Cx* dst = reinterpret_cast<Cx*>(a.dst[f]);
+    for (int e = threadIdx.x; e < TOT; e += NTHR) {
+      const int c = e % C, j = e / C;
+      dst[((long)z * NY + j) * g.Kxp + i0 + c] = buf[e];
+    }
+    __syncthreads();
+  }
+}
+
+// Padding of the contiguous-row LDS layout (one slot per 32 FP32 / 16 FP64
+// complex values; chosen by measured bank conflicts).
+template <typename R> constexpr int row_pad_shift() { return sizeof(R) == 4 ? 5 : 4; }
+template <typename R> constexpr int row_lds_slots(int n) { return n + (n >> row_pad_shift<R>()) + 1; }
+
+// x-r2c post-processing on a padded LDS sequence starting at slot `base`.
+template <typename R, int PS>
+__device__ __forceinline__ cplx<R> r2c_post_pad(const cplx<R>* buf, int base, int k, int N2,
+                                                const cplx<R>* __restrict__ twr) {
+  using Cx = cplx<R>;
+  const Cx a = buf[padded<PS>(base + (k == N2 ? 0 : k))];
+  const Cx b = buf[padded<PS>(base + (k == 0 ? 0 : N2 - k))];
+  const Cx e = cmk<Cx>((R)0.5 * (a.x + b.x), (R)0.5 * (a.y - b.y));
+  const Cx dd = cmk<Cx>((R)0.5 * (a.x - b.x), (R)0.5 * (a.y + b.y));
+  const Cx o = cmk<Cx>(dd.y, -dd.x);
+  return cadd(e, cmul(twr[k], o));
+}
+
+// P2: a persistent grid; each workgroup takes groups of ROWS consecutive rows
+// (z, y) (ny is a multiple of ROWS, so a group never spans two layers) and
+// prefetches the next group's bins into registers while it transforms the
+// current one.  LDS: 4 padded sequences of N2 per row, the per-pass FFT
+// tables and the real-FFT post-processing table.
+template <typename R, int N2, int ROWS, int NTHR>
+__global__ void __launch_bounds__(NTHR) k_row_jac_ct(RowJacArgs a, Geom g, const cplx<R>* __restrict__ twx,
+                                                     const cplx<R>* __restrict__ twr, R scale, int ngroups) {
+  using Cx = cplx<R>;
+  constexpr int PS = row_pad_shift<R>();
+  constexpr int EL = (ROWS * N2 + NTHR - 1) / NTHR;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  Cx* buf = reinterpret_cast<Cx*>(lds_raw);
+  Cx* ts = buf + row_lds_slots<R>(ROWS * 4 * N2);   // per-pass tables of the N2-point FFT
+  Cx* tr = ts + ct_tw_size(N2);                     // exp(-2 pi i k / nx), k <= N2
+  load_table<R>(ts, twx, ct_tw_size(N2));
+  load_table<R>(tr, twr, N2 + 1);
+  const int Kx = g.Kx;
+  const long Kxp = g.Kxp;
+  const Cx* srcs[4] = {reinterpret_cast<const Cx*>(a.U), reinterpret_cast<const Cx*>(a.V),
+                       reinterpret_cast<const Cx*>(a.QX), reinterpret_cast<const Cx*>(a.QY)};
+  const Cx zero = cmk<Cx>((R)0, (R)0);
+  Cx ra[4][EL], rb[4][EL];
+  auto fetch = [&](int grp) {
+    const long row0 = (long)grp * ROWS;
+#pragma unroll
+    for (int b = 0; b < EL; ++b) {
+      const int e = threadIdx.x + b * NTHR;
+      const int r = e / N2, k = e - r * N2;
+      const bool ok = (ROWS * N2) % NTHR == 0 || e < ROWS * N2;
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const Cx* p = srcs[s] + (row0 + r) * Kxp;
+        ra[s][b] = (ok && k < Kx) ? p[k] : zero;
+        rb[s][b] = (ok && N2 - k < Kx) ? p[N2 - k] : zero;
+      }
+    }
+  };
+  int grp = blockIdx.x;
+  if (grp < ngroups) fetch(grp);
+  __syncthreads();
+  for (; grp < ngroups; grp += gridDim.x) {
+    const int row0 = grp * ROWS;
+    const int z = row0 / g.ny;
+    const int j0 = row0 - z * g.ny;
+    const bool bottom = a.drag && (z == g.nz - 1);
+#pragma unroll
+    for (int b = 0; b < EL; ++b) {
+      const int e = threadIdx.x + b * NTHR;
+      if ((ROWS * N2) % NTHR == 0 || e < ROWS * N2) {
+        const int r = e / N2, k = e - r * N2;
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+          buf[padded<PS>((4 * r + s) * N2 + k)] = c2r_pre_ab<R>(ra[s][b], rb[s][b], k, tr, scale);
+      }
+    }
+    if (grp + (int)gridDim.x < ngroups) fetch(grp + gridDim.x);   // in flight during the FFTs below
+    __syncthreads();
+    ct_fft<R, N2, CtLayout<4 * ROWS, N2, 1, false, PS>, NTHR>(buf, ts, +1);
+    // physical values: element m of a sequence holds x[2m] + i x[2m+1]
+    for (int e = threadIdx.x; e < ROWS * N2; e += NTHR) {
+      const int r = e / N2, m = e - r * N2;
+      const int b0 = 4 * r * N2 + m;
+      const Cx u = buf[padded<PS>(b0)], v = buf[padded<PS>(b0 + N2)];
+      const Cx qx = buf[padded<PS>(b0 + 2 * N2)], qy = buf[padded<PS>(b0 + 3 * N2)];
+      buf[padded<PS>(b0)] = cmk<Cx>(u.x * qx.x + v.x * qy.x, u.y * qx.y + v.y * qy.y);
+      if (bottom) {   // URMS = sqrt(u^2 + v^2) (QGN_Module.f90:285-289)
+        const R r0 = sqrt(u.x * u.x + v.x * v.x), r1 = sqrt(u.y * u.y + v.y * v.y);
+        buf[padded<PS>(b0 + N2)] = cmk<Cx>(r0 * u.x, r1 * u.y);
+        buf[padded<PS>(b0 + 2 * N2)] = cmk<Cx>(r0 * v.x, r1 * v.y);
+      }
+    }
+    __syncthreads();
+    if (bottom) ct_fft<R, N2, CtLayout<4 * ROWS, N2, 1, false, PS>, NTHR>(buf, ts, -1);
+    else ct_fft<R, N2, CtLayout<ROWS, 4 * N2, 1, false, PS>, NTHR>(buf, ts, -1);
+    Cx* jo = reinterpret_cast<Cx*>(a.J) + (long)row0 * Kxp;
+    for (int e = threadIdx.x; e < ROWS * Kx; e += NTHR) {
+      const int r = e / Kx, k = e - r * Kx;
+      jo[(long)r * Kxp + k] = r2c_post_pad<R, PS>(buf, 4 * r * N2, k, N2, tr);
+    }
+    if (bottom) {
+      Cx* du = reinterpret_cast<Cx*>(a.DU) + (long)j0 * Kxp;
+      Cx* dv = reinterpret_cast<Cx*>(a.DV) + (long)j0 * Kxp;
+      for (int e = threadIdx.x; e < ROWS * Kx; e += NTHR) {
+        const int r = e / Kx, k = e - r * Kx;
+        du[(long)r * Kxp + k] = r2c_post_pad<R, PS>(buf, (4 * r + 1) * N2, k, N2, tr);
+        dv[(long)r * Kxp + k] = r2c_post_pad<R, PS>(buf, (4 * r + 2) * N2, k, N2, tr);
+      }
+    }
+    __syncthreads();   // buf is refilled by the next group
+  }
+}
+
+// P3: y-forward FFT of J, then
+//   RHS = -J_hat - Ubar_k i kx q - (beta + Qy_k) i kx psi
+//         [- (A2 k2 + A8 k8) q  if is_explicit]  [+ r0 Htot/H_nz k2 psi on layer nz]
+// on the kept modes (DeAlias, QGN_Module.f90:294), (0,0) and padding zero.
+template <typename R, typename S, int NY, int C, int NTHR>
+__global__ void __launch_bounds__(NTHR) k_col_rhs_ct(ColRhsArgs a, Geom g, const cplx<R>* __restrict__ twy) {
+  using Cx = cplx<R>;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  Cx* buf = reinterpret_cast<Cx*>(lds_raw);
+  Cx* tw = buf + C * NY;
+  using L = CtLayout<C, 1, C, true>;
+  constexpr int TOT = C * NY;
+  const int i0 = blockIdx.x * C;
+  const int z = blockIdx.y;
+  load_table<R>(tw, twy, ct_tw_size(NY));
+  const Cx* src = reinterpret_cast<const Cx*>(a.J);
+  for (int e = threadIdx.x; e < TOT; e += NTHR) {
+    const int c = e % C, j = e / C;
+    buf[e] = src[((long)z * NY + j) * g.Kxp + i0 + c];
+  }
+  __syncthreads();
+  ct_fft<R, NY, L, NTHR>(buf, tw, -1);
+  const Cx* psi = reinterpret_cast<const Cx*>(a.psi);
+  cplx<S>* out = reinterpret_cast<cplx<S>*>(a.out);
+  const R ub = (R)a.ubar[z];
+  const R bq = (R)(a.beta + a.qybar[z]);
+  const bool bottom = (z == g.nz - 1);
+  const int nown = C * g.Ky;
+  for (int o = threadIdx.x; o < nown; o += NTHR) {
+    const int c = o % C, jc = o / C, i = i0 + c;
+    const int j = compact_row_to_full(jc, g);
+    const long idx = ((long)z * g.Ky + jc) * g.Kxp + i;
+    const Cx v = buf[j * C + c];
+    const Cx q = to_c<R>(a.q[idx]);
+    const Cx ps = psi[idx];
+    const R kx = (R)kx_of(i, g), ky = (R)ky_of(j, g);
+    Cx r = cmk<Cx>(-v.x, -v.y);
+    r.x += ub * kx * q.y + bq * kx * ps.y;
+    r.y -= ub * kx * q.x + bq * kx * ps.x;
+    if (a.explicit_visc) {
+      const R k2 = kx * kx + ky * ky;
+      const R k4 = k2 * k2;
+      const R lam = (R)a.A2 * k2 + (R)a.A8 * (k4 * k4);
+      r.x -= lam * q.x;
+      r.y -= lam * q.y;
+    }
+    if (bottom) {
+      const R k2 = kx * kx + ky * ky;
+      r.x += (R)a.ekman * k2 * ps.x;
+      r.y += (R)a.ekman * k2 * ps.y;
+    }
+    if ((i == 0 && j == 0) || i >= g.Kx) r = cmk<Cx>((R)0, (R)0);
+    out[idx] = cmk<cplx<S>>((S)r.x, (S)r.y);
+  }
+}
+
+// Bottom-layer quadratic drag (QGN_Module.f90:284-292): y-forward FFT of
+// |u| u and |u| v, then out(nz) += C_d Htot/H_nz (i ky du - i kx dv).
+template <typename R, typename S, int NY, int C, int NTHR>
+__global__ void __launch_bounds__(NTHR) k_col_drag_ct(DragArgs a, Geom g, const cplx<R>* __restrict__ twy) {
+  using Cx = cplx<R>;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  Cx* bu = reinterpret_cast<Cx*>(lds_raw);
+  Cx* bv = bu + C * NY;
+  Cx* tw = bv + C * NY;
+  using L = CtLayout<C, 1, C, true>;
+  constexpr int TOT = C * NY;
+  const int i0 = blockIdx.x * C;
+  const int z = g.nz - 1;
+  load_table<R>(tw, twy, ct_tw_size(NY));
+  const Cx* du = reinterpret_cast<const Cx*>(a.DU);
+  const Cx* dv = reinterpret_cast<const Cx*>(a.DV);
+  for (int e = threadIdx.x; e < TOT; e += NTHR) {
+    const int c = e % C, j = e / C;
+    bu[e] = du[(long)j * g.Kxp + i0 + c];
+    bv[e] = dv[(long)j * g.Kxp + i0 + c];
+  }
+  __syncthreads();
+  ct_fft<R, NY, L, NTHR>(bu, tw, -1);
+  ct_fft<R, NY, L, NTHR>(bv, tw, -1);
+  cplx<S>* out = reinterpret_cast<cplx<S>*>(a.out);
+  const R cd = (R)a.cdrag;
+  const int nown = C * g.Ky;
+  for (int o = threadIdx.x; o < nown; o += NTHR) {
+    const int c = o % C, jc = o / C, i = i0 + c;
+    if (i >= g.Kx) continue;
+    const int j = compact_row_to_full(jc, g);
+    if (i == 0 && j == 0) continue;
+    const long idx = ((long)z * g.Ky + jc) * g.Kxp + i;
+    const Cx x = bu[j * C + c], y = bv[j * C + c];
+    const R kx = (R)kx_of(i, g), ky = (R)ky_of(j, g);
+    // i ky x - i kx y = i (ky x - kx y)
+    const R px = ky * x.x - kx * y.x, py = ky * x.y - kx * y.y;
+    cplx<S> r = out[idx];
+    r.x += (S)(-cd * py);
+    r.y += (S)(cd * px);
+    out[idx] = r;
+  }
+}
+
+// Max-norm of the c2r of mixed rows: max |x| over the grid into one 64-bit
+// slot (a non-negative double's bit pattern orders like its value; a NaN
+// anywhere stores a NaN, which orders above +Inf).  ARK43_QGLeith.f90:121-124
+// take the max over all layers.
+template <typename R, int N2, int ROWS, int NTHR>
+__global__ void __launch_bounds__(NTHR) k_row_maxabs_ct(const cplx<R>* __restrict__ mixed, int Kin, int stride,
+                                                        Geom g, const cplx<R>* __restrict__ twx,
+                                                        const cplx<R>* __restrict__ twr,
+                                                        unsigned long long* __restrict__ partial) {
+  using Cx = cplx<R>;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  constexpr int PS = row_pad_shift<R>();
+  Cx* buf = reinterpret_cast<Cx*>(lds_raw);
+  Cx* ts = buf + row_lds_slots<R>(ROWS * N2);
+  Cx* tr = ts + ct_tw_size(N2);
+  __shared__ double red[16];
+  __shared__ int bad;
+  if (threadIdx.x == 0) bad = 0;
+  load_table<R>(ts, twx, ct_tw_size(N2));
+  load_table<R>(tr, twr, N2 + 1);
+  __syncthreads();
+  const long row0 = (long)blockIdx.x * ROWS;
+  for (int e = threadIdx.x; e < ROWS * N2; e += NTHR) {
+    const int r = e / N2, k = e - r * N2;
+    buf[padded<PS>(e)] = c2r_pre<R>(mixed + (row0 + r) * stride, Kin, k, N2, tr, (R)1);
+  }
+  __syncthreads();
+  ct_fft<R, N2, CtLayout<ROWS, N2, 1, false, PS>, NTHR>(buf, ts, +1);
+  double m = 0.0;
+  bool nan = false;
+  for (int e = threadIdx.x; e < ROWS * N2; e += NTHR) {
+    const Cx v = buf[padded<PS>(e)];
+    nan |= (v.x != v.x) || (v.y != v.y);
+    m = fmax(m, fmax(fabs((double)v.x), fabs((double)v.y)));
+  }
+  if (nan) bad = 1;
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) red[wid] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double mm = 0.0;
+    for (int w = 0; w < (NTHR + 63) / 64; ++w) mm = fmax(mm, red[w]);
+    unsigned long long bits = (unsigned long long)__double_as_longlong(mm);
+    if (bad) bits = 0x7FF8000000000000ull;   // NaN orders above every finite value and +Inf
+    partial[blockIdx.x] = bits;
+  }
+}
+
+// Final max over the per-block partials (one workgroup, fixed order).
+__global__ void __launch_bounds__(1024) k_max_final(const unsigned long long* __restrict__ partial, int n,
+                                                    unsigned long long* __restrict__ slot) {
+  __shared__ unsigned long long red[16];
+  unsigned long long m = 0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) m = partial[i] > m ? partial[i] : m;
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_down(m, off, 64);
+    m = o > m ? o : m;
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) red[wid] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long mm = 0;
+    for (int w = 0; w < (int)(blockDim.x + 63) / 64; ++w) mm = red[w] > mm ? red[w] : mm;
+    slot[0] = mm;
+  }
+}
+
+}  // namespace qgn

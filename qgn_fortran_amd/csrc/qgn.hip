@@ -2,6 +2,7 @@
 // time steppers and the C-ABI declared in include/qgn.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -10,7 +11,7 @@
 #include <utility>
 #include <vector>
 
-#include "kernels.hpp"
+#include "kernels_ct.hpp"
 #include "qgn.h"
 
 using namespace qgn;
@@ -87,10 +88,32 @@ std::vector<double> twiddle_table(int n, int count) {
 }
 
 template <typename R> struct Tw {
-  cplx<R>* y = nullptr;  // length ny
-  cplx<R>* x = nullptr;  // length nx/2
-  cplx<R>* r = nullptr;  // length nx/2+1, exp(-2 pi i k / nx)
+  cplx<R>* y = nullptr;   // length ny, exp(-2 pi i m / ny)
+  cplx<R>* x = nullptr;   // length nx/2
+  cplx<R>* r = nullptr;   // length nx/2+1, exp(-2 pi i k / nx)
+  cplx<R>* sy = nullptr;  // per-pass tables of the compiled ny-point FFT
+  cplx<R>* sx = nullptr;  // per-pass tables of the compiled nx/2-point FFT
 };
+
+// Per-pass twiddles in the layout of ct_tw_offset (fft_ct.hpp).
+std::vector<double> stage_table(int n) {
+  const CtPlan PL = make_ct_plan(n);
+  std::vector<double> t;
+  if (PL.np <= 0) return t;
+  int P = 1;
+  for (int q = 0; q < PL.np; ++q) {
+    const int R = PL.rad[q];
+    if (P > 1)
+      for (int j = 1; j < R; ++j)
+        for (int k = 0; k < P; ++k) {
+          const long double a = -2.0L * kPi * (long double)(j * k) / (long double)(P * R);
+          t.push_back((double)cosl(a));
+          t.push_back((double)sinl(a));
+        }
+    P *= R;
+  }
+  return t;
+}
 
 template <typename R>
 int upload_tw(const std::vector<double>& src, cplx<R>** dst) {
@@ -107,6 +130,9 @@ int make_tw(int nx, int ny, Tw<R>* tw) {
   CHK(upload_tw<R>(twiddle_table(ny, ny), &tw->y));
   CHK(upload_tw<R>(twiddle_table(nx / 2, nx / 2), &tw->x));
   CHK(upload_tw<R>(twiddle_table(nx, nx / 2 + 1), &tw->r));
+  const std::vector<double> sy = stage_table(ny), sx = stage_table(nx / 2);
+  if (!sy.empty()) CHK(upload_tw<R>(sy, &tw->sy));
+  if (!sx.empty()) CHK(upload_tw<R>(sx, &tw->sx));
   return 0;
 }
 
@@ -115,7 +141,9 @@ void free_tw(Tw<R>* tw) {
   if (tw->y) (void)hipFree(tw->y);
   if (tw->x) (void)hipFree(tw->x);
   if (tw->r) (void)hipFree(tw->r);
-  tw->y = tw->x = tw->r = nullptr;
+  if (tw->sy) (void)hipFree(tw->sy);
+  if (tw->sx) (void)hipFree(tw->sx);
+  tw->y = tw->x = tw->r = tw->sy = tw->sx = nullptr;
 }
 
 int col_tile(int ny) {
@@ -137,6 +165,7 @@ Geom make_geom(int nx, int ny, int nz, double Lx, double Ly) {
   g.nxh = nx / 2 + 1;
   g.N2 = nx / 2;
   g.Kx = nx / 3;
+  g.Kxp = ((g.Kx + 15) / 16) * 16;
   g.ya = ny / 3;
   g.yb = 2 * (ny / 3) + 1;
   if (g.yb > ny) g.yb = ny;
@@ -219,6 +248,7 @@ struct qgn_ctx {
   double *HM = nullptr, *MD = nullptr, *evals = nullptr, *ubar = nullptr, *qybar = nullptr;
   double *a8l = nullptr, *partial = nullptr;
   int leith_nblk = 64;
+  int ncu = 256;         // compute units of the device
   double c12 = 0.0;
   double Hnz = 1.0;
   // compact arrays
@@ -231,9 +261,10 @@ struct qgn_ctx {
   double2* Y[6] = {};    // Y[0] unused
   void* hist[3] = {};    // AB3 / BSRK3 saved RHS, cplx<R>
   // mixed arrays, cplx<R>
-  void *Qm = nullptr, *Um = nullptr, *Vm = nullptr, *UQm = nullptr, *VQm = nullptr, *DUm = nullptr, *DVm = nullptr;
+  void *Um = nullptr, *Vm = nullptr, *QXm = nullptr, *QYm = nullptr, *Jm = nullptr, *DUm = nullptr, *DVm = nullptr;
   double2* Em = nullptr; // FP64 mixed for the error norm
   unsigned long long* errslot = nullptr;
+  unsigned long long* errpart = nullptr;  // per-block partial maxima (ny*nz entries at most)
   unsigned long long* h_errslot = nullptr;  // pinned
   // full-layout scratch for the boundary
   double2 *fa = nullptr, *fb = nullptr;
@@ -295,6 +326,51 @@ int grid1d(long n, int threads = 256) {
   return (int)b;
 }
 
+template <int... Ns> struct IntList {};
+using CtSizes = IntList<32, 48, 64, 96, 128, 192, 256, 384, 512, 768, 1024, 1536, 2048>;
+
+// Calls f(std::integral_constant<int, n>) when n is one of the compiled sizes.
+template <class F, int... Ns>
+bool with_size(int n, F&& f, IntList<Ns...>) {
+  bool hit = false;
+  ((n == Ns ? (f(std::integral_constant<int, Ns>{}), hit = true) : false), ...);
+  return hit;
+}
+
+constexpr int pow2_floor(int x) {
+  int p = 1;
+  while (p * 2 <= x) p *= 2;
+  return p;
+}
+
+// Column tiles: C adjacent kx columns per workgroup, 8 elements per thread
+// (measured best on MI355X at 1024 points: 8 FP32 / 4 FP64 columns, i.e.
+// 64-byte row segments, two workgroups per CU).
+template <typename R, int NY> struct ColCfg {
+  static constexpr bool f32 = std::is_same<R, float>::value;
+  // a power of two, so it divides the 16-column padding of Kxp
+  static constexpr int C = f32 ? (NY <= 1024 ? 8 : pow2_floor(8192 / NY)) : (NY <= 2048 ? 4 : 2);
+  static constexpr int NT0 = C * NY / 8;
+  static constexpr int NTHR = NT0 < 64 ? 64 : ((NT0 + 63) / 64) * 64;
+  static constexpr size_t LDS = (size_t)(C * NY + ct_tw_size(NY)) * sizeof(cplx<R>);
+};
+// Row passes: ROWS rows per workgroup (every compiled ny is a multiple of
+// 16, so a workgroup stays inside one layer), 8 elements per thread.
+template <typename R, int N2, int NSEQ> struct RowCfg {
+  static constexpr int ROWS = N2 >= 512 ? 1 : (512 / N2 > 16 ? 16 : pow2_floor(512 / N2));
+  static constexpr int NT0 = ROWS * NSEQ * N2 / 8;
+  static constexpr int NTHR0 = NT0 < 64 ? 64 : ((NT0 + 63) / 64) * 64;
+  static constexpr int NTHR = NTHR0 > 1024 ? 1024 : NTHR0;
+};
+
+template <typename K>
+int allow_lds(K kernel, size_t bytes) {
+  if (bytes > 65536)
+    HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)bytes));
+  return 0;
+}
+
 template <typename R> const Tw<R>& tw_of(qgn_ctx* c);
 template <> const Tw<double>& tw_of<double>(qgn_ctx* c) { return c->twd; }
 template <> const Tw<float>& tw_of<float>(qgn_ctx* c) { return c->twf; }
@@ -306,7 +382,7 @@ int launch_project(qgn_ctx* c, const double2* q, void* psi) {
   int pp = 256 / nz;
   if (pp < 1) pp = 1;
   const int threads = ((pp * nz + 63) / 64) * 64;
-  const long plane = (long)c->g.Ky * c->g.Kx;
+  const long plane = (long)c->g.Ky * c->g.Kxp;
   const int blocks = (int)((plane + pp - 1) / pp);
   const size_t lds = 2 * (size_t)nz * nz * sizeof(double) + 2 * (size_t)nz * pp * sizeof(double2);
   return timed(c, QGN_K_PROJECT, [&] {
@@ -315,68 +391,112 @@ int launch_project(qgn_ctx* c, const double2* q, void* psi) {
   });
 }
 
-// y-inverse FFT of up to 3 compact fields into mixed arrays
-template <typename R>
-int launch_col_inv(qgn_ctx* c, const ColInvArgs& a) {
-  const int C = col_tile(c->g.ny);
-  dim3 grid((c->g.Kx + C - 1) / C, c->g.nz);
-  const size_t lds = (size_t)C * c->g.ny * sizeof(cplx<R>);
-  const Tw<R>& tw = tw_of<R>(c);
-  return timed(c, QGN_K_COL_INV, [&] {
-    hipLaunchKernelGGL((k_col_inv<R, kEPT>), grid, dim3(kColThreads), lds, c->stream, a, c->g, c->py, tw.y, C);
-  });
+bool rhs_size_ok(int nx, int ny) {
+  auto nop = [](auto) {};
+  // every compiled size is a multiple of 16, which the row passes rely on
+  return nx % 2 == 0 && nx <= 2048 && ny <= 2048 && ny % 16 == 0 && with_size(nx / 2, nop, CtSizes{}) &&
+         with_size(ny, nop, CtSizes{});
 }
 
+// P1: y-inverse FFT of up to 4 compact fields into mixed arrays
 template <typename R>
-int rhs_impl(qgn_ctx* c, const double2* qin, void* out) {
+int launch_col_inv(qgn_ctx* c, const ColInvArgs& a) {
+  const Geom& g = c->g;
+  const Tw<R>& tw = tw_of<R>(c);
+  int rc = 0;
+  bool hit = with_size(g.ny, [&](auto nyc) {
+    constexpr int NY = decltype(nyc)::value;
+    using Cfg = ColCfg<R, NY>;
+    auto kern = k_col_inv_ct<R, NY, Cfg::C, Cfg::NTHR>;
+    rc = allow_lds(kern, Cfg::LDS);
+    if (rc) return;
+    dim3 grid(g.Kxp / Cfg::C, g.nz, a.nf);
+    rc = timed(c, QGN_K_COL_INV, [&] {
+      hipLaunchKernelGGL(kern, grid, dim3(Cfg::NTHR), Cfg::LDS, c->stream, a, g, tw.sy);
+    });
+  }, CtSizes{});
+  if (!hit) return fail(QGN_EINVAL, "ny not supported by the RHS passes");
+  return rc;
+}
+
+// GetRHS on compact arrays: out = RHS(qin)
+template <typename R>
+int rhs_impl(qgn_ctx* c, const double2* qin, void* out, bool psi_ready) {
   const Geom& g = c->g;
   const Tw<R>& tw = tw_of<R>(c);
   void* psi = std::is_same<R, double>::value ? (void*)c->psi64 : c->psi;
-  CHK(launch_project<R>(c, qin, psi));
+  const int isd = std::is_same<R, double>::value;
+  if (!psi_ready) CHK(launch_project<R>(c, qin, psi));
   ColInvArgs ca{};
-  ca.nf = 3;
-  ca.src[0] = qin; ca.src_is_double[0] = 1; ca.mult[0] = 0; ca.dst[0] = c->Qm;
-  ca.src[1] = psi; ca.src_is_double[1] = std::is_same<R, double>::value; ca.mult[1] = 1; ca.dst[1] = c->Um;
-  ca.src[2] = psi; ca.src_is_double[2] = std::is_same<R, double>::value; ca.mult[2] = 2; ca.dst[2] = c->Vm;
+  ca.nf = 4;
+  ca.src[0] = psi; ca.src_is_double[0] = isd; ca.mult[0] = 1; ca.dst[0] = c->Um;    // u = -i ky psi
+  ca.src[1] = psi; ca.src_is_double[1] = isd; ca.mult[1] = 2; ca.dst[1] = c->Vm;    // v =  i kx psi
+  ca.src[2] = qin; ca.src_is_double[2] = 1; ca.mult[2] = 2; ca.dst[2] = c->QXm;     // q_x
+  ca.src[3] = qin; ca.src_is_double[3] = 1; ca.mult[3] = 3; ca.dst[3] = c->QYm;     // q_y
   CHK(launch_col_inv<R>(c, ca));
   RowJacArgs ra{};
-  ra.Q = c->Qm; ra.U = c->Um; ra.V = c->Vm; ra.UQ = c->UQm; ra.VQ = c->VQm; ra.DU = c->DUm; ra.DV = c->DVm;
+  ra.U = c->Um; ra.V = c->Vm; ra.QX = c->QXm; ra.QY = c->QYm; ra.J = c->Jm; ra.DU = c->DUm; ra.DV = c->DVm;
   ra.drag = c->p.C_d != 0.0;
-  {
-    const int nseq = ra.drag ? 4 : 3;
-    const int threads = row_threads(4, g.N2);
-    const size_t lds = (size_t)nseq * g.N2 * sizeof(cplx<R>);
-    const R scale = (R)(1.0 / ((double)g.nx * (double)g.ny));
-    CHK(timed(c, QGN_K_ROW_JAC, [&] {
-      hipLaunchKernelGGL((k_row_jac<R, kEPT>), dim3(g.ny * g.nz), dim3(threads), lds, c->stream, ra, g, c->px,
-                         tw.x, tw.r, scale);
-    }));
-  }
+  const R scale = (R)(1.0 / ((double)g.nx * (double)g.ny));
+  int rc = 0;
+  bool hit = with_size(g.N2, [&](auto nc) {
+    constexpr int N2 = decltype(nc)::value;
+    using Cfg = RowCfg<cplx<R>, N2, 4>;
+    constexpr int NT = Cfg::NTHR, RW = Cfg::ROWS;
+    const size_t lds = (size_t)(row_lds_slots<R>(RW * 4 * N2) + ct_tw_size(N2) + N2 + 1) * sizeof(cplx<R>);
+    auto kern = k_row_jac_ct<R, N2, RW, NT>;
+    rc = allow_lds(kern, lds);
+    if (rc) return;
+    const int ngroups = g.ny * g.nz / RW;
+    int per_cu = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, NT, lds) != hipSuccess || per_cu < 1)
+      per_cu = 1;
+    const int grid = std::min(ngroups, per_cu * c->ncu);
+    rc = timed(c, QGN_K_ROW_JAC, [&] {
+      hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), lds, c->stream, ra, g, tw.sx, tw.r, scale, ngroups);
+    });
+  }, CtSizes{});
+  if (!hit) return fail(QGN_EINVAL, "nx/2 not supported by the RHS passes");
+  CHK(rc);
   ColRhsArgs ka{};
-  ka.UQ = c->UQm; ka.VQ = c->VQm; ka.DU = c->DUm; ka.DV = c->DVm;
-  ka.q = qin; ka.psi = psi; ka.out = out;
+  ka.J = c->Jm; ka.q = qin; ka.psi = psi; ka.out = out;
   ka.ubar = c->ubar; ka.qybar = c->qybar;
   ka.beta = c->p.beta;
   ka.ekman = c->p.r0 * c->p.Htot / c->Hnz;
-  ka.cdrag = c->p.C_d * c->p.Htot / c->Hnz;
   ka.A2 = c->p.A2; ka.A8 = c->p.A8;
   ka.explicit_visc = c->p.is_explicit;
-  ka.drag = ra.drag;
-  {
-    const int C = col_tile(g.ny);
-    dim3 grid((g.Kx + C - 1) / C, g.nz);
-    const size_t lds = (size_t)C * g.ny * sizeof(cplx<R>);
-    CHK(timed(c, QGN_K_COL_RHS, [&] {
-      hipLaunchKernelGGL((k_col_rhs<R, R, kEPT>), grid, dim3(kColThreads), lds, c->stream, ka, g, c->py, tw.y, C);
-    }));
-  }
+  DragArgs da{};
+  da.DU = c->DUm; da.DV = c->DVm; da.out = out;
+  da.cdrag = c->p.C_d * c->p.Htot / c->Hnz;
+  with_size(g.ny, [&](auto nyc) {
+    constexpr int NY = decltype(nyc)::value;
+    using Cfg = ColCfg<R, NY>;
+    auto kern = k_col_rhs_ct<R, R, NY, Cfg::C, Cfg::NTHR>;
+    rc = allow_lds(kern, Cfg::LDS);
+    if (rc) return;
+    rc = timed(c, QGN_K_COL_RHS, [&] {
+      hipLaunchKernelGGL(kern, dim3(g.Kxp / Cfg::C, g.nz), dim3(Cfg::NTHR), Cfg::LDS, c->stream, ka, g, tw.sy);
+    });
+    if (rc || !ra.drag) return;
+    constexpr int CD = Cfg::C / 2 > 0 ? Cfg::C / 2 : 1;
+    constexpr int NTD0 = 2 * CD * NY / 8;
+    constexpr int NTD = NTD0 < 64 ? 64 : ((NTD0 + 63) / 64) * 64;
+    constexpr size_t LDSD = (size_t)(2 * CD * NY + ct_tw_size(NY)) * sizeof(cplx<R>);
+    auto dk = k_col_drag_ct<R, R, NY, CD, NTD>;
+    rc = allow_lds(dk, LDSD);
+    if (rc) return;
+    hipLaunchKernelGGL(dk, dim3(g.Kxp / CD), dim3(NTD), LDSD, c->stream, da, g, tw.sy);
+    rc = hipGetLastError() == hipSuccess ? 0 : fail(QGN_EHIP, "drag kernel launch failed");
+  }, CtSizes{});
+  CHK(rc);
   c->st.rhs_evals++;
   return 0;
 }
 
-int rhs(qgn_ctx* c, const double2* qin, void* out) {
-  if (c->p.precision == QGN_MIXED) return rhs_impl<float>(c, qin, out);
-  return rhs_impl<double>(c, qin, out);
+// psi_ready: psi of qin was already written by k_stage_proj
+int rhs(qgn_ctx* c, const double2* qin, void* out, bool psi_ready = false) {
+  if (c->p.precision == QGN_MIXED) return rhs_impl<float>(c, qin, out, psi_ready);
+  return rhs_impl<double>(c, qin, out, psi_ready);
 }
 
 // max over the physical grid of |c2r(E)| (unnormalised), FP64
@@ -386,14 +506,29 @@ int errnorm(qgn_ctx* c, const double2* Ecmp, double* maxv) {
   ca.nf = 1;
   ca.src[0] = Ecmp; ca.src_is_double[0] = 1; ca.mult[0] = 0; ca.dst[0] = c->Em;
   CHK(launch_col_inv<double>(c, ca));
-  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, c->stream, c->errslot, 1ull * 0);
-  HIPCHK(hipGetLastError());
-  const int threads = row_threads(1, g.N2);
-  const size_t lds = (size_t)g.N2 * sizeof(double2);
-  CHK(timed(c, QGN_K_ERRNORM, [&] {
-    hipLaunchKernelGGL((k_row_maxabs<double, kEPT>), dim3(g.ny * g.nz), dim3(threads), lds, c->stream, c->Em,
-                       g.Kx, g, c->px, c->twd.x, c->twd.r, c->errslot);
-  }));
+  {
+    int rc = 0;
+    bool hit = with_size(g.N2, [&](auto nc) {
+      constexpr int N2 = decltype(nc)::value;
+      using Cfg = RowCfg<float2, N2, 4>;   // 4 rows of one sequence where the Jacobian has 1 row of 4
+      constexpr int RW = Cfg::ROWS * 4 > 16 ? 16 : Cfg::ROWS * 4;
+      constexpr int NT0 = RW * N2 / 8;
+      constexpr int NT = NT0 < 64 ? 64 : (NT0 > 1024 ? 1024 : ((NT0 + 63) / 64) * 64);
+      const size_t lds = (size_t)(row_lds_slots<double>(RW * N2) + ct_tw_size(N2) + N2 + 1) * sizeof(double2);
+      auto kern = k_row_maxabs_ct<double, N2, RW, NT>;
+      rc = allow_lds(kern, lds);
+      if (rc) return;
+      rc = timed(c, QGN_K_ERRNORM, [&] {
+        hipLaunchKernelGGL(kern, dim3(g.ny * g.nz / RW), dim3(NT), lds, c->stream, c->Em, g.Kx, g.Kxp, g,
+                           c->twd.sx, c->twd.r, c->errpart);
+      });
+      if (rc) return;
+      hipLaunchKernelGGL(k_max_final, dim3(1), dim3(1024), 0, c->stream, c->errpart, g.ny * g.nz / RW, c->errslot);
+      rc = hipGetLastError() == hipSuccess ? 0 : fail(QGN_EHIP, "max reduction launch failed");
+    }, CtSizes{});
+    if (!hit) return fail(QGN_EINVAL, "nx/2 not supported by the error norm");
+    CHK(rc);
+  }
   HIPCHK(hipMemcpyAsync(c->h_errslot, c->errslot, sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   unsigned long long bits = *c->h_errslot;
@@ -423,6 +558,27 @@ int launch_stage(qgn_ctx* c, const StageArgs& a) {
 }
 int stage(qgn_ctx* c, const StageArgs& a) {
   return c->p.precision == QGN_MIXED ? launch_stage<float>(c, a) : launch_stage<double>(c, a);
+}
+
+// Stage combination fused with GetPsi (k_stage_proj).
+template <typename S>
+int launch_stage_proj(qgn_ctx* c, const StageArgs& a, bool* done) {
+  *done = true;
+  const int nz = c->g.nz;
+  int pp = 256 / nz;
+  if (pp < 1) pp = 1;
+  const int threads = ((pp * nz + 63) / 64) * 64;
+  const long plane = (long)c->g.Ky * c->g.Kxp;
+  const int blocks = (int)((plane + pp - 1) / pp);
+  const size_t lds = 2 * (size_t)nz * nz * sizeof(double) + 2 * (size_t)nz * pp * sizeof(double2);
+  void* psi = std::is_same<S, double>::value ? (void*)c->psi64 : c->psi;
+  return timed(c, QGN_K_STAGE, [&] {
+    hipLaunchKernelGGL((k_stage_proj<S, S>), dim3(blocks), dim3(threads), lds, c->stream, a, c->g, c->HM, c->MD,
+                       c->evals, 1.0 / c->p.Htot, pp, reinterpret_cast<cplx<S>*>(psi));
+  });
+}
+int stage_proj(qgn_ctx* c, const StageArgs& a, bool* done) {
+  return c->p.precision == QGN_MIXED ? launch_stage_proj<float>(c, a, done) : launch_stage_proj<double>(c, a, done);
 }
 
 int update(qgn_ctx* c, const UpdateArgs& a) {
@@ -495,8 +651,10 @@ int step_ark43(qgn_ctx* c, double* t, double* dt, bool qgleith) {
       a.A2 = p.A2;
       a.implicit_visc = 1;
       a.out = c->Y[i - 1];
-      CHK(stage(c, a));
-      CHK(rhs(c, c->Y[i - 1], c->N[i - 1]));
+      bool fused = false;
+      CHK(stage_proj(c, a, &fused));
+      if (!fused) CHK(stage(c, a));
+      CHK(rhs(c, c->Y[i - 1], c->N[i - 1], fused));
     }
     UpdateArgs u{};
     u.q = c->q;
@@ -684,6 +842,8 @@ int dft_bwd(hipStream_t s, const Geom& g, const FftPlan& px, const FftPlan& py, 
 int alloc(void** p, size_t bytes) {
   hipError_t e = hipMalloc(p, bytes);
   if (e != hipSuccess) return fail(QGN_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  // padding columns of compact/mixed arrays must read as zero
+  HIPCHK(hipMemset(*p, 0, bytes));
   return 0;
 }
 
@@ -691,7 +851,7 @@ void destroy_ctx(qgn_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   void* ptrs[] = {c->HM, c->MD, c->evals, c->ubar, c->qybar, c->a8l, c->partial, c->q, c->qn, c->E, c->psi64,
-                  c->psi, c->Qm, c->Um, c->Vm, c->UQm, c->VQm, c->DUm, c->DVm, c->Em, c->errslot, c->fa, c->fb,
+                  c->psi, c->Um, c->Vm, c->QXm, c->QYm, c->Jm, c->DUm, c->DVm, c->Em, c->errslot, c->errpart, c->fa, c->fb,
                   c->phys};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -723,7 +883,10 @@ int create_impl(qgn_ctx* c, const qgn_params* p, const double* H, const double* 
   c->p = *p;
   std::string why;
   if (!size_ok(p->nx, p->ny, &why)) return fail(QGN_EINVAL, why);
-  if (p->nz < 1 || p->nz > 64) return fail(QGN_EINVAL, "nz must be in [1, 64]");
+  if (!rhs_size_ok(p->nx, p->ny))
+    return fail(QGN_EINVAL, "the RHS supports nx/2 and ny in {32,48,64,96,128,192,256,384,512,768,1024,1536,2048} "
+                            "with nx, ny <= 2048");
+  if (p->nz < 2 || p->nz > 64) return fail(QGN_EINVAL, "nz must be in [1, 64]");
   if (p->scheme < QGN_AB3 || p->scheme > QGN_ARK43_QGLEITH) return fail(QGN_EINVAL, "unknown scheme");
   if (p->precision != QGN_FP64 && p->precision != QGN_MIXED) return fail(QGN_EINVAL, "unknown precision");
   if (!(p->Lx > 0) || !(p->Ly > 0) || !(p->Htot > 0)) return fail(QGN_EINVAL, "Lx, Ly, Htot must be positive");
@@ -732,6 +895,12 @@ int create_impl(qgn_ctx* c, const qgn_params* p, const double* H, const double* 
   make_plan(p->ny, &c->py);
   make_plan(p->nx / 2, &c->px);
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  {
+    int dev = 0, n = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev));
+    if (n > 0) c->ncu = n;
+  }
   CHK(make_tw<double>(p->nx, p->ny, &c->twd));
   CHK(make_tw<float>(p->nx, p->ny, &c->twf));
   // constants
@@ -761,8 +930,8 @@ int create_impl(qgn_ctx* c, const qgn_params* p, const double* H, const double* 
     c->c12 = b8 * b4;
   }
   const Geom& g = c->g;
-  c->ncmp = (long)nz * g.Ky * g.Kx;
-  c->nmix = (long)nz * g.ny * g.Kx;
+  c->ncmp = (long)nz * g.Ky * g.Kxp;
+  c->nmix = (long)nz * g.ny * g.Kxp;
   const size_t rs = p->precision == QGN_MIXED ? sizeof(float2) : sizeof(double2);
   c->rsize = rs;
   const size_t cb = (size_t)c->ncmp * sizeof(double2);
@@ -771,20 +940,20 @@ int create_impl(qgn_ctx* c, const qgn_params* p, const double* H, const double* 
   CHK(alloc((void**)&c->E, cb));
   CHK(alloc((void**)&c->psi64, cb));
   CHK(alloc(&c->psi, (size_t)c->ncmp * rs));
-  HIPCHK(hipMemset(c->q, 0, cb));
   for (int i = 0; i < 6; ++i) CHK(alloc(&c->N[i], (size_t)c->ncmp * rs));
   for (int i = 1; i < 6; ++i) CHK(alloc((void**)&c->Y[i], cb));
   for (int i = 0; i < 3; ++i) CHK(alloc(&c->hist[i], (size_t)c->ncmp * rs));
   const size_t mb = (size_t)c->nmix * rs;
-  CHK(alloc(&c->Qm, mb));
   CHK(alloc(&c->Um, mb));
   CHK(alloc(&c->Vm, mb));
-  CHK(alloc(&c->UQm, mb));
-  CHK(alloc(&c->VQm, mb));
-  CHK(alloc(&c->DUm, (size_t)g.ny * g.Kx * rs));
-  CHK(alloc(&c->DVm, (size_t)g.ny * g.Kx * rs));
+  CHK(alloc(&c->QXm, mb));
+  CHK(alloc(&c->QYm, mb));
+  CHK(alloc(&c->Jm, mb));
+  CHK(alloc(&c->DUm, (size_t)g.ny * g.Kxp * rs));
+  CHK(alloc(&c->DVm, (size_t)g.ny * g.Kxp * rs));
   CHK(alloc((void**)&c->Em, (size_t)c->nmix * sizeof(double2)));
   CHK(alloc((void**)&c->errslot, sizeof(unsigned long long)));
+  CHK(alloc((void**)&c->errpart, (size_t)g.ny * nz * sizeof(unsigned long long)));
   HIPCHK(hipHostMalloc((void**)&c->h_errslot, sizeof(unsigned long long), hipHostMallocDefault));
   const size_t fbytes = (size_t)nz * g.ny * g.nxh * sizeof(double2);
   CHK(alloc((void**)&c->fa, fbytes));
