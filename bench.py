@@ -139,7 +139,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=30.0)
     ap.add_argument("--calib-copy", action="store_true",
-                    help="first copy 1 GiB three times (reference kernel for PMC byte counters)")
+                    help="first run a 1 GiB read + 1 GiB write multiply three times (PMC byte-counter reference)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (members/slabs would "
@@ -152,10 +152,12 @@ def main():
         print("no GPU", file=sys.stderr)
         sys.exit(1)
     if a.calib_copy:
+        # reference kernel for the PMC byte counters: an elementwise multiply
+        # that reads 1 GiB and writes 1 GiB (torch's MulFunctor kernel)
         src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
         for _ in range(3):
-            dst.copy_(src)
+            torch.mul(src, 2.0, out=dst)
         torch.cuda.synchronize()
         del src, dst
     wl = configs.WORKLOADS[a.workload]()

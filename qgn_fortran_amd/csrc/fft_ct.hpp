@@ -14,23 +14,37 @@ struct CtPlan {
   int rad[kMaxPasses] = {};
 };
 
-constexpr CtPlan make_ct_plan(int n) {
-  CtPlan P{};
-  P.n = n;
-  int m = n, e2 = 0;
+// Radix plan for length n.  When 64 divides n the first and the last pass
+// are radix 8 (the row passes keep those two in registers); the rest is
+// 3s, 5s, then 8s, 4s and 2s.
+constexpr int ct_factor_middle(int m, int* rad, int k) {
+  int e2 = 0;
   while (m % 2 == 0) { m /= 2; ++e2; }
   int n3 = 0, n5 = 0;
   while (m % 3 == 0) { m /= 3; ++n3; }
   while (m % 5 == 0) { m /= 5; ++n5; }
-  int k = 0;
-  for (int i = 0; i < n3; ++i) P.rad[k++] = 3;
-  for (int i = 0; i < n5; ++i) P.rad[k++] = 5;
+  for (int i = 0; i < n3; ++i) rad[k++] = 3;
+  for (int i = 0; i < n5; ++i) rad[k++] = 5;
   int n8 = e2 / 3, rem = e2 % 3;
-  if (rem == 1 && n8 > 0) { n8 -= 1; P.rad[k++] = 4; P.rad[k++] = 4; }
-  else if (rem == 1) P.rad[k++] = 2;
-  else if (rem == 2) P.rad[k++] = 4;
-  for (int i = 0; i < n8; ++i) P.rad[k++] = 8;
-  P.np = (m == 1) ? k : -1;
+  if (rem == 1 && n8 > 0) { n8 -= 1; rad[k++] = 4; rad[k++] = 4; }
+  else if (rem == 1) rad[k++] = 2;
+  else if (rem == 2) rad[k++] = 4;
+  for (int i = 0; i < n8; ++i) rad[k++] = 8;
+  return m == 1 ? k : -1;
+}
+
+constexpr CtPlan make_ct_plan(int n) {
+  CtPlan P{};
+  P.n = n;
+  if (n % 64 == 0) {
+    P.rad[0] = 8;
+    int k = ct_factor_middle(n / 64, P.rad, 1);
+    if (k < 0) { P.np = -1; return P; }
+    P.rad[k++] = 8;
+    P.np = k;
+    return P;
+  }
+  P.np = ct_factor_middle(n, P.rad, 0);
   return P;
 }
 
@@ -119,6 +133,18 @@ __device__ __forceinline__ void ct_passes(cplx<Rt>* buf, const cplx<Rt>* __restr
     constexpr int RAD = PL.rad[PASS];
     ct_pass<Rt, N, RAD, P, L, NTHR, ct_tw_offset(N, PASS)>(buf, tw, sign);
     ct_passes<Rt, N, PASS + 1, P * RAD, L, NTHR>(buf, tw, sign);
+  }
+}
+
+// Passes [PASS, END) of the length-N plan; P = product of the radices of the
+// passes before PASS.
+template <typename Rt, int N, int PASS, int END, int P, class L, int NTHR>
+__device__ __forceinline__ void ct_passes_range(cplx<Rt>* buf, const cplx<Rt>* __restrict__ tw, int sign) {
+  constexpr CtPlan PL = make_ct_plan(N);
+  if constexpr (PASS < END) {
+    constexpr int RAD = PL.rad[PASS];
+    ct_pass<Rt, N, RAD, P, L, NTHR, ct_tw_offset(N, PASS)>(buf, tw, sign);
+    ct_passes_range<Rt, N, PASS + 1, END, P * RAD, L, NTHR>(buf, tw, sign);
   }
 }
 

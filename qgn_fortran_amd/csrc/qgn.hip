@@ -441,6 +441,18 @@ int rhs_impl(qgn_ctx* c, const double2* qin, void* out, bool psi_ready) {
   int rc = 0;
   bool hit = with_size(g.N2, [&](auto nc) {
     constexpr int N2 = decltype(nc)::value;
+    if constexpr (std::is_same<R, float>::value && N2 % 64 == 0 && N2 <= 1024) {
+      // FP32: register-fused row pass, 4 rows per workgroup (measured best)
+      constexpr int RW = 4, NT = RW * N2 / 8;
+      const size_t lds = (size_t)(row_lds_slots<R>(RW * N2) + ct_tw_size(N2) + N2 + 1) * sizeof(cplx<R>);
+      auto kern = k_row_jac_fused1<R, N2, RW>;
+      rc = allow_lds(kern, lds);
+      if (rc) return;
+      rc = timed(c, QGN_K_ROW_JAC, [&] {
+        hipLaunchKernelGGL(kern, dim3(g.ny * g.nz / RW), dim3(NT), lds, c->stream, ra, g, tw.sx, tw.r, scale);
+      });
+      return;
+    }
     using Cfg = RowCfg<cplx<R>, N2, 4>;
     constexpr int NT = Cfg::NTHR, RW = Cfg::ROWS;
     const size_t lds = (size_t)(row_lds_slots<R>(RW * 4 * N2) + ct_tw_size(N2) + N2 + 1) * sizeof(cplx<R>);
