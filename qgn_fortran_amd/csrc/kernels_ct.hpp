@@ -17,6 +17,17 @@
 
 namespace qgn {
 
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2).  Column tiles of 64-byte row segments share 128-byte lines with their
+// neighbour, so map consecutive dispatch slots of one XCD to consecutive
+// tiles: XCD x takes the x-th eighth of the tiles in order.  A permutation of
+// [0, total); the last total % 8 ids map to themselves.
+__device__ __forceinline__ int xcd_tile(int w, int total) {
+  const int per = total >> 3;
+  if (w >= (per << 3)) return w;
+  return (w & 7) * per + (w >> 3);
+}
+
 // P1: y-inverse FFT of up to four fields built from compact inputs.
 template <typename R, int NY, int C, int NTHR>
 __global__ void __launch_bounds__(NTHR) k_col_inv_ct(ColInvArgs a, Geom g, const cplx<R>* __restrict__ twy) {
@@ -26,11 +37,16 @@ __global__ void __launch_bounds__(NTHR) k_col_inv_ct(ColInvArgs a, Geom g, const
   Cx* tw = buf + C * NY;
   using L = CtLayout<C, 1, C, true>;
   constexpr int TOT = C * NY;
-  const int i0 = blockIdx.x * C;
-  const int z = blockIdx.y;
+  const int ntile = g.Kxp / C;
+  // grid = nf * ntile * nz, field fastest: the fields that share a source
+  // (u, v from psi; q_x, q_y from q) and neighbouring tiles run together on
+  // one XCD and share its L2
+  const int w = xcd_tile(blockIdx.x, gridDim.x);
+  const int f = w % a.nf;
+  const int i0 = ((w / a.nf) % ntile) * C;
+  const int z = w / (a.nf * ntile);
   load_table<R>(tw, twy, ct_tw_size(NY));
   {
-    const int f = blockIdx.z;   // one field per grid slice
     const int mult = a.mult[f];
     const bool isd = a.src_is_double[f];
     const void* src = a.src[f];
@@ -402,8 +418,10 @@ __global__ void __launch_bounds__(NTHR) k_col_rhs_ct(ColRhsArgs a, Geom g, const
   Cx* tw = buf + C * NY;
   using L = CtLayout<C, 1, C, true>;
   constexpr int TOT = C * NY;
-  const int i0 = blockIdx.x * C;
-  const int z = blockIdx.y;
+  const int ntile = g.Kxp / C;
+  const int w = xcd_tile(blockIdx.x, gridDim.x);   // grid = ntile * nz
+  const int i0 = (w % ntile) * C;
+  const int z = w / ntile;
   load_table<R>(tw, twy, ct_tw_size(NY));
   const Cx* src = reinterpret_cast<const Cx*>(a.J);
   for (int e = threadIdx.x; e < TOT; e += NTHR) {
@@ -457,7 +475,7 @@ __global__ void __launch_bounds__(NTHR) k_col_drag_ct(DragArgs a, Geom g, const 
   Cx* tw = bv + C * NY;
   using L = CtLayout<C, 1, C, true>;
   constexpr int TOT = C * NY;
-  const int i0 = blockIdx.x * C;
+  const int i0 = xcd_tile(blockIdx.x, gridDim.x) * C;   // grid = Kxp / C
   const int z = g.nz - 1;
   load_table<R>(tw, twy, ct_tw_size(NY));
   const Cx* du = reinterpret_cast<const Cx*>(a.DU);

@@ -410,7 +410,7 @@ int launch_col_inv(qgn_ctx* c, const ColInvArgs& a) {
     auto kern = k_col_inv_ct<R, NY, Cfg::C, Cfg::NTHR>;
     rc = allow_lds(kern, Cfg::LDS);
     if (rc) return;
-    dim3 grid(g.Kxp / Cfg::C, g.nz, a.nf);
+    dim3 grid(g.Kxp / Cfg::C * g.nz * a.nf);
     rc = timed(c, QGN_K_COL_INV, [&] {
       hipLaunchKernelGGL(kern, grid, dim3(Cfg::NTHR), Cfg::LDS, c->stream, a, g, tw.sy);
     });
@@ -487,7 +487,7 @@ int rhs_impl(qgn_ctx* c, const double2* qin, void* out, bool psi_ready) {
     rc = allow_lds(kern, Cfg::LDS);
     if (rc) return;
     rc = timed(c, QGN_K_COL_RHS, [&] {
-      hipLaunchKernelGGL(kern, dim3(g.Kxp / Cfg::C, g.nz), dim3(Cfg::NTHR), Cfg::LDS, c->stream, ka, g, tw.sy);
+      hipLaunchKernelGGL(kern, dim3(g.Kxp / Cfg::C * g.nz), dim3(Cfg::NTHR), Cfg::LDS, c->stream, ka, g, tw.sy);
     });
     if (rc || !ra.drag) return;
     constexpr int CD = Cfg::C / 2 > 0 ? Cfg::C / 2 : 1;
