@@ -359,6 +359,133 @@ __global__ void __launch_bounds__(256) k_stage_proj(StageArgs a, Geom g, const d
   }
 }
 
+// ---------------------------------------------------------------------------
+// Split-BF16 MFMA projection (mixed precision, nz <= 16).
+// psi = MD diag(1/(-k2 + EVals)) (HM / Htot) q as two 16x16x16 contractions
+// on v_mfma_f32_16x16x16bf16_1k.  Each FP32 operand x is split into
+// hi = bf16(x) and lo = bf16(x - hi); a product is accumulated in FP32 as
+// hi*hi + hi*lo + lo*hi (3 MFMAs), about 2^-16 relative per product.
+// Operand layouts (checked on gfx950): A[i][k] in lane l at i = l%16,
+// k = 4(l/16)+r; B[k][j] at j = l%16, k = 4(l/16)+r; D[i][j] at
+// i = 4(l/16)+r, j = l%16.  D of the first product is the B operand of the
+// second as it stands.
+// ---------------------------------------------------------------------------
+typedef short qgn_short4 __attribute__((ext_vector_type(4)));
+typedef float qgn_float4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ short bf16_rne(float x) {
+  unsigned u = __float_as_uint(x);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (short)(u >> 16);
+}
+__device__ __forceinline__ float bf16_to_f(short h) { return __uint_as_float(((unsigned)(unsigned short)h) << 16); }
+__device__ __forceinline__ short bf16_lo(float x, short hi) { return bf16_rne(x - bf16_to_f(hi)); }
+__device__ __forceinline__ qgn_float4 mfma3(const qgn_short4& ah, const qgn_short4& al, const qgn_short4& bh,
+                                            const qgn_short4& bl, qgn_float4 c) {
+  c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, c, 0, 0, 0);
+  return c;
+}
+
+// Stage combination (or, with a.ns = 0 and a.out = nullptr, a plain copy of
+// q) followed by the MFMA projection.  Workgroup: 16 points x 16 layers, one
+// thread per (point, layer) for the FP64 stage sums; waves 0 and 1 then
+// project 8 points each (16 columns: re, im).
+template <typename S>
+__global__ void __launch_bounds__(256) k_stage_proj_mfma(StageArgs a, Geom g, const double* __restrict__ HM,
+                                                         const double* __restrict__ MD,
+                                                         const double* __restrict__ evals, double inv_htot,
+                                                         float2* __restrict__ psi) {
+  __shared__ double2 sq[16][16];    // [layer][point]
+  const int nz = g.nz;
+  const int tid = threadIdx.x;
+  const long plane = (long)g.Ky * g.Kxp;
+  {
+    const int pp = tid & 15, k = tid >> 4;
+    const long pt = (long)blockIdx.x * 16 + pp;
+    double2 qt = make_double2(0.0, 0.0);
+    if (k < nz && pt < plane) {
+      const long e = k * plane + pt;
+      const double2 q = a.q[e];
+      double sx = 0.0, sy = 0.0;
+      for (int jj = 0; jj < a.ns; ++jj) {
+        const cplx<S> v = reinterpret_cast<const cplx<S>*>(a.N[jj])[e];
+        sx += a.ae[jj] * (double)v.x;
+        sy += a.ae[jj] * (double)v.y;
+      }
+      double mq = 1.0;
+      if (a.implicit_visc) {
+        const int i = (int)(pt % g.Kxp);
+        const int j = compact_row_to_full((int)(pt / g.Kxp), g);
+        const double kx = kx_of(i, g), ky = ky_of(j, g);
+        const double k2 = kx * kx + ky * ky;
+        const double k4 = k2 * k2;
+        const double a8 = a.a8l ? a.a8l[k] : a.A8;
+        const double lam = a.A2 * k2 + a8 * (k4 * k4);
+        for (int jj = 0; jj < a.ns; ++jj) {
+          const double2 y = jj == 0 ? q : a.Y[jj][e];
+          sx += a.ai[jj] * (-lam * y.x);
+          sy += a.ai[jj] * (-lam * y.y);
+        }
+        mq = 1.0 / (1.0 + 0.25 * a.dt * lam);
+      }
+      qt = make_double2(mq * (q.x + a.dt * sx), mq * (q.y + a.dt * sy));
+      if (a.out) a.out[e] = qt;
+    }
+    sq[k][pp] = qt;
+  }
+  __syncthreads();
+  if (tid >= 128) return;
+  const int l = tid & 63;
+  const int tile = tid >> 6;                 // points 8*tile .. 8*tile+7
+  const int row = l & 15, kb = 4 * (l >> 4);  // A: i = row; B, D: j = row
+  qgn_short4 a1h, a1l, a2h, a2l, bh, bl;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int kk = kb + r;
+    const float m1 = (row < nz && kk < nz) ? (float)(HM[row * nz + kk] * inv_htot) : 0.f;  // (HM/Htot)[mode][layer]
+    const float m2 = (row < nz && kk < nz) ? (float)MD[row * nz + kk] : 0.f;               // Modes[layer][mode]
+    a1h[r] = bf16_rne(m1);
+    a1l[r] = bf16_lo(m1, a1h[r]);
+    a2h[r] = bf16_rne(m2);
+    a2l[r] = bf16_lo(m2, a2h[r]);
+    const double2 v = sq[kk][8 * tile + (row >> 1)];
+    const float x = (float)((row & 1) ? v.y : v.x);
+    bh[r] = bf16_rne(x);
+    bl[r] = bf16_lo(x, bh[r]);
+  }
+  qgn_float4 d = {0.f, 0.f, 0.f, 0.f};
+  d = mfma3(a1h, a1l, bh, bl, d);            // modal amplitudes, D[mode][column]
+  const long pt = (long)blockIdx.x * 16 + 8 * tile + (row >> 1);
+  bool zero_mode = pt >= plane;
+  double k2 = 0.0;
+  if (!zero_mode) {
+    const int i = (int)(pt % g.Kxp);
+    const int j = compact_row_to_full((int)(pt / g.Kxp), g);
+    const double kx = kx_of(i, g), ky = ky_of(j, g);
+    k2 = kx * kx + ky * ky;
+    zero_mode = (i == 0 && j == 0) || i >= g.Kx;
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int mode = kb + r;
+    const float pm = (zero_mode || mode >= nz) ? 0.f : (float)((double)d[r] / (-k2 + evals[mode]));
+    bh[r] = bf16_rne(pm);
+    bl[r] = bf16_lo(pm, bh[r]);
+  }
+  qgn_float4 p = {0.f, 0.f, 0.f, 0.f};
+  p = mfma3(a2h, a2l, bh, bl, p);            // D[layer][column]
+  if (pt < plane) {
+    float* out = reinterpret_cast<float*>(psi);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int layer = kb + r;
+      if (layer < nz) out[2 * (layer * plane + pt) + (row & 1)] = p[r];
+    }
+  }
+}
+
 // Embedded error field and candidate update of ARK43 (ARK43*.f90 error and
 // update blocks), or of any scheme written as q + dt * sum_j b_j (N_j + L_j):
 //   E  = sum_j be_j (N_j + L_j),   qn = q + dt sum_j b_j (N_j + L_j), qn(0,0)=0

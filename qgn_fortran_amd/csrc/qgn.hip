@@ -419,6 +419,17 @@ int launch_col_inv(qgn_ctx* c, const ColInvArgs& a) {
   return rc;
 }
 
+// Mixed precision with nz <= 16: the stage + split-BF16 MFMA projection.
+int launch_stage_proj_mfma(qgn_ctx* c, const StageArgs& a, int kernel_id) {
+  const long plane = (long)c->g.Ky * c->g.Kxp;
+  const int blocks = (int)((plane + 15) / 16);
+  return timed(c, kernel_id, [&] {
+    hipLaunchKernelGGL((k_stage_proj_mfma<float>), dim3(blocks), dim3(256), 0, c->stream, a, c->g, c->HM, c->MD,
+                       c->evals, 1.0 / c->p.Htot, reinterpret_cast<float2*>(c->psi));
+  });
+}
+bool use_mfma_projection(const qgn_ctx* c) { return c->p.precision == QGN_MIXED && c->g.nz <= 16; }
+
 // GetRHS on compact arrays: out = RHS(qin)
 template <typename R>
 int rhs_impl(qgn_ctx* c, const double2* qin, void* out, bool psi_ready) {
@@ -426,7 +437,18 @@ int rhs_impl(qgn_ctx* c, const double2* qin, void* out, bool psi_ready) {
   const Tw<R>& tw = tw_of<R>(c);
   void* psi = std::is_same<R, double>::value ? (void*)c->psi64 : c->psi;
   const int isd = std::is_same<R, double>::value;
-  if (!psi_ready) CHK(launch_project<R>(c, qin, psi));
+  if (!psi_ready) {
+    if (use_mfma_projection(c)) {
+      StageArgs pa{};
+      pa.q = qin;
+      pa.ns = 0;
+      pa.dt = 0.0;
+      pa.out = nullptr;
+      CHK(launch_stage_proj_mfma(c, pa, QGN_K_PROJECT));
+    } else {
+      CHK(launch_project<R>(c, qin, psi));
+    }
+  }
   ColInvArgs ca{};
   ca.nf = 4;
   ca.src[0] = psi; ca.src_is_double[0] = isd; ca.mult[0] = 1; ca.dst[0] = c->Um;    // u = -i ky psi
@@ -590,6 +612,10 @@ int launch_stage_proj(qgn_ctx* c, const StageArgs& a, bool* done) {
   });
 }
 int stage_proj(qgn_ctx* c, const StageArgs& a, bool* done) {
+  if (use_mfma_projection(c)) {
+    *done = true;
+    return launch_stage_proj_mfma(c, a, QGN_K_STAGE);
+  }
   return c->p.precision == QGN_MIXED ? launch_stage_proj<float>(c, a, done) : launch_stage_proj<double>(c, a, done);
 }
 
