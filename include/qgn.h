@@ -90,6 +90,12 @@ int qgn_get_psi(qgn_ctx* ctx, const double* q_hat, double* psi_hat);
  * q_phys and psi_phys real(nx,ny,nz) = c2r(.)/(nx*ny).  Either may be NULL. */
 int qgn_fields(qgn_ctx* ctx, double* q_phys, double* psi_phys);
 
+/* WriteQPmodal's fields (QGN_Module.f90:381-416) of the current state:
+ * modal q, psi and Jacobian (the full, not dealiased, flux-form Jacobian),
+ * each real(nx,ny,nz) = c2r(.)/(nx*ny) with mode k in slot k.  Any may be
+ * NULL.  Allocates its work arrays on first use. */
+int qgn_modal_fields(qgn_ctx* ctx, double* q_mode, double* psi_mode, double* j_mode);
+
 /* QG-Leith coefficients A8L(nz) of the current state
  * (ARK43_QGLeith.f90:62-73). */
 int qgn_leith(qgn_ctx* ctx, double* a8l);

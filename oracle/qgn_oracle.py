@@ -249,6 +249,19 @@ class Model:
         """QGN_Module.f90:177-183: forward FFT of the physical IC, then DeAlias."""
         return dealias(self.r2c(np.asarray(q0, dtype=np.float64)), self.p.nx, self.p.ny)
 
+    def modal_fields(self, q_hat):
+        """WriteQPmodal's fields (QGN_Module.f90:381-416): modal q, psi and
+        the full flux-form Jacobian, each c2r(.)/(nx ny)."""
+        p = self.p
+        nn = float(p.nx * p.ny)
+        jaco_hat = self.jacobian(q_hat)
+        Hm = self.H[:, None] * self.Modes
+        q_mode = np.einsum("ak,ayx->kyx", Hm, q_hat) / p.Htot
+        j_mode = np.einsum("ak,ayx->kyx", Hm, jaco_hat) / p.Htot
+        psi_mode = q_mode / (-self.k2[None, :, :] + self.EVals[:, None, None])
+        psi_mode[:, 0, 0] = 0.0
+        return self.c2r(q_mode) / nn, self.c2r(psi_mode) / nn, self.c2r(j_mode) / nn
+
     def get_psi_phys(self, q_hat):
         """WriteQP's psi field (QGN_Module.f90:353-361)."""
         nn = float(self.p.nx * self.p.ny)

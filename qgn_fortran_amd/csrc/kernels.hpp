@@ -655,6 +655,103 @@ __global__ void k_leith_final(const double2* __restrict__ q, Geom g, const doubl
 }
 
 // ---------------------------------------------------------------------------
+// Full-spectrum helpers for the modal output of WriteQPmodal
+// (QGN_Module.f90:381-416); arrays (nz, ny, nx/2+1).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void full_k(long pt, const Geom& g, double* kx, double* ky) {
+  const int i = (int)(pt % g.nxh);
+  const int j = (int)(pt / g.nxh);
+  *kx = kx_of(i, g);
+  *ky = ky_of(j, g);
+}
+
+// out = W2 . diag(div ? 1/(-k2 + evals) : 1) . W1 . in over the layers at
+// each point (W1, W2 row-major nz x nz, nullptr = identity); (0,0) of the
+// divided quantity is zero (GetPsi, QGN_Module.f90:326-328).
+__global__ void k_layer_mix_full(const double2* __restrict__ in, double2* __restrict__ out, Geom g,
+                                 const double* __restrict__ W1, double s1, int div,
+                                 const double* __restrict__ evals, const double* __restrict__ W2) {
+  const long plane = (long)g.ny * g.nxh;
+  const int nz = g.nz;
+  for (long pt = blockIdx.x * (long)blockDim.x + threadIdx.x; pt < plane; pt += (long)gridDim.x * blockDim.x) {
+    double kx, ky;
+    full_k(pt, g, &kx, &ky);
+    const double k2 = kx * kx + ky * ky;
+    double2 m[64];
+    for (int k = 0; k < nz; ++k) {
+      double2 acc = make_double2(0.0, 0.0);
+      if (W1) {
+        for (int kk = 0; kk < nz; ++kk) {
+          const double w = W1[k * nz + kk];
+          const double2 v = in[kk * plane + pt];
+          acc.x += w * v.x;
+          acc.y += w * v.y;
+        }
+      } else {
+        acc = in[k * plane + pt];
+      }
+      acc.x *= s1;
+      acc.y *= s1;
+      if (div) {
+        const double den = -k2 + evals[k];
+        acc = pt == 0 ? make_double2(0.0, 0.0) : make_double2(acc.x / den, acc.y / den);
+      }
+      m[k] = acc;
+    }
+    for (int k = 0; k < nz; ++k) {
+      double2 acc = make_double2(0.0, 0.0);
+      if (W2) {
+        for (int kk = 0; kk < nz; ++kk) {
+          const double w = W2[k * nz + kk];
+          acc.x += m[kk].x * w;
+          acc.y += m[kk].y * w;
+        }
+      } else {
+        acc = m[k];
+      }
+      out[k * plane + pt] = acc;
+    }
+  }
+}
+
+// u = -i ky psi, v = i kx psi on the full spectrum (QGN_Module.f90:222-225)
+__global__ void k_uv_full(const double2* __restrict__ psi, double2* __restrict__ u, double2* __restrict__ v,
+                          Geom g) {
+  const long plane = (long)g.ny * g.nxh;
+  const long n = plane * g.nz;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    double kx, ky;
+    full_k(e % plane, g, &kx, &ky);
+    const double2 p = psi[e];
+    u[e] = make_double2(ky * p.y, -ky * p.x);
+    v[e] = make_double2(-kx * p.y, kx * p.x);
+  }
+}
+
+// physical products u q, v q (QGN_Module.f90:237,240)
+__global__ void k_prod_phys(const double* __restrict__ u, const double* __restrict__ v,
+                            const double* __restrict__ q, double* __restrict__ uq, double* __restrict__ vq,
+                            long n) {
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    uq[e] = u[e] * q[e];
+    vq[e] = v[e] * q[e];
+  }
+}
+
+// jaco_hat = i kx uq_hat + i ky vq_hat (QGN_Module.f90:244-247)
+__global__ void k_jac_full(const double2* __restrict__ uq, const double2* __restrict__ vq,
+                           double2* __restrict__ jac, Geom g) {
+  const long plane = (long)g.ny * g.nxh;
+  const long n = plane * g.nz;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    double kx, ky;
+    full_k(e % plane, g, &kx, &ky);
+    const double2 a = uq[e], b = vq[e];
+    jac[e] = make_double2(-kx * a.y - ky * b.y, kx * a.x + ky * b.x);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // General batched 2-D FFT pieces with the DFTI layout (MKL_DFTI_Setup.f90),
 // used by Initialize's forward transform and by the output fields.
 // ---------------------------------------------------------------------------

@@ -269,6 +269,9 @@ struct qgn_ctx {
   // full-layout scratch for the boundary
   double2 *fa = nullptr, *fb = nullptr;
   double* phys = nullptr;
+  // full-spectrum and physical work arrays of the modal output (on first use)
+  double2* mf[6] = {};
+  double* mp[5] = {};
   // stepper state (the reference's `save` variables)
   bool first = true, second = false;
   double err0 = 0.0, last_err = 0.0;
@@ -899,6 +902,10 @@ void destroy_ctx(qgn_ctx* c) {
   }
   for (int i = 0; i < 3; ++i)
     if (c->hist[i]) (void)hipFree(c->hist[i]);
+  for (int i = 0; i < 6; ++i)
+    if (c->mf[i]) (void)hipFree(c->mf[i]);
+  for (int i = 0; i < 5; ++i)
+    if (c->mp[i]) (void)hipFree(c->mp[i]);
   if (c->h_errslot) (void)hipHostFree(c->h_errslot);
   for (auto& e : c->ev) {
     (void)hipEventDestroy(e.first);
@@ -1042,6 +1049,54 @@ int cmp_to_phys_host(qgn_ctx* c, const double2* cmp, double* out) {
   return 0;
 }
 
+// WriteQPmodal's fields (QGN_Module.f90:381-416): the full (not dealiased)
+// flux-form Jacobian of the current state, the modal q, psi and J, each
+// returned as c2r(.)/(nx ny).
+int modal_fields(qgn_ctx* c, double* q_mode, double* psi_mode, double* j_mode) {
+  const Geom& g = c->g;
+  const long nf = (long)g.nz * g.ny * g.nxh;
+  const long np = (long)g.nz * g.ny * g.nx;
+  for (int i = 0; i < 6; ++i)
+    if (!c->mf[i]) CHK(alloc((void**)&c->mf[i], nf * sizeof(double2)));
+  for (int i = 0; i < 5; ++i)
+    if (!c->mp[i]) CHK(alloc((void**)&c->mp[i], np * sizeof(double)));
+  double2** F = c->mf;
+  double** P = c->mp;
+  const double nn = 1.0 / ((double)g.nx * (double)g.ny);
+  const double ih = 1.0 / c->p.Htot;
+  hipStream_t s = c->stream;
+  const int gb = grid1d(nf);
+  const int gp = grid1d((long)g.ny * g.nxh);
+  hipLaunchKernelGGL(k_unpack, dim3(gb), dim3(256), 0, s, c->q, F[0], g);
+  hipLaunchKernelGGL(k_layer_mix_full, dim3(gp), dim3(256), 0, s, F[0], F[1], g, c->HM, ih, 1, c->evals, c->MD);
+  hipLaunchKernelGGL(k_uv_full, dim3(gb), dim3(256), 0, s, F[1], F[2], F[3], g);
+  HIPCHK(hipGetLastError());
+  CHK(dft_bwd<double>(s, g, c->px, c->py, c->twd, g.nz, F[0], P[0], F[4], nn));
+  CHK(dft_bwd<double>(s, g, c->px, c->py, c->twd, g.nz, F[2], P[1], F[4], nn));
+  CHK(dft_bwd<double>(s, g, c->px, c->py, c->twd, g.nz, F[3], P[2], F[4], nn));
+  hipLaunchKernelGGL(k_prod_phys, dim3(grid1d(np)), dim3(256), 0, s, P[1], P[2], P[0], P[3], P[4], np);
+  HIPCHK(hipGetLastError());
+  CHK(dft_fwd<double>(s, g, c->px, c->py, c->twd, g.nz, P[3], F[2], F[4]));
+  CHK(dft_fwd<double>(s, g, c->px, c->py, c->twd, g.nz, P[4], F[3], F[4]));
+  hipLaunchKernelGGL(k_jac_full, dim3(gb), dim3(256), 0, s, F[2], F[3], F[5], g);
+  // modal q, psi, J
+  hipLaunchKernelGGL(k_layer_mix_full, dim3(gp), dim3(256), 0, s, F[0], F[1], g, c->HM, ih, 0, c->evals,
+                     (const double*)nullptr);
+  hipLaunchKernelGGL(k_layer_mix_full, dim3(gp), dim3(256), 0, s, F[1], F[2], g, (const double*)nullptr, 1.0, 1,
+                     c->evals, (const double*)nullptr);
+  hipLaunchKernelGGL(k_layer_mix_full, dim3(gp), dim3(256), 0, s, F[5], F[3], g, c->HM, ih, 0, c->evals,
+                     (const double*)nullptr);
+  HIPCHK(hipGetLastError());
+  double* outs[3] = {q_mode, psi_mode, j_mode};
+  for (int k = 0; k < 3; ++k) {
+    if (!outs[k]) continue;
+    CHK(dft_bwd<double>(s, g, c->px, c->py, c->twd, g.nz, F[1 + k], P[0], F[4], nn));
+    HIPCHK(hipMemcpyAsync(outs[k], P[0], np * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+  }
+  return 0;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1132,6 +1187,11 @@ int qgn_fields(qgn_ctx* c, double* q_phys, double* psi_phys) {
     CHK(cmp_to_phys_host(c, c->psi64, psi_phys));
   }
   return 0;
+}
+
+int qgn_modal_fields(qgn_ctx* c, double* q_mode, double* psi_mode, double* j_mode) {
+  if (!c) return fail(QGN_EINVAL, "null argument");
+  return modal_fields(c, q_mode, psi_mode, j_mode);
 }
 
 int qgn_leith(qgn_ctx* c, double* a8l) {

@@ -72,6 +72,12 @@ module qgn_c
       real(c_double), intent(out) :: q_phys(*), psi_phys(*)
     end function
 
+    integer(c_int) function qgn_modal_fields(ctx, q_mode, psi_mode, j_mode) bind(C, name="qgn_modal_fields")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: q_mode(*), psi_mode(*), j_mode(*)
+    end function
+
     integer(c_int) function qgn_get_stats(ctx, s) bind(C, name="qgn_get_stats")
       import :: c_int, c_ptr, qgn_stats
       type(c_ptr), value :: ctx

@@ -62,6 +62,7 @@ do i = N0, N0 + Nt
     if (mod(i, chkpt_freq) == 0) then
         if (.not. sync_host) call GetState(q_hat)
         call WriteQP(q_hat, 1, i)
+        call WriteQPmodal(q_hat, 1, i)
     end if
 end do
 call Cleanup

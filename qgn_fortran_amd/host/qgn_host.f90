@@ -10,8 +10,8 @@ use iso_c_binding
 use qgn_c
 implicit none
 private
-public dp, nx, ny, nz, pi, Initialize, TimeStep, WriteQP, Cleanup, GetRHS, ReadParameters, GetState, &
-       sync_host
+public dp, nx, ny, nz, pi, Initialize, TimeStep, WriteQP, WriteQPmodal, Cleanup, GetRHS, ReadParameters, &
+       GetState, sync_host
 
 include "parameters.f90"
 include "ARK43_QGLeith.f90"
@@ -145,6 +145,33 @@ subroutine WriteQP(q_hat, RecNo, nt)
     write(u) psi_phys
     close(u)
 end subroutine WriteQP
+
+! WriteQPmodal (QGN_Module.f90:381-446): modal q, psi and Jacobian of q_hat
+! as q_mode.%09d.dat, p_mode.%09d.dat and J_mode.%09d.dat.
+subroutine WriteQPmodal(q_hat, RecNo, nt)
+    complex(dp), intent(in) :: q_hat(nx/2+1,ny,nz)
+    integer,     intent(in) :: RecNo, nt
+    real(dp), allocatable :: q_mode(:,:,:), psi_mode(:,:,:), j_mode(:,:,:)
+    allocate(q_mode(nx,ny,nz), psi_mode(nx,ny,nz), j_mode(nx,ny,nz))
+    call qgn_check(qgn_set_state(ctx, q_hat), 'qgn_set_state')
+    call qgn_check(qgn_modal_fields(ctx, q_mode, psi_mode, j_mode), 'qgn_modal_fields')
+    call write_field("q_mode.", nt, RecNo, q_mode)
+    call write_field("p_mode.", nt, RecNo, psi_mode)
+    call write_field("J_mode.", nt, RecNo, j_mode)
+end subroutine WriteQPmodal
+
+subroutine write_field(prefix, nt, RecNo, f)
+    character(len=*), intent(in) :: prefix
+    integer,          intent(in) :: nt, RecNo
+    real(dp),         intent(in) :: f(nx,ny,nz)
+    character(len=50) :: file_name
+    integer :: u
+    write (file_name,'(A,I0.9,A4)') prefix, nt, '.dat'
+    open(newunit=u, file=file_name, access='STREAM', status='UNKNOWN')
+    if (RecNo == 1) rewind u
+    write(u) f
+    close(u)
+end subroutine write_field
 
 ! Cleanup (QGN_Module.f90:448-458)
 subroutine Cleanup

@@ -153,6 +153,13 @@ class QGN:
         check(self.lib.qgn_fields(self._ctx, dptr(q), dptr(psi)))
         return q, psi
 
+    def modal_fields(self):
+        """WriteQPmodal's q_mode, psi_mode, J_mode of the current state
+        (QGN_Module.f90:381-416)."""
+        out = [np.empty(self.phys_shape()) for _ in range(3)]
+        check(self.lib.qgn_modal_fields(self._ctx, dptr(out[0]), dptr(out[1]), dptr(out[2])))
+        return tuple(out)
+
     def leith(self):
         """A8L(nz) of the current state (ARK43_QGLeith.f90:62-73)."""
         out = np.empty(self.p.nz)

@@ -1,7 +1,8 @@
 """The Fortran host (qgn_fortran_amd/host) driving the HIP library end to end:
 profile files in, restart file in, q/p checkpoint files out, compared with
 the oracle stepping the same ARK43_QGLeith sequence (QGN_Driver.f90
-semantics: restart from q.N0.dat, steps N0..N0+Nt, WriteQP every step)."""
+semantics: restart from q.N0.dat, steps N0..N0+Nt, WriteQP and WriteQPmodal
+every step)."""
 import os
 import subprocess
 
@@ -53,6 +54,10 @@ def test_fortran_driver_restart_and_checkpoints(tmp_path):
         gp = initial.read_field(tmp_path / initial.restart_name("p", n), nx, ny, nz)
         assert rel_l2(gq, qp) < 1e-10, n
         assert rel_l2(gp, pp) < 1e-10, n
+        # WriteQPmodal's files (QGN_Driver.f90:59)
+        for pre, ref in zip(("q_mode", "p_mode", "J_mode"), om.modal_fields(q)):
+            gm = initial.read_field(tmp_path / f"{pre}.{n:09d}.dat", nx, ny, nz)
+            assert rel_l2(gm, ref) < 1e-10, (pre, n)
     for name in ("L.dat", "modes.dat", "Qy.dat"):
         assert (tmp_path / name).exists()
     assert np.allclose(np.fromfile(tmp_path / "Qy.dat"), qy, rtol=1e-14, atol=0)

@@ -99,6 +99,29 @@ def test_leith_coefficients():
     gm.close()
 
 
+@pytest.mark.parametrize("nx,ny,nz", [(128, 128, 2), (192, 128, 4)])
+def test_modal_fields(nx, ny, nz):
+    """WriteQPmodal's fields (QGN_Module.f90:381-416): modal q, psi and the
+    full (not dealiased) flux-form Jacobian, relative L2 <= 1e-12."""
+    print("modal", nx, ny, nz, flush=True)
+    kw = dict(profile="two") if nz == 2 else dict(Ly=0.7e6)
+    case = make_case(nx, ny, nz, k_peak=10, **kw)
+    om = _oracle_model(case)
+    gm = _gpu_model(case)
+    qh = om.initialize(case["q0"])
+    gm.set_state(qh)
+    got = gm.modal_fields()
+    ref = om.modal_fields(qh)
+    for name, a, b in zip(("q_mode", "psi_mode", "J_mode"), got, ref):
+        err = rel_l2(a, b, axis=(1, 2))
+        print(name, err)
+        assert np.all(err < 1e-12), name
+    # a second call reuses the work arrays
+    again = gm.modal_fields()
+    assert all(np.array_equal(a, b) for a, b in zip(got, again))
+    gm.close()
+
+
 STEP_CASES = [
     # (scheme, case, dt0, steps); adaptive cases start above the stable dt so
     # rejections occur and err1 sits near TOL (not at round-off level)

@@ -174,6 +174,38 @@ def test_jacobian_two_mode_closed_form():
     assert np.abs(jac - exp).max() < 1e-10 * np.abs(exp).max()
 
 
+def test_modal_fields_two_mode_closed_form():
+    """WriteQPmodal (QGN_Module.f90:381-416): for two Fourier modes, the
+    modal q, psi and J recombine through Modes into the layer fields built
+    analytically (J undealiased: the products lie inside the grid)."""
+    m, L = _model(nx=48, ny=48, nz=3)
+    nx = ny = 48
+    Lx = 1e6
+    x = (np.arange(nx) * Lx / nx)[None, :]
+    y = (np.arange(ny) * Lx / ny)[:, None]
+    modes = [((2, 3), np.array([1.0 + 0.5j, -0.3j, 0.2])), ((5, -1), np.array([0.4, 0.7 - 0.1j, -0.6j]))]
+    q = np.zeros((3, ny, nx)); psi = np.zeros_like(q)
+    psi_x = np.zeros_like(q); psi_y = np.zeros_like(q); q_x = np.zeros_like(q); q_y = np.zeros_like(q)
+    for (mx, my), amp in modes:
+        kx, ky = 2 * np.pi * mx / Lx, 2 * np.pi * my / Lx
+        alpha = np.linalg.solve(-(kx * kx + ky * ky) * np.eye(3) + L, amp)
+        ph = np.exp(1j * (kx * x + ky * y))
+        for z in range(3):
+            q[z] += np.real(amp[z] * ph)
+            psi[z] += np.real(alpha[z] * ph)
+            q_x[z] += np.real(1j * kx * amp[z] * ph)
+            q_y[z] += np.real(1j * ky * amp[z] * ph)
+            psi_x[z] += np.real(1j * kx * alpha[z] * ph)
+            psi_y[z] += np.real(1j * ky * alpha[z] * ph)
+    J = psi_x * q_y - psi_y * q_x
+    qm, pm, jm = m.modal_fields(orc.rfft2(q))
+    back = lambda f: np.einsum("zk,kyx->zyx", m.Modes, f)
+    for got, exp in ((back(qm), q), (back(pm), psi), (back(jm), J)):
+        assert np.abs(got - exp).max() < 1e-10 * np.abs(exp).max()
+    # the H-weighted projection: q_mode_k = sum_z H_z Modes_zk q_z / Htot
+    assert np.allclose(qm[0], np.einsum("z,zyx->yx", m.H * m.Modes[:, 0], q) / 4000.0, rtol=0, atol=1e-13)
+
+
 def test_jacobian_conserves_energy_and_enstrophy_n_multiple_of_3():
     m, _ = _model(nx=48, ny=48, nz=3)
     rng = np.random.default_rng(5)
