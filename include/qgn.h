@@ -96,6 +96,16 @@ int qgn_fields(qgn_ctx* ctx, double* q_phys, double* psi_phys);
  * NULL.  Allocates its work arrays on first use. */
 int qgn_modal_fields(qgn_ctx* ctx, double* q_mode, double* psi_mode, double* j_mode);
 
+/* Isotropic spectra of the current state, per layer, each (nbins, nz) with
+ * the bin fastest: ke = 1/2 sum k^2 |psi_hat|^2 and ens = 1/2 sum |q_hat|^2
+ * over the wavenumbers of the full plane with floor(|k|/dk + 1/2) = bin,
+ * dk = min(2 pi/Lx, 2 pi/Ly), normalised by (nx ny)^2 so that the sums over
+ * bins are the domain means of (u^2+v^2)/2 and q^2/2 of the physical fields.
+ * Wavenumbers beyond the last bin are dropped; 1 <= nbins <= 4096.  No
+ * counterpart in the reference: QGN_Driver.f90:52-55 leaves this
+ * diagnostics hook to the user. */
+int qgn_spectra(qgn_ctx* ctx, int nbins, double* ke, double* ens);
+
 /* QG-Leith coefficients A8L(nz) of the current state
  * (ARK43_QGLeith.f90:62-73). */
 int qgn_leith(qgn_ctx* ctx, double* a8l);

@@ -262,6 +262,32 @@ class Model:
         psi_mode[:, 0, 0] = 0.0
         return self.c2r(q_mode) / nn, self.c2r(psi_mode) / nn, self.c2r(j_mode) / nn
 
+    def spectra(self, q_hat, nbins):
+        """Isotropic KE and enstrophy spectra per layer from the physical
+        fields of q_hat (the diagnostics hook QGN_Driver.f90:52-55 leaves
+        open): full-plane fft2 of psi and q, bin floor(|k|/dk + 1/2),
+        dk = min(2 pi/Lx, 2 pi/Ly), KE = 1/2 k^2 |psi_hat|^2 and
+        Z = 1/2 |q_hat|^2, both over (nx ny)^2."""
+        p = self.p
+        nn = float(p.nx * p.ny)
+        q_phys, psi_phys = self.get_psi_phys(q_hat)
+        Q = np.fft.fft2(q_phys, axes=(-2, -1))
+        Ps = np.fft.fft2(psi_phys, axes=(-2, -1))
+        kx1 = (2.0 * PI / p.Lx) * np.fft.fftfreq(p.nx, 1.0 / p.nx)
+        ky1 = (2.0 * PI / p.Ly) * np.fft.fftfreq(p.ny, 1.0 / p.ny)
+        k2 = kx1[None, :] ** 2 + ky1[:, None] ** 2
+        dk = min(2.0 * PI / p.Lx, 2.0 * PI / p.Ly)
+        b = np.floor(np.sqrt(k2) / dk + 0.5).astype(np.int64).ravel()
+        keep = b < nbins
+        ke = np.zeros((p.nz, nbins))
+        ens = np.zeros((p.nz, nbins))
+        for z in range(p.nz):
+            e = (0.5 * k2 * np.abs(Ps[z]) ** 2 / nn ** 2).ravel()
+            w = (0.5 * np.abs(Q[z]) ** 2 / nn ** 2).ravel()
+            ke[z] = np.bincount(b[keep], weights=e[keep], minlength=nbins)[:nbins]
+            ens[z] = np.bincount(b[keep], weights=w[keep], minlength=nbins)[:nbins]
+        return ke, ens
+
     def get_psi_phys(self, q_hat):
         """WriteQP's psi field (QGN_Module.f90:353-361)."""
         nn = float(self.p.nx * self.p.ny)

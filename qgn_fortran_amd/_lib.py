@@ -18,7 +18,7 @@ KERNELS = {"row_jac": 0, "col_inv": 1, "col_rhs": 2, "project": 3, "stage": 4, "
 # Every symbol include/qgn.h declares.
 EXPORTS = [
     "qgn_create", "qgn_destroy", "qgn_initialize", "qgn_set_state", "qgn_get_state", "qgn_time_step",
-    "qgn_rhs", "qgn_get_psi", "qgn_fields", "qgn_modal_fields", "qgn_leith", "qgn_get_stats", "qgn_get_stream",
+    "qgn_rhs", "qgn_get_psi", "qgn_fields", "qgn_modal_fields", "qgn_spectra", "qgn_leith", "qgn_get_stats", "qgn_get_stream",
     "qgn_prof_enable", "qgn_prof_read", "qgn_dft_create", "qgn_dft_forward", "qgn_dft_backward",
     "qgn_dft_forward_device", "qgn_dft_backward_device", "qgn_dft_free", "qgn_vertical_modes",
     "qgn_last_error", "qgn_version",
@@ -71,6 +71,7 @@ def load():
         "qgn_get_psi": (I, [P, D, D]),
         "qgn_fields": (I, [P, D, D]),
         "qgn_modal_fields": (I, [P, D, D, D]),
+        "qgn_spectra": (I, [P, I, D, D]),
         "qgn_leith": (I, [P, D]),
         "qgn_get_stats": (I, [P, C.POINTER(QgnStats)]),
         "qgn_get_stream": (I, [P, C.POINTER(P)]),

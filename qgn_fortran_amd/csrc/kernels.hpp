@@ -752,6 +752,62 @@ __global__ void k_jac_full(const double2* __restrict__ uq, const double2* __rest
 }
 
 // ---------------------------------------------------------------------------
+// Isotropic KE and enstrophy spectra of the dealiased state, per layer:
+//   KE(b) = 1/2 sum w k^2 |psi_hat|^2 / (nx ny)^2,  Z(b) = 1/2 sum w |q_hat|^2 / (nx ny)^2
+// over the half plane with w = 2 for kx > 0; bin b = floor(|k|/dk + 1/2).
+// The kx = 0 column enters symmetrised, s(ky) = (x(ky) + conj x(-ky))/2 at
+// every ky with weight 1, which is the spectrum of the physical field that
+// c2r produces from a non-Hermitian column.  Bins accumulate in LDS, then in
+// out (ke then z, each nz x nbins), which the caller zeroes.
+// ---------------------------------------------------------------------------
+__global__ void k_spectra(const double2* __restrict__ q, const double2* __restrict__ psi, Geom g, double dk,
+                          int nbins, double norm, double* __restrict__ out) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  double* hk = reinterpret_cast<double*>(lds_raw);
+  double* hz = hk + nbins;
+  const int z = blockIdx.y;
+  for (int b = threadIdx.x; b < 2 * nbins; b += blockDim.x) hk[b] = 0.0;
+  __syncthreads();
+  const long plane = (long)g.Ky * g.Kxp;
+  const double2* qz = q + z * plane;
+  const double2* pz = psi + z * plane;
+  for (long pt = blockIdx.x * (long)blockDim.x + threadIdx.x; pt < plane; pt += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(pt % g.Kxp);
+    if (i == 0 || i >= g.Kx) continue;
+    const int j = compact_row_to_full((int)(pt / g.Kxp), g);
+    const double kx = kx_of(i, g), ky = ky_of(j, g);
+    const double k2 = kx * kx + ky * ky;
+    const int b = (int)floor(sqrt(k2) / dk + 0.5);
+    if (b >= nbins) continue;
+    const double2 a = pz[pt], c = qz[pt];
+    atomicAdd(&hk[b], k2 * (a.x * a.x + a.y * a.y) * norm);
+    atomicAdd(&hz[b], (c.x * c.x + c.y * c.y) * norm);
+  }
+  if (blockIdx.x == 0) {  // kx = 0 column over every full row
+    for (int j = threadIdx.x; j < g.ny; j += blockDim.x) {
+      if (j == 0) continue;
+      const int jc = full_row_to_compact(j, g);
+      const int jmc = full_row_to_compact((g.ny - j) % g.ny, g);
+      double2 pa = make_double2(0.0, 0.0), pb = pa, qa = pa, qb = pa;
+      if (jc >= 0) { pa = pz[(long)jc * g.Kxp]; qa = qz[(long)jc * g.Kxp]; }
+      if (jmc >= 0) { pb = pz[(long)jmc * g.Kxp]; qb = qz[(long)jmc * g.Kxp]; }
+      const double psx = 0.5 * (pa.x + pb.x), psy = 0.5 * (pa.y - pb.y);
+      const double qsx = 0.5 * (qa.x + qb.x), qsy = 0.5 * (qa.y - qb.y);
+      const double ky = ky_of(j, g);
+      const int b = (int)floor(fabs(ky) / dk + 0.5);
+      if (b >= nbins) continue;
+      atomicAdd(&hk[b], 0.5 * ky * ky * (psx * psx + psy * psy) * norm);
+      atomicAdd(&hz[b], 0.5 * (qsx * qsx + qsy * qsy) * norm);
+    }
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < nbins; b += blockDim.x) {
+    if (hk[b] != 0.0) atomicAdd(&out[(long)z * nbins + b], hk[b]);
+    if (hz[b] != 0.0) atomicAdd(&out[(long)(g.nz + z) * nbins + b], hz[b]);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // General batched 2-D FFT pieces with the DFTI layout (MKL_DFTI_Setup.f90),
 // used by Initialize's forward transform and by the output fields.
 // ---------------------------------------------------------------------------

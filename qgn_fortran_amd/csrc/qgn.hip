@@ -272,6 +272,8 @@ struct qgn_ctx {
   // full-spectrum and physical work arrays of the modal output (on first use)
   double2* mf[6] = {};
   double* mp[5] = {};
+  double* spec = nullptr;  // spectra accumulator, 2 nz nbins
+  long spec_n = 0;
   // stepper state (the reference's `save` variables)
   bool first = true, second = false;
   double err0 = 0.0, last_err = 0.0;
@@ -906,6 +908,7 @@ void destroy_ctx(qgn_ctx* c) {
     if (c->mf[i]) (void)hipFree(c->mf[i]);
   for (int i = 0; i < 5; ++i)
     if (c->mp[i]) (void)hipFree(c->mp[i]);
+  if (c->spec) (void)hipFree(c->spec);
   if (c->h_errslot) (void)hipHostFree(c->h_errslot);
   for (auto& e : c->ev) {
     (void)hipEventDestroy(e.first);
@@ -1192,6 +1195,34 @@ int qgn_fields(qgn_ctx* c, double* q_phys, double* psi_phys) {
 int qgn_modal_fields(qgn_ctx* c, double* q_mode, double* psi_mode, double* j_mode) {
   if (!c) return fail(QGN_EINVAL, "null argument");
   return modal_fields(c, q_mode, psi_mode, j_mode);
+}
+
+int qgn_spectra(qgn_ctx* c, int nbins, double* ke, double* ens) {
+  if (!c || !ke || !ens) return fail(QGN_EINVAL, "null argument");
+  const Geom& g = c->g;
+  if (nbins < 1 || 2L * nbins * (long)sizeof(double) > 64 * 1024)
+    return fail(QGN_EINVAL, "qgn_spectra: nbins must be in [1, 4096]");
+  const long nout = 2L * g.nz * nbins;
+  if (!c->spec || c->spec_n < nout) {
+    if (c->spec) HIPCHK(hipFree(c->spec));
+    c->spec = nullptr;
+    CHK(alloc((void**)&c->spec, nout * sizeof(double)));
+    c->spec_n = nout;
+  }
+  CHK(launch_project<double>(c, c->q, c->psi64));
+  HIPCHK(hipMemsetAsync(c->spec, 0, nout * sizeof(double), c->stream));
+  const double dk = std::fmin(g.dkx, g.dky);
+  const double nn = (double)g.nx * (double)g.ny;
+  const long plane = (long)g.Ky * g.Kxp;
+  int nblk = (int)std::min<long>((plane + 255) / 256, 64);
+  hipLaunchKernelGGL(k_spectra, dim3(nblk, g.nz), dim3(256), 2 * (size_t)nbins * sizeof(double), c->stream, c->q,
+                     reinterpret_cast<const double2*>(c->psi64), g, dk, nbins, 1.0 / (nn * nn), c->spec);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(ke, c->spec, g.nz * nbins * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(ens, c->spec + (long)g.nz * nbins, g.nz * nbins * sizeof(double), hipMemcpyDeviceToHost,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
 }
 
 int qgn_leith(qgn_ctx* c, double* a8l) {

@@ -78,6 +78,13 @@ module qgn_c
       real(c_double), intent(out) :: q_mode(*), psi_mode(*), j_mode(*)
     end function
 
+    integer(c_int) function qgn_spectra(ctx, nbins, ke, ens) bind(C, name="qgn_spectra")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nbins
+      real(c_double), intent(out) :: ke(*), ens(*)
+    end function
+
     integer(c_int) function qgn_get_stats(ctx, s) bind(C, name="qgn_get_stats")
       import :: c_int, c_ptr, qgn_stats
       type(c_ptr), value :: ctx

@@ -58,7 +58,10 @@ do i = N0, N0 + Nt
     call TimeStep(q_hat, t, dt)
     if (mod(i, out_freq) == 0) print *, 'Time since inception = ', real(t/86400._dp), &
         'days; time step size = ', real(dt), 's'
-    if (mod(i, diag_freq) == 0) print *, ' Diagnostics, RecNo = ', i/diag_freq
+    if (mod(i, diag_freq) == 0) then
+        print *, ' Diagnostics, RecNo = ', i/diag_freq
+        call WriteSpectra(i)
+    end if
     if (mod(i, chkpt_freq) == 0) then
         if (.not. sync_host) call GetState(q_hat)
         call WriteQP(q_hat, 1, i)

@@ -11,7 +11,7 @@ use qgn_c
 implicit none
 private
 public dp, nx, ny, nz, pi, Initialize, TimeStep, WriteQP, WriteQPmodal, Cleanup, GetRHS, ReadParameters, &
-       GetState, sync_host
+       GetState, WriteSpectra, sync_host
 
 include "parameters.f90"
 include "ARK43_QGLeith.f90"
@@ -159,6 +159,26 @@ subroutine WriteQPmodal(q_hat, RecNo, nt)
     call write_field("p_mode.", nt, RecNo, psi_mode)
     call write_field("J_mode.", nt, RecNo, j_mode)
 end subroutine WriteQPmodal
+
+! Diagnostics for the hook of QGN_Driver.f90:52-55: isotropic KE and
+! enstrophy spectra of the device state, per layer, as spec.%09d.dat holding
+! ke(nbins,nz) then ens(nbins,nz); bin b covers |k| within dk/2 of b*dk.
+subroutine WriteSpectra(nt)
+    integer, intent(in) :: nt
+    real(dp), allocatable :: ke(:,:), ens(:,:)
+    character(len=50) :: file_name
+    integer :: u, nbins
+    real(dp) :: dkx, dky
+    dkx = 2._dp*pi/Lx
+    dky = 2._dp*pi/Ly
+    nbins = floor(hypot(0.5_dp*nx*dkx, 0.5_dp*ny*dky)/min(dkx, dky) + 0.5_dp) + 1
+    allocate(ke(nbins,nz), ens(nbins,nz))
+    call qgn_check(qgn_spectra(ctx, int(nbins, c_int), ke, ens), 'qgn_spectra')
+    write (file_name,'(A5,I0.9,A4)') "spec.", nt, '.dat'
+    open(newunit=u, file=file_name, access='STREAM', status='REPLACE')
+    write(u) ke, ens
+    close(u)
+end subroutine WriteSpectra
 
 subroutine write_field(prefix, nt, RecNo, f)
     character(len=*), intent(in) :: prefix

@@ -76,6 +76,14 @@ def vertical_modes(H_raw, S, uBar, Htot):
     return H, L, Modes, ev, qy
 
 
+def spectra_bins(p: Params):
+    """Number of shells of width dk = min(2 pi/Lx, 2 pi/Ly) that covers every
+    wavenumber of the (nx, ny) grid."""
+    dkx, dky = 2.0 * np.pi / p.Lx, 2.0 * np.pi / p.Ly
+    kmax = np.hypot(0.5 * p.nx * dkx, 0.5 * p.ny * dky)
+    return int(np.floor(kmax / min(dkx, dky) + 0.5)) + 1
+
+
 class QGN:
     """One QGN model instance on the GPU (module state of QGN_Module.f90:11-41)."""
 
@@ -159,6 +167,17 @@ class QGN:
         out = [np.empty(self.phys_shape()) for _ in range(3)]
         check(self.lib.qgn_modal_fields(self._ctx, dptr(out[0]), dptr(out[1]), dptr(out[2])))
         return tuple(out)
+
+    def spectra(self, nbins=None):
+        """Isotropic KE and enstrophy spectra per layer, each (nz, nbins);
+        bin b holds |k| in [(b - 1/2) dk, (b + 1/2) dk), dk = min(2 pi/Lx, 2 pi/Ly)."""
+        p = self.p
+        if nbins is None:
+            nbins = spectra_bins(p)
+        ke = np.empty((p.nz, nbins))
+        ens = np.empty((p.nz, nbins))
+        check(self.lib.qgn_spectra(self._ctx, int(nbins), dptr(ke), dptr(ens)))
+        return ke, ens
 
     def leith(self):
         """A8L(nz) of the current state (ARK43_QGLeith.f90:62-73)."""

@@ -145,6 +145,41 @@ def test_get_psi_solves_helmholtz_per_wavenumber():
     assert np.all(psi[:, 0, 0] == 0)
 
 
+def test_spectra_single_mode_and_parseval():
+    """Shell spectra: one Fourier mode lands in bin round(|k|/dk) with
+    Z = A^2/4 and KE = k^2 B^2/4 (B the psi amplitude); for a random
+    dealiased state at n = 128 (non-Hermitian kx = 0 column) the bins sum to
+    the physical means of q^2/2 and |grad psi|^2/2 (finite-difference free:
+    gradients of the trigonometric interpolant)."""
+    m, L = _model(nx=48, ny=48, nz=3)
+    nx = ny = 48
+    x = np.arange(nx)[None, :]
+    y = np.arange(ny)[:, None]
+    A = 2.0e-6
+    q = np.zeros((3, ny, nx))
+    q[1] = A * np.cos(2 * np.pi * (5 * x / nx + 3 * y / ny))
+    ke, ens = m.spectra(orc.rfft2(q), 40)
+    k = 2 * np.pi / 1e6 * np.hypot(5, 3)
+    b = int(np.floor(np.hypot(5, 3) + 0.5))
+    assert np.isclose(ens[1, b], A * A / 4, rtol=1e-12) and np.sum(ens) - ens[1, b] < 1e-12 * ens[1, b]
+    psi = m.get_psi_phys(orc.rfft2(q))[1]
+    for z in range(3):
+        B = np.sqrt(2 * np.mean(psi[z] ** 2))
+        assert np.isclose(ke[z, b], k * k * B * B / 4, rtol=1e-10, atol=0)
+    m, L = _model(nx=128, ny=128, nz=3)
+    rng = np.random.default_rng(5)
+    qh = orc.dealias(orc.rfft2(rng.standard_normal((3, 128, 128))), 128, 128)
+    assert np.abs(qh[:, 1:, 0] - np.conj(qh[:, :0:-1, 0])).max() > 1.0   # non-Hermitian kx=0 column
+    ke, ens = m.spectra(qh, 200)
+    qp, pp = m.get_psi_phys(qh)
+    kx = (2 * np.pi / 1e6) * np.fft.fftfreq(128, 1 / 128)
+    P = np.fft.fft2(pp)
+    ux = np.real(np.fft.ifft2(1j * kx[None, None, :] * P))
+    uy = np.real(np.fft.ifft2(1j * kx[None, :, None] * P))
+    assert np.allclose(ens.sum(axis=1), 0.5 * np.mean(qp ** 2, axis=(1, 2)), rtol=1e-12, atol=0)
+    assert np.allclose(ke.sum(axis=1), 0.5 * np.mean(ux ** 2 + uy ** 2, axis=(1, 2)), rtol=1e-12, atol=0)
+
+
 # ------------------------------------------------------------- Jacobian ----
 def test_jacobian_two_mode_closed_form():
     """J(psi, q) = psi_x q_y - psi_y q_x for q made of two Fourier modes, with
