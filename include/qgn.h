@@ -64,6 +64,45 @@ int qgn_create(qgn_ctx** ctx, const qgn_params* p, const double* H, const double
                const double* EVals, const double* uBar, const double* qyBar);
 int qgn_destroy(qgn_ctx* ctx);
 
+/* ---- Batches of members (ensembles) ------------------------------------
+ * No counterpart in the reference, which runs one model per process.
+ * A batch stacks `members` models of one grid size along z: each member has
+ * p->nz layers and its own vertical structure (H, EVals, uBar, qyBar: nz
+ * values per member; Modes: nz*nz per member, member after member) and its
+ * own Ekman coefficient r0[m] (NULL: p->r0 for all); the other parameters
+ * are shared.  State arrays of the other entry points then hold
+ * members*nz layers, member after member.  qgn_create is the batch of one. */
+int qgn_create_batch(qgn_ctx** ctx, const qgn_params* p, int members, const double* H, const double* Modes,
+                     const double* EVals, const double* uBar, const double* qyBar, const double* r0);
+
+/* Per-member time and step size (arrays of `members`); resets the
+ * controller memory to FirstCall (err0 = TOL) and the counters. */
+int qgn_batch_set_time(qgn_ctx* ctx, const double* t, const double* dt);
+
+/* Enqueue `attempts` ARK43 / ARK43_QGLeith attempts of every member with
+ * the accept/reject and PI.3.4 control of ARK43*.f90 on the device, each
+ * member with its own dt: an accepted attempt advances that member, a
+ * rejected one shrinks its dt.  The attempt is captured once into a HIP
+ * graph; this call only launches it and returns without waiting. */
+int qgn_batch_step(qgn_ctx* ctx, int attempts);
+
+/* Wait for the context's work and read per-member t, dt, accepted steps,
+ * rejected attempts and status (0 ok, 1 non-finite error norm, 2 more than
+ * max_rejects consecutive rejections; such a member stops advancing).  Any
+ * argument may be NULL.  Returns QGN_ENONFINITE / QGN_EREJECT if a member
+ * has that status. */
+int qgn_batch_get_time(qgn_ctx* ctx, double* t, double* dt, long long* steps, long long* rejects, int* status);
+
+/* Several batches (e.g. one per grid size) advanced by one graph launch per
+ * attempt, the batches' attempt graphs as independent children.  Each
+ * context needs qgn_batch_set_time first; qgn_ensemble_step is ordered
+ * after the work already queued on every context and before what is
+ * queued on them afterwards, and returns without waiting. */
+typedef struct qgn_ens qgn_ens;
+int qgn_ensemble_create(qgn_ens** ens, qgn_ctx* const* ctxs, int n);
+int qgn_ensemble_step(qgn_ens* ens, int attempts);
+int qgn_ensemble_destroy(qgn_ens* ens);
+
 /* Initialize's transform of the initial condition (QGN_Module.f90:177-183):
  * forward r2c of qp real(nx,ny,nz), DeAlias, store as the device state.
  * q_hat (may be NULL) receives the dealiased spectrum. */

@@ -21,7 +21,8 @@ EXPORTS = [
     "qgn_rhs", "qgn_get_psi", "qgn_fields", "qgn_modal_fields", "qgn_spectra", "qgn_leith", "qgn_get_stats", "qgn_get_stream",
     "qgn_prof_enable", "qgn_prof_read", "qgn_dft_create", "qgn_dft_forward", "qgn_dft_backward",
     "qgn_dft_forward_device", "qgn_dft_backward_device", "qgn_dft_free", "qgn_vertical_modes",
-    "qgn_last_error", "qgn_version",
+    "qgn_last_error", "qgn_version", "qgn_create_batch", "qgn_batch_set_time", "qgn_batch_step",
+    "qgn_batch_get_time", "qgn_ensemble_create", "qgn_ensemble_step", "qgn_ensemble_destroy",
 ]
 
 
@@ -84,6 +85,13 @@ def load():
         "qgn_dft_backward_device": (I, [P, P, P, P]),
         "qgn_dft_free": (I, [P]),
         "qgn_vertical_modes": (I, [I, C.c_double, D, D, D, D, D, D, D, D]),
+        "qgn_create_batch": (I, [C.POINTER(P), C.POINTER(QgnParams), I, D, D, D, D, D, D]),
+        "qgn_batch_set_time": (I, [P, D, D]),
+        "qgn_batch_step": (I, [P, I]),
+        "qgn_batch_get_time": (I, [P, D, D, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.POINTER(I)]),
+        "qgn_ensemble_create": (I, [C.POINTER(P), C.POINTER(P), I]),
+        "qgn_ensemble_step": (I, [P, I]),
+        "qgn_ensemble_destroy": (I, [P]),
         "qgn_last_error": (C.c_char_p, []),
         "qgn_version": (C.c_char_p, []),
     }

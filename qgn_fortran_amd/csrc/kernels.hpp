@@ -18,7 +18,8 @@
 namespace qgn {
 
 struct Geom {
-  int nx, ny, nz;
+  int nx, ny, nz;   // nz: layers of all members together
+  int nzm;          // layers per member (a batch stacks members along z)
   int nxh;   // nx/2 + 1
   int N2;    // nx/2
   int Kx;    // kept kx indices: 0 .. nx/3-1
@@ -85,13 +86,24 @@ __global__ void k_unpack(const double2* __restrict__ cmp, double2* __restrict__ 
 // points of a workgroup exchange their modal amplitudes through LDS.
 //   HM[k*nz + kk]    = H(kk) * Modes(kk, k)
 //   MD[k*nz + kk]    = Modes(k, kk)
+// with nz the layers of one member; a batch holds one such pair of
+// matrices (and nz eigenvalues) per member, and grid.y runs over members.
 // ---------------------------------------------------------------------------
 template <typename Tout>
 __global__ void k_project(const double2* __restrict__ q, cplx<Tout>* __restrict__ psi,
                           const double* __restrict__ HM, const double* __restrict__ MD,
                           const double* __restrict__ evals, double inv_htot, int pp_per_block, Geom g) {
   extern __shared__ __align__(16) double lds_d[];
-  const int nz = g.nz;
+  const int nz = g.nzm;
+  {  // member blockIdx.y: its layers, matrices and eigenvalues
+    const int m = blockIdx.y;
+    const long off = (long)m * nz * g.Ky * g.Kxp;
+    q += off;
+    psi += off;
+    HM += m * nz * nz;
+    MD += m * nz * nz;
+    evals += m * nz;
+  }
   double* sHM = lds_d;
   double* sMD = sHM + nz * nz;
   double2* sq = reinterpret_cast<double2*>(sMD + nz * nz);  // [nz][pp]
@@ -155,7 +167,7 @@ struct ColInvArgs {
 struct RowJacArgs {
   const void* U; const void* V; const void* QX; const void* QY;  // mixed (nz, ny, Kxp)
   void* J;                                                       // mixed (nz, ny, Kxp)
-  void* DU; void* DV;                                            // (ny, Kxp), bottom layer
+  void* DU; void* DV;                                            // (members, ny, Kxp), bottom layers
   int drag;
 };
 
@@ -165,16 +177,17 @@ struct ColRhsArgs {
   const double2* q;      // compact
   const void* psi;       // compact cplx<R>
   void* out;             // compact cplx<S>
-  const double* ubar; const double* qybar;
-  double beta, ekman, A2, A8;
+  const double* ubar; const double* qybar;   // per layer
+  const double* ekman;   // per member: r0 Htot / H_nz, applied on its bottom layer
+  double beta, A2, A8;
   int explicit_visc;
 };
 
 // Bottom-layer quadratic drag: out(nz) += cdrag (i ky du_hat - i kx dv_hat).
 struct DragArgs {
-  const void* DU; const void* DV;   // (ny, Kxp) cplx<R>
-  void* out;                        // compact cplx<S>, layer nz-1 updated in place
-  double cdrag;
+  const void* DU; const void* DV;   // (members, ny, Kxp) cplx<R>
+  void* out;                        // compact cplx<S>, each member's bottom layer updated in place
+  const double* cdrag;              // per member: C_d Htot / H_nz
 };
 
 // ---------------------------------------------------------------------------
@@ -240,6 +253,7 @@ struct StageArgs {
   double ae[6], ai[6];
   int ns;                 // number of previous stages used
   double dt;
+  const double* dtv;      // per-member dt on the device (nullptr -> dt)
   const double* a8l;      // per-layer A8 (nullptr -> A8 below)
   double A8, A2;
   int implicit_visc;      // use Mq and L_j terms
@@ -251,6 +265,7 @@ __global__ void k_stage(StageArgs a, Geom g) {
   const long plane = (long)g.Ky * g.Kxp;
   const long n = plane * g.nz;
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    const double dt = a.dtv ? a.dtv[(int)(e / plane) / g.nzm] : a.dt;
     const double2 q = a.q[e];
     double sx = 0.0, sy = 0.0;
     for (int jj = 0; jj < a.ns; ++jj) {
@@ -274,9 +289,9 @@ __global__ void k_stage(StageArgs a, Geom g) {
         sx += a.ai[jj] * (-lam * y.x);
         sy += a.ai[jj] * (-lam * y.y);
       }
-      mq = 1.0 / (1.0 + 0.25 * a.dt * lam);
+      mq = 1.0 / (1.0 + 0.25 * dt * lam);
     }
-    a.out[e] = make_double2(mq * (q.x + a.dt * sx), mq * (q.y + a.dt * sy));
+    a.out[e] = make_double2(mq * (q.x + dt * sx), mq * (q.y + dt * sy));
   }
 }
 
@@ -289,7 +304,12 @@ __global__ void __launch_bounds__(256) k_stage_proj(StageArgs a, Geom g, const d
                                                     const double* __restrict__ MD, const double* __restrict__ evals,
                                                     double inv_htot, int pp_per_block, cplx<Tp>* __restrict__ psi) {
   extern __shared__ __align__(16) double lds_d[];
-  const int nz = g.nz;
+  const int nz = g.nzm;
+  const int mb = blockIdx.y;           // member
+  HM += mb * nz * nz;
+  MD += mb * nz * nz;
+  evals += mb * nz;
+  const double dt = a.dtv ? a.dtv[mb] : a.dt;
   double* sHM = lds_d;
   double* sMD = sHM + nz * nz;
   double2* sq = reinterpret_cast<double2*>(sMD + nz * nz);  // [nz][pp]
@@ -299,6 +319,7 @@ __global__ void __launch_bounds__(256) k_stage_proj(StageArgs a, Geom g, const d
   const long plane = (long)g.Ky * g.Kxp;
   const int pp = tid % pp_per_block;
   const int k = tid / pp_per_block;
+  const int kl = mb * nz + k;          // layer within the batch
   const long pt = (long)blockIdx.x * pp_per_block + pp;
   const bool active = (k < nz) && (pt < plane);
   int i = 0, j = 0;
@@ -308,7 +329,7 @@ __global__ void __launch_bounds__(256) k_stage_proj(StageArgs a, Geom g, const d
     j = compact_row_to_full((int)(pt / g.Kxp), g);
     const double kx = kx_of(i, g), ky = ky_of(j, g);
     k2 = kx * kx + ky * ky;
-    const long e = k * plane + pt;
+    const long e = kl * plane + pt;
     const double2 q = a.q[e];
     double sx = 0.0, sy = 0.0;
     for (int jj = 0; jj < a.ns; ++jj) {
@@ -319,16 +340,16 @@ __global__ void __launch_bounds__(256) k_stage_proj(StageArgs a, Geom g, const d
     double mq = 1.0;
     if (a.implicit_visc) {
       const double k4 = k2 * k2;
-      const double a8 = a.a8l ? a.a8l[k] : a.A8;
+      const double a8 = a.a8l ? a.a8l[kl] : a.A8;
       const double lam = a.A2 * k2 + a8 * (k4 * k4);
       for (int jj = 0; jj < a.ns; ++jj) {
         const double2 y = jj == 0 ? q : a.Y[jj][e];
         sx += a.ai[jj] * (-lam * y.x);
         sy += a.ai[jj] * (-lam * y.y);
       }
-      mq = 1.0 / (1.0 + 0.25 * a.dt * lam);
+      mq = 1.0 / (1.0 + 0.25 * dt * lam);
     }
-    const double2 qt = make_double2(mq * (q.x + a.dt * sx), mq * (q.y + a.dt * sy));
+    const double2 qt = make_double2(mq * (q.x + dt * sx), mq * (q.y + dt * sy));
     a.out[e] = qt;
     sq[k * pp_per_block + pp] = qt;
   }
@@ -355,7 +376,7 @@ __global__ void __launch_bounds__(256) k_stage_proj(StageArgs a, Geom g, const d
       acc.x += v.x * w;
       acc.y += v.y * w;
     }
-    psi[k * plane + pt] = cmk<cplx<Tp>>((Tp)acc.x, (Tp)acc.y);
+    psi[kl * plane + pt] = cmk<cplx<Tp>>((Tp)acc.x, (Tp)acc.y);
   }
 }
 
@@ -398,7 +419,13 @@ __global__ void __launch_bounds__(256) k_stage_proj_mfma(StageArgs a, Geom g, co
                                                          const double* __restrict__ evals, double inv_htot,
                                                          float2* __restrict__ psi) {
   __shared__ double2 sq[16][16];    // [layer][point]
-  const int nz = g.nz;
+  const int nz = g.nzm;
+  const int mb = blockIdx.y;         // member
+  HM += mb * nz * nz;
+  MD += mb * nz * nz;
+  evals += mb * nz;
+  const long moff = (long)mb * nz * g.Ky * g.Kxp;   // first element of the member
+  const double dt = a.dtv ? a.dtv[mb] : a.dt;
   const int tid = threadIdx.x;
   const long plane = (long)g.Ky * g.Kxp;
   {
@@ -406,7 +433,7 @@ __global__ void __launch_bounds__(256) k_stage_proj_mfma(StageArgs a, Geom g, co
     const long pt = (long)blockIdx.x * 16 + pp;
     double2 qt = make_double2(0.0, 0.0);
     if (k < nz && pt < plane) {
-      const long e = k * plane + pt;
+      const long e = moff + k * plane + pt;
       const double2 q = a.q[e];
       double sx = 0.0, sy = 0.0;
       for (int jj = 0; jj < a.ns; ++jj) {
@@ -421,16 +448,16 @@ __global__ void __launch_bounds__(256) k_stage_proj_mfma(StageArgs a, Geom g, co
         const double kx = kx_of(i, g), ky = ky_of(j, g);
         const double k2 = kx * kx + ky * ky;
         const double k4 = k2 * k2;
-        const double a8 = a.a8l ? a.a8l[k] : a.A8;
+        const double a8 = a.a8l ? a.a8l[mb * nz + k] : a.A8;
         const double lam = a.A2 * k2 + a8 * (k4 * k4);
         for (int jj = 0; jj < a.ns; ++jj) {
           const double2 y = jj == 0 ? q : a.Y[jj][e];
           sx += a.ai[jj] * (-lam * y.x);
           sy += a.ai[jj] * (-lam * y.y);
         }
-        mq = 1.0 / (1.0 + 0.25 * a.dt * lam);
+        mq = 1.0 / (1.0 + 0.25 * dt * lam);
       }
-      qt = make_double2(mq * (q.x + a.dt * sx), mq * (q.y + a.dt * sy));
+      qt = make_double2(mq * (q.x + dt * sx), mq * (q.y + dt * sy));
       if (a.out) a.out[e] = qt;
     }
     sq[k][pp] = qt;
@@ -481,7 +508,7 @@ __global__ void __launch_bounds__(256) k_stage_proj_mfma(StageArgs a, Geom g, co
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int layer = kb + r;
-      if (layer < nz) out[2 * (layer * plane + pt) + (row & 1)] = p[r];
+      if (layer < nz) out[2 * (moff + layer * plane + pt) + (row & 1)] = p[r];
     }
   }
 }
@@ -496,6 +523,7 @@ struct UpdateArgs {
   double b[6], be[6];
   int ns;
   double dt;
+  const double* dtv;  // per-member dt on the device (nullptr -> dt)
   const double* a8l;
   double A8, A2;
   int implicit_visc;
@@ -511,6 +539,7 @@ __global__ void k_update(UpdateArgs a, Geom g) {
     const double2 q = a.q[e];
     const int z = (int)(e / plane);
     const long pt = e % plane;
+    const double dt = a.dtv ? a.dtv[z / g.nzm] : a.dt;
     double lam = 0.0;
     if (a.implicit_visc) {
       const int i = (int)(pt % g.Kxp);
@@ -537,7 +566,7 @@ __global__ void k_update(UpdateArgs a, Geom g) {
       uy += a.b[jj] * ty;
     }
     if (a.E) a.E[e] = make_double2(ex, ey);
-    double2 r = make_double2(q.x + a.dt * ux, q.y + a.dt * uy);
+    double2 r = make_double2(q.x + dt * ux, q.y + dt * uy);
     if (pt == 0) r = make_double2(0.0, 0.0);
     a.qn[e] = r;
   }
@@ -749,6 +778,87 @@ __global__ void k_jac_full(const double2* __restrict__ uq, const double2* __rest
     const double2 a = uq[e], b = vq[e];
     jac[e] = make_double2(-kx * a.y - ky * b.y, kx * a.x + ky * b.x);
   }
+}
+
+// ---------------------------------------------------------------------------
+// Device-side ARK43 step control of a batch (ARK43*.f90 accept/reject and
+// PI.3.4 blocks), one thread per member, after the error norm of an attempt:
+//   err1 = dt max|c2r(E)| / (nx ny)
+//   err1 > TOL: dt *= reject_factor                              (reject)
+//   else: t += dt, dt = min(dt (safety TOL/err1)^0.075 (err0/err1)^0.1,
+//         dt_max), err0 = err1                                   (accept)
+// with the arithmetic of the single-model host controller.  A non-finite
+// err1 sets status 1 and more than max_rejects consecutive rejections
+// status 2; a member with a status keeps its state and time from then on.
+// ---------------------------------------------------------------------------
+struct BatchCtl {
+  double *t, *dt, *err0, *last;
+  long long *steps, *attempts, *rejects;
+  int *nrej, *status, *accept;
+  const unsigned long long* maxbits;   // per member, from k_max_final
+  double* report;   // single model: accept, t, dt, err1, status, err0 of member 0 (or nullptr)
+  double TOL, safety, reject_factor, dt_max, nn;
+  int max_rejects, nm;
+};
+
+__device__ void batch_control_member(const BatchCtl& c, int m) {
+  c.accept[m] = 0;
+  if (c.status[m]) return;
+  const double mx = __longlong_as_double((long long)c.maxbits[m]);
+  const double dt = c.dt[m];
+  const double err1 = dt * mx / c.nn;
+  c.last[m] = err1;
+  c.attempts[m] += 1;
+  if (!isfinite(err1)) { c.status[m] = 1; return; }
+  if (err1 > c.TOL) {
+    c.dt[m] = c.reject_factor * dt;
+    c.rejects[m] += 1;
+    if (++c.nrej[m] > c.max_rejects) c.status[m] = 2;
+    return;
+  }
+  c.nrej[m] = 0;
+  c.accept[m] = 1;
+  c.t[m] = c.t[m] + dt;
+  c.dt[m] = fmin(dt * pow(c.safety * c.TOL / err1, 0.075) * pow(c.err0[m] / err1, 0.1), c.dt_max);
+  c.err0[m] = err1;
+  c.steps[m] += 1;
+}
+
+__global__ void k_batch_control(BatchCtl c) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= c.nm) return;
+  batch_control_member(c, m);
+  if (c.report && m == 0) {
+    c.report[0] = c.accept[0];
+    c.report[1] = c.t[0];
+    c.report[2] = c.dt[0];
+    c.report[3] = c.last[0];
+    c.report[4] = c.status[0];
+    c.report[5] = c.err0[0];
+  }
+}
+
+// Single model: the caller's t and dt (and, on the first step, err0 = TOL)
+// into the controller state of member 0; clears its reject count and status.
+__global__ void k_control_set(double* t, double* dt, double* err0, int* nrej, int* status, double tv, double dtv,
+                              double err0v) {
+  if (threadIdx.x != 0) return;
+  t[0] = tv;
+  dt[0] = dtv;
+  if (err0v >= 0.0) err0[0] = err0v;
+  nrej[0] = 0;
+  status[0] = 0;
+}
+
+// q <- qn for the members that accepted; grid (blocks, members)
+__global__ void k_batch_commit(double2* __restrict__ q, const double2* __restrict__ qn, const int* __restrict__ accept,
+                               Geom g) {
+  const int m = blockIdx.y;
+  if (!accept[m]) return;
+  const long n = (long)g.nzm * g.Ky * g.Kxp;
+  const long off = (long)m * n;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x)
+    q[off + e] = qn[off + e];
 }
 
 // ---------------------------------------------------------------------------

@@ -145,7 +145,7 @@ __global__ void __launch_bounds__(NTHR) k_row_jac_ct(RowJacArgs a, Geom g, const
     const int row0 = grp * ROWS;
     const int z = row0 / g.ny;
     const int j0 = row0 - z * g.ny;
-    const bool bottom = a.drag && (z == g.nz - 1);
+    const bool bottom = a.drag && (z % g.nzm == g.nzm - 1);
 #pragma unroll
     for (int b = 0; b < EL; ++b) {
       const int e = threadIdx.x + b * NTHR;
@@ -181,8 +181,8 @@ __global__ void __launch_bounds__(NTHR) k_row_jac_ct(RowJacArgs a, Geom g, const
       jo[(long)r * Kxp + k] = r2c_post_pad<R, PS>(buf, 4 * r * N2, k, N2, tr);
     }
     if (bottom) {
-      Cx* du = reinterpret_cast<Cx*>(a.DU) + (long)j0 * Kxp;
-      Cx* dv = reinterpret_cast<Cx*>(a.DV) + (long)j0 * Kxp;
+      Cx* du = reinterpret_cast<Cx*>(a.DU) + (long)((z / g.nzm) * g.ny + j0) * Kxp;
+      Cx* dv = reinterpret_cast<Cx*>(a.DV) + (long)((z / g.nzm) * g.ny + j0) * Kxp;
       for (int e = threadIdx.x; e < ROWS * Kx; e += NTHR) {
         const int r = e / Kx, k = e - r * Kx;
         du[(long)r * Kxp + k] = r2c_post_pad<R, PS>(buf, (4 * r + 1) * N2, k, N2, tr);
@@ -224,7 +224,7 @@ __global__ void __launch_bounds__(ROWS * (N2 / 8)) k_row_jac_fused(RowJacArgs a,
   const int row0 = blockIdx.x * ROWS;
   const int z = row0 / g.ny;
   const int j0 = row0 - z * g.ny;
-  const bool bottom = a.drag && (z == g.nz - 1);
+  const bool bottom = a.drag && (z % g.nzm == g.nzm - 1);
   const int Kx = g.Kx;
   const long Kxp = g.Kxp;
   const long roff = (long)(row0 + r) * Kxp;
@@ -301,8 +301,8 @@ __global__ void __launch_bounds__(ROWS * (N2 / 8)) k_row_jac_fused(RowJacArgs a,
     jo[(long)rr * Kxp + k] = r2c_post_pad<R, PS>(buf, 4 * rr * N2, k, N2, tr);
   }
   if (bottom) {
-    Cx* du = reinterpret_cast<Cx*>(a.DU) + (long)j0 * Kxp;
-    Cx* dv = reinterpret_cast<Cx*>(a.DV) + (long)j0 * Kxp;
+    Cx* du = reinterpret_cast<Cx*>(a.DU) + (long)((z / g.nzm) * g.ny + j0) * Kxp;
+    Cx* dv = reinterpret_cast<Cx*>(a.DV) + (long)((z / g.nzm) * g.ny + j0) * Kxp;
     for (int e = threadIdx.x; e < ROWS * Kx; e += NTHR) {
       const int rr = e / Kx, k = e - rr * Kx;
       du[(long)rr * Kxp + k] = r2c_post_pad<R, PS>(buf, (4 * rr + 1) * N2, k, N2, tr);
@@ -339,7 +339,7 @@ __global__ void __launch_bounds__(ROWS * (N2 / 8)) k_row_jac_fused1(RowJacArgs a
   const int row0 = blockIdx.x * ROWS;
   const int z = row0 / g.ny;
   const int j0 = row0 - z * g.ny;
-  const bool bottom = a.drag && (z == g.nz - 1);
+  const bool bottom = a.drag && (z % g.nzm == g.nzm - 1);
   const int Kx = g.Kx;
   const long Kxp = g.Kxp;
   const long roff = (long)(row0 + r) * Kxp;
@@ -375,8 +375,9 @@ __global__ void __launch_bounds__(ROWS * (N2 / 8)) k_row_jac_fused1(RowJacArgs a
     }
     Dft<Cx, 8>::run(f[s], +1);
   }
-  Cx* outs[3] = {reinterpret_cast<Cx*>(a.J) + (long)row0 * Kxp, reinterpret_cast<Cx*>(a.DU) + (long)j0 * Kxp,
-                 reinterpret_cast<Cx*>(a.DV) + (long)j0 * Kxp};
+  const long drow = (long)((z / g.nzm) * g.ny + j0) * Kxp;   // bottom layer of member z / nzm
+  Cx* outs[3] = {reinterpret_cast<Cx*>(a.J) + (long)row0 * Kxp, reinterpret_cast<Cx*>(a.DU) + drow,
+                 reinterpret_cast<Cx*>(a.DV) + drow};
   const int nout = bottom ? 3 : 1;
   for (int o = 0; o < nout; ++o) {
     Cx w8[8];
@@ -434,7 +435,8 @@ __global__ void __launch_bounds__(NTHR) k_col_rhs_ct(ColRhsArgs a, Geom g, const
   cplx<S>* out = reinterpret_cast<cplx<S>*>(a.out);
   const R ub = (R)a.ubar[z];
   const R bq = (R)(a.beta + a.qybar[z]);
-  const bool bottom = (z == g.nz - 1);
+  const bool bottom = (z % g.nzm == g.nzm - 1);
+  const R ek = bottom ? (R)a.ekman[z / g.nzm] : (R)0;
   const int nown = C * g.Ky;
   for (int o = threadIdx.x; o < nown; o += NTHR) {
     const int c = o % C, jc = o / C, i = i0 + c;
@@ -456,8 +458,8 @@ __global__ void __launch_bounds__(NTHR) k_col_rhs_ct(ColRhsArgs a, Geom g, const
     }
     if (bottom) {
       const R k2 = kx * kx + ky * ky;
-      r.x += (R)a.ekman * k2 * ps.x;
-      r.y += (R)a.ekman * k2 * ps.y;
+      r.x += ek * k2 * ps.x;
+      r.y += ek * k2 * ps.y;
     }
     if ((i == 0 && j == 0) || i >= g.Kx) r = cmk<Cx>((R)0, (R)0);
     out[idx] = cmk<cplx<S>>((S)r.x, (S)r.y);
@@ -475,11 +477,12 @@ __global__ void __launch_bounds__(NTHR) k_col_drag_ct(DragArgs a, Geom g, const 
   Cx* tw = bv + C * NY;
   using L = CtLayout<C, 1, C, true>;
   constexpr int TOT = C * NY;
-  const int i0 = xcd_tile(blockIdx.x, gridDim.x) * C;   // grid = Kxp / C
-  const int z = g.nz - 1;
+  const int i0 = xcd_tile(blockIdx.x, gridDim.x) * C;   // grid = (Kxp / C, members)
+  const int mb = blockIdx.y;
+  const int z = mb * g.nzm + g.nzm - 1;                  // bottom layer of member mb
   load_table<R>(tw, twy, ct_tw_size(NY));
-  const Cx* du = reinterpret_cast<const Cx*>(a.DU);
-  const Cx* dv = reinterpret_cast<const Cx*>(a.DV);
+  const Cx* du = reinterpret_cast<const Cx*>(a.DU) + (long)mb * NY * g.Kxp;
+  const Cx* dv = reinterpret_cast<const Cx*>(a.DV) + (long)mb * NY * g.Kxp;
   for (int e = threadIdx.x; e < TOT; e += NTHR) {
     const int c = e % C, j = e / C;
     bu[e] = du[(long)j * g.Kxp + i0 + c];
@@ -489,7 +492,7 @@ __global__ void __launch_bounds__(NTHR) k_col_drag_ct(DragArgs a, Geom g, const 
   ct_fft<R, NY, L, NTHR>(bu, tw, -1);
   ct_fft<R, NY, L, NTHR>(bv, tw, -1);
   cplx<S>* out = reinterpret_cast<cplx<S>*>(a.out);
-  const R cd = (R)a.cdrag;
+  const R cd = (R)a.cdrag[mb];
   const int nown = C * g.Ky;
   for (int o = threadIdx.x; o < nown; o += NTHR) {
     const int c = o % C, jc = o / C, i = i0 + c;
@@ -557,10 +560,12 @@ __global__ void __launch_bounds__(NTHR) k_row_maxabs_ct(const cplx<R>* __restric
   }
 }
 
-// Final max over the per-block partials (one workgroup, fixed order).
+// Final max over the per-block partials: workgroup b reduces partials
+// [b n, (b+1) n) into slot[b] (one per member of a batch).
 __global__ void __launch_bounds__(1024) k_max_final(const unsigned long long* __restrict__ partial, int n,
                                                     unsigned long long* __restrict__ slot) {
   __shared__ unsigned long long red[16];
+  partial += (long)blockIdx.x * n;
   unsigned long long m = 0;
   for (int i = threadIdx.x; i < n; i += blockDim.x) m = partial[i] > m ? partial[i] : m;
   for (int off = 32; off > 0; off >>= 1) {
@@ -573,7 +578,7 @@ __global__ void __launch_bounds__(1024) k_max_final(const unsigned long long* __
   if (threadIdx.x == 0) {
     unsigned long long mm = 0;
     for (int w = 0; w < (int)(blockDim.x + 63) / 64; ++w) mm = red[w] > mm ? red[w] : mm;
-    slot[0] = mm;
+    slot[blockIdx.x] = mm;
   }
 }
 

@@ -162,6 +162,7 @@ int row_threads(int nseq, int N2) {
 Geom make_geom(int nx, int ny, int nz, double Lx, double Ly) {
   Geom g;
   g.nx = nx; g.ny = ny; g.nz = nz;
+  g.nzm = nz;
   g.nxh = nx / 2 + 1;
   g.N2 = nx / 2;
   g.Kx = nx / 3;
@@ -250,7 +251,8 @@ struct qgn_ctx {
   int leith_nblk = 64;
   int ncu = 256;         // compute units of the device
   double c12 = 0.0;
-  double Hnz = 1.0;
+  int nm = 1;            // members stacked along z (g.nz = nm * g.nzm)
+  double *ekm = nullptr, *cdm = nullptr;  // per member r0 Htot/H_nz and C_d Htot/H_nz
   // compact arrays
   long ncmp = 0;         // complex elements of a compact array
   long nmix = 0;         // complex elements of a mixed (nz, ny, Kx) array
@@ -274,6 +276,16 @@ struct qgn_ctx {
   double* mp[5] = {};
   double* spec = nullptr;  // spectra accumulator, 2 nz nbins
   long spec_n = 0;
+  // device-controlled ARK43 attempts of a batch: per-member controller state
+  double *bt = nullptr, *bdt = nullptr, *berr0 = nullptr, *blast = nullptr;
+  long long* bcount = nullptr;  // steps, attempts, rejects per member (3 nm)
+  double* brep = nullptr;       // single-model controller report (6)
+  double* h_rep = nullptr;      // pinned copy of it
+  int* bint = nullptr;          // consecutive rejects, status, accept per member (3 nm)
+  bool batch_ready = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  double2* graph_q = nullptr;   // the buffer the graph takes as the state
   // stepper state (the reference's `save` variables)
   bool first = true, second = false;
   double err0 = 0.0, last_err = 0.0;
@@ -284,6 +296,17 @@ struct qgn_ctx {
   int ev_used = 0;
   double prof_ms = 0.0;
   int prof_count = 0;
+};
+
+// Several batches (e.g. one per grid size) advanced together: one graph
+// with each batch's attempt graph as an independent child.
+struct qgn_ens {
+  std::vector<qgn_ctx*> ctx;
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  std::vector<hipEvent_t> ready;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
 };
 
 namespace {
@@ -383,7 +406,7 @@ template <> const Tw<float>& tw_of<float>(qgn_ctx* c) { return c->twf; }
 // psi = P(q): GetPsi on compact arrays
 template <typename R>
 int launch_project(qgn_ctx* c, const double2* q, void* psi) {
-  const int nz = c->g.nz;
+  const int nz = c->g.nzm;
   int pp = 256 / nz;
   if (pp < 1) pp = 1;
   const int threads = ((pp * nz + 63) / 64) * 64;
@@ -391,7 +414,7 @@ int launch_project(qgn_ctx* c, const double2* q, void* psi) {
   const int blocks = (int)((plane + pp - 1) / pp);
   const size_t lds = 2 * (size_t)nz * nz * sizeof(double) + 2 * (size_t)nz * pp * sizeof(double2);
   return timed(c, QGN_K_PROJECT, [&] {
-    hipLaunchKernelGGL((k_project<R>), dim3(blocks), dim3(threads), lds, c->stream, q,
+    hipLaunchKernelGGL((k_project<R>), dim3(blocks, c->nm), dim3(threads), lds, c->stream, q,
                        reinterpret_cast<cplx<R>*>(psi), c->HM, c->MD, c->evals, 1.0 / c->p.Htot, pp, c->g);
   });
 }
@@ -429,11 +452,11 @@ int launch_stage_proj_mfma(qgn_ctx* c, const StageArgs& a, int kernel_id) {
   const long plane = (long)c->g.Ky * c->g.Kxp;
   const int blocks = (int)((plane + 15) / 16);
   return timed(c, kernel_id, [&] {
-    hipLaunchKernelGGL((k_stage_proj_mfma<float>), dim3(blocks), dim3(256), 0, c->stream, a, c->g, c->HM, c->MD,
+    hipLaunchKernelGGL((k_stage_proj_mfma<float>), dim3(blocks, c->nm), dim3(256), 0, c->stream, a, c->g, c->HM, c->MD,
                        c->evals, 1.0 / c->p.Htot, reinterpret_cast<float2*>(c->psi));
   });
 }
-bool use_mfma_projection(const qgn_ctx* c) { return c->p.precision == QGN_MIXED && c->g.nz <= 16; }
+bool use_mfma_projection(const qgn_ctx* c) { return c->p.precision == QGN_MIXED && c->g.nzm <= 16; }
 
 // GetRHS on compact arrays: out = RHS(qin)
 template <typename R>
@@ -501,12 +524,12 @@ int rhs_impl(qgn_ctx* c, const double2* qin, void* out, bool psi_ready) {
   ka.J = c->Jm; ka.q = qin; ka.psi = psi; ka.out = out;
   ka.ubar = c->ubar; ka.qybar = c->qybar;
   ka.beta = c->p.beta;
-  ka.ekman = c->p.r0 * c->p.Htot / c->Hnz;
+  ka.ekman = c->ekm;
   ka.A2 = c->p.A2; ka.A8 = c->p.A8;
   ka.explicit_visc = c->p.is_explicit;
   DragArgs da{};
   da.DU = c->DUm; da.DV = c->DVm; da.out = out;
-  da.cdrag = c->p.C_d * c->p.Htot / c->Hnz;
+  da.cdrag = c->cdm;
   with_size(g.ny, [&](auto nyc) {
     constexpr int NY = decltype(nyc)::value;
     using Cfg = ColCfg<R, NY>;
@@ -524,7 +547,7 @@ int rhs_impl(qgn_ctx* c, const double2* qin, void* out, bool psi_ready) {
     auto dk = k_col_drag_ct<R, R, NY, CD, NTD>;
     rc = allow_lds(dk, LDSD);
     if (rc) return;
-    hipLaunchKernelGGL(dk, dim3(g.Kxp / CD), dim3(NTD), LDSD, c->stream, da, g, tw.sy);
+    hipLaunchKernelGGL(dk, dim3(g.Kxp / CD, c->nm), dim3(NTD), LDSD, c->stream, da, g, tw.sy);
     rc = hipGetLastError() == hipSuccess ? 0 : fail(QGN_EHIP, "drag kernel launch failed");
   }, CtSizes{});
   CHK(rc);
@@ -538,8 +561,9 @@ int rhs(qgn_ctx* c, const double2* qin, void* out, bool psi_ready = false) {
   return rhs_impl<double>(c, qin, out, psi_ready);
 }
 
-// max over the physical grid of |c2r(E)| (unnormalised), FP64
-int errnorm(qgn_ctx* c, const double2* Ecmp, double* maxv) {
+// max over the physical grid of |c2r(E)| (unnormalised), FP64, per member
+// into c->errslot[m] as ordered bit patterns (device only)
+int errnorm_launch(qgn_ctx* c, const double2* Ecmp) {
   const Geom& g = c->g;
   ColInvArgs ca{};
   ca.nf = 1;
@@ -562,12 +586,20 @@ int errnorm(qgn_ctx* c, const double2* Ecmp, double* maxv) {
                            c->twd.sx, c->twd.r, c->errpart);
       });
       if (rc) return;
-      hipLaunchKernelGGL(k_max_final, dim3(1), dim3(1024), 0, c->stream, c->errpart, g.ny * g.nz / RW, c->errslot);
+      // RW divides ny, so every block's rows belong to one member
+      hipLaunchKernelGGL(k_max_final, dim3(c->nm), dim3(1024), 0, c->stream, c->errpart, g.ny * g.nzm / RW,
+                         c->errslot);
       rc = hipGetLastError() == hipSuccess ? 0 : fail(QGN_EHIP, "max reduction launch failed");
     }, CtSizes{});
     if (!hit) return fail(QGN_EINVAL, "nx/2 not supported by the error norm");
     CHK(rc);
   }
+  return 0;
+}
+
+// the same for a single model, read back to the host
+int errnorm(qgn_ctx* c, const double2* Ecmp, double* maxv) {
+  CHK(errnorm_launch(c, Ecmp));
   HIPCHK(hipMemcpyAsync(c->h_errslot, c->errslot, sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   unsigned long long bits = *c->h_errslot;
@@ -603,7 +635,7 @@ int stage(qgn_ctx* c, const StageArgs& a) {
 template <typename S>
 int launch_stage_proj(qgn_ctx* c, const StageArgs& a, bool* done) {
   *done = true;
-  const int nz = c->g.nz;
+  const int nz = c->g.nzm;
   int pp = 256 / nz;
   if (pp < 1) pp = 1;
   const int threads = ((pp * nz + 63) / 64) * 64;
@@ -612,7 +644,7 @@ int launch_stage_proj(qgn_ctx* c, const StageArgs& a, bool* done) {
   const size_t lds = 2 * (size_t)nz * nz * sizeof(double) + 2 * (size_t)nz * pp * sizeof(double2);
   void* psi = std::is_same<S, double>::value ? (void*)c->psi64 : c->psi;
   return timed(c, QGN_K_STAGE, [&] {
-    hipLaunchKernelGGL((k_stage_proj<S, S>), dim3(blocks), dim3(threads), lds, c->stream, a, c->g, c->HM, c->MD,
+    hipLaunchKernelGGL((k_stage_proj<S, S>), dim3(blocks, c->nm), dim3(threads), lds, c->stream, a, c->g, c->HM, c->MD,
                        c->evals, 1.0 / c->p.Htot, pp, reinterpret_cast<cplx<S>*>(psi));
   });
 }
@@ -660,78 +692,168 @@ int lincomb(qgn_ctx* c, const void* X0, const void* X1, const void* X2, const vo
 // ---------------------------------------------------------------------------
 // Steppers (the TimeStep include fragments)
 // ---------------------------------------------------------------------------
-int step_ark43(qgn_ctx* c, double* t, double* dt, bool qgleith) {
+// Stages 2-6 and the candidate update of one ARK43 attempt from N[0] =
+// RHS(q) (ARK43*.f90 stage and update blocks): Y_i, N_i, then E and qn.
+// dtv: per-member dt on the device (a batch), else dt.
+int ark43_stages(qgn_ctx* c, double dt, const double* dtv, const double* a8l) {
+  const qgn_params& p = c->p;
+  for (int i = 2; i <= 6; ++i) {
+    StageArgs a{};
+    a.q = c->q;
+    a.ns = i - 1;
+    for (int j = 1; j < i; ++j) {
+      a.N[j - 1] = c->N[j - 1];
+      a.Y[j - 1] = c->Y[j - 1];
+      a.ae[j - 1] = kArk.ae[i][j];
+      a.ai[j - 1] = kArk.ai[i][j];
+    }
+    a.dt = dt;
+    a.dtv = dtv;
+    a.a8l = a8l;
+    a.A8 = p.A8;
+    a.A2 = p.A2;
+    a.implicit_visc = 1;
+    a.out = c->Y[i - 1];
+    bool fused = false;
+    CHK(stage_proj(c, a, &fused));
+    if (!fused) CHK(stage(c, a));
+    CHK(rhs(c, c->Y[i - 1], c->N[i - 1], fused));
+  }
+  UpdateArgs u{};
+  u.q = c->q;
+  u.ns = 6;
+  for (int j = 1; j <= 6; ++j) {
+    u.N[j - 1] = c->N[j - 1];
+    u.Y[j - 1] = c->Y[j - 1];
+    u.b[j - 1] = kArk.b[j];
+    u.be[j - 1] = kArk.be[j];
+  }
+  u.dt = dt;
+  u.dtv = dtv;
+  u.a8l = a8l;
+  u.A8 = p.A8;
+  u.A2 = p.A2;
+  u.implicit_visc = 1;
+  u.E = c->E;
+  u.qn = c->qn;
+  return update(c, u);
+}
+
+int launch_control(qgn_ctx* c, double* report);
+
+// One device-controlled ARK43 attempt of every member of a batch; only
+// kernel launches, so it can be captured into a graph.
+int batch_attempt(qgn_ctx* c) {
   const qgn_params& p = c->p;
   const Geom& g = c->g;
-  if (c->first) {              // ARK43*.f90 FirstCall
-    c->err0 = p.TOL;
-    c->first = false;
-  }
+  const bool qgleith = p.scheme == QGN_ARK43_QGLEITH;
   const double* a8l = nullptr;
   if (qgleith) {
     CHK(leith(c, c->q));
     a8l = c->a8l;
   }
   CHK(rhs(c, c->q, c->N[0]));
-  const double reject_factor = qgleith ? 0.5 : 0.75;
-  const double safety = qgleith ? 0.2 : 0.75;
-  const int maxrej = p.max_rejects > 0 ? p.max_rejects : 1000;
-  int tries = 0;
+  CHK(ark43_stages(c, 0.0, c->bdt, a8l));
+  CHK(errnorm_launch(c, c->E));
+  CHK(launch_control(c, nullptr));
+  const long per = (long)g.nzm * g.Ky * g.Kxp;
+  hipLaunchKernelGGL(k_batch_commit, dim3(grid1d(per) > 2048 ? 2048 : grid1d(per), c->nm), dim3(256), 0, c->stream,
+                     c->q, c->qn, c->bint + 2 * c->nm, g);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// The ARK43 step control of every member (k_batch_control) after the error
+// norm of an attempt; report: the single model's outcome, or nullptr.
+int launch_control(qgn_ctx* c, double* report) {
+  const qgn_params& p = c->p;
+  const Geom& g = c->g;
+  const bool qgleith = p.scheme == QGN_ARK43_QGLEITH;
+  BatchCtl k{};
+  const int nm = c->nm;
+  k.t = c->bt; k.dt = c->bdt; k.err0 = c->berr0; k.last = c->blast;
+  k.steps = c->bcount; k.attempts = c->bcount + nm; k.rejects = c->bcount + 2 * nm;
+  k.nrej = c->bint; k.status = c->bint + nm; k.accept = c->bint + 2 * nm;
+  k.maxbits = c->errslot;
+  k.TOL = p.TOL;
+  k.safety = qgleith ? 0.2 : 0.75;
+  k.reject_factor = qgleith ? 0.5 : 0.75;
+  k.dt_max = p.dt_max;
+  k.nn = (double)((long)g.nx * g.ny);
+  k.max_rejects = p.max_rejects > 0 ? p.max_rejects : 1000;
+  k.nm = nm;
+  k.report = report;
+  hipLaunchKernelGGL(k_batch_control, dim3((nm + 63) / 64), dim3(64), 0, c->stream, k);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// qgn_time_step swaps the state buffers; put the state back where the
+// captured graph reads it.
+int graph_state_sync(qgn_ctx* c) {
+  if (c->gexec && c->q != c->graph_q) {
+    HIPCHK(hipMemcpyAsync(c->graph_q, c->q, (size_t)c->ncmp * sizeof(double2), hipMemcpyDeviceToDevice, c->stream));
+    std::swap(c->q, c->qn);
+  }
+  return 0;
+}
+
+// Capture batch_attempt into c->graph / c->gexec.
+int build_graph(qgn_ctx* c) {
+  if (c->gexec) return 0;
+  const int prof = c->prof_id;
+  c->prof_id = -1;   // no event records inside the graph
+  HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+  const int rc = batch_attempt(c);
+  hipGraph_t gr = nullptr;
+  const hipError_t e = hipStreamEndCapture(c->stream, &gr);
+  c->prof_id = prof;
+  if (rc) {
+    if (gr) (void)hipGraphDestroy(gr);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(QGN_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
+  c->graph = gr;
+  c->graph_q = c->q;
+  HIPCHK(hipGraphInstantiate(&c->gexec, gr, nullptr, nullptr, 0));
+  return 0;
+}
+
+// ARK43 / ARK43_QGLeith TimeStep.  The accept/reject decision and the
+// PI.3.4 update run on the device in the controller kernel that batches
+// use (one member here), so a member of a batch and the same model alone
+// take identical steps; one small read-back per attempt.
+int step_ark43(qgn_ctx* c, double* t, double* dt, bool qgleith) {
+  const qgn_params& p = c->p;
+  hipLaunchKernelGGL(k_control_set, dim3(1), dim3(64), 0, c->stream, c->bt, c->bdt, c->berr0, c->bint,
+                     c->bint + 1, *t, *dt, c->first ? p.TOL : -1.0);   // ARK43*.f90 FirstCall: err0 = TOL
+  HIPCHK(hipGetLastError());
+  c->first = false;
+  const double* a8l = nullptr;
+  if (qgleith) {
+    CHK(leith(c, c->q));
+    a8l = c->a8l;
+  }
+  CHK(rhs(c, c->q, c->N[0]));
   while (true) {
-    for (int i = 2; i <= 6; ++i) {
-      StageArgs a{};
-      a.q = c->q;
-      a.ns = i - 1;
-      for (int j = 1; j < i; ++j) {
-        a.N[j - 1] = c->N[j - 1];
-        a.Y[j - 1] = c->Y[j - 1];
-        a.ae[j - 1] = kArk.ae[i][j];
-        a.ai[j - 1] = kArk.ai[i][j];
-      }
-      a.dt = *dt;
-      a.a8l = a8l;
-      a.A8 = p.A8;
-      a.A2 = p.A2;
-      a.implicit_visc = 1;
-      a.out = c->Y[i - 1];
-      bool fused = false;
-      CHK(stage_proj(c, a, &fused));
-      if (!fused) CHK(stage(c, a));
-      CHK(rhs(c, c->Y[i - 1], c->N[i - 1], fused));
-    }
-    UpdateArgs u{};
-    u.q = c->q;
-    u.ns = 6;
-    for (int j = 1; j <= 6; ++j) {
-      u.N[j - 1] = c->N[j - 1];
-      u.Y[j - 1] = c->Y[j - 1];
-      u.b[j - 1] = kArk.b[j];
-      u.be[j - 1] = kArk.be[j];
-    }
-    u.dt = *dt;
-    u.a8l = a8l;
-    u.A8 = p.A8;
-    u.A2 = p.A2;
-    u.implicit_visc = 1;
-    u.E = c->E;
-    u.qn = c->qn;
-    CHK(update(c, u));
-    double mx;
-    CHK(errnorm(c, c->E, &mx));
-    const double err1 = *dt * mx / (double)((long)g.nx * g.ny);
-    c->last_err = err1;
+    CHK(ark43_stages(c, 0.0, c->bdt, a8l));
+    CHK(errnorm_launch(c, c->E));
+    CHK(launch_control(c, c->brep));
+    HIPCHK(hipMemcpyAsync(c->h_rep, c->brep, 6 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const double* r = c->h_rep;
+    c->last_err = r[3];
     c->st.attempts++;
-    if (!std::isfinite(err1)) return fail(QGN_ENONFINITE, "ARK43: non-finite error norm");
-    if (err1 > p.TOL) {        // reject
-      *dt = reject_factor * *dt;
+    *dt = r[2];
+    if (r[4] == 1.0) return fail(QGN_ENONFINITE, "ARK43: non-finite error norm");
+    if (r[0] == 0.0) {         // reject
       c->st.rejects++;
-      if (++tries > maxrej) return fail(QGN_EREJECT, "ARK43: too many rejected steps");
+      if (r[4] == 2.0) return fail(QGN_EREJECT, "ARK43: too many rejected steps");
       continue;
     }
     std::swap(c->q, c->qn);
-    *t = *t + *dt;
-    *dt = std::fmin(*dt * std::pow(safety * p.TOL / err1, 0.075) * std::pow(c->err0 / err1, 0.1), p.dt_max);
-    c->err0 = err1;
+    *t = r[1];
+    c->err0 = r[5];
     c->st.steps++;
     return 0;
   }
@@ -909,6 +1031,12 @@ void destroy_ctx(qgn_ctx* c) {
   for (int i = 0; i < 5; ++i)
     if (c->mp[i]) (void)hipFree(c->mp[i]);
   if (c->spec) (void)hipFree(c->spec);
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  void* bp[] = {c->ekm, c->cdm, c->bt, c->bdt, c->berr0, c->blast, c->bcount, c->bint, c->brep};
+  for (void* x : bp)
+    if (x) (void)hipFree(x);
+  if (c->h_rep) (void)hipHostFree(c->h_rep);
   if (c->h_errslot) (void)hipHostFree(c->h_errslot);
   for (auto& e : c->ev) {
     (void)hipEventDestroy(e.first);
@@ -926,20 +1054,26 @@ int upload(double** dst, const double* src, size_t n) {
   return 0;
 }
 
-int create_impl(qgn_ctx* c, const qgn_params* p, const double* H, const double* Modes, const double* EVals,
-                const double* uBar, const double* qyBar) {
+// members > 1: a batch; H, EVals, uBar, qyBar hold nz values per member and
+// Modes nz*nz per member, member after member; r0 (may be null) one each.
+int create_impl(qgn_ctx* c, const qgn_params* p, int members, const double* H, const double* Modes,
+                const double* EVals, const double* uBar, const double* qyBar, const double* r0) {
   c->p = *p;
+  if (members < 1 || members > 4096) return fail(QGN_EINVAL, "members must be in [1, 4096]");
   std::string why;
   if (!size_ok(p->nx, p->ny, &why)) return fail(QGN_EINVAL, why);
   if (!rhs_size_ok(p->nx, p->ny))
     return fail(QGN_EINVAL, "the RHS supports nx/2 and ny in {32,48,64,96,128,192,256,384,512,768,1024,1536,2048} "
                             "with nx, ny <= 2048");
-  if (p->nz < 2 || p->nz > 64) return fail(QGN_EINVAL, "nz must be in [1, 64]");
+  if (p->nz < 2 || p->nz > 64) return fail(QGN_EINVAL, "nz must be in [2, 64]");
   if (p->scheme < QGN_AB3 || p->scheme > QGN_ARK43_QGLEITH) return fail(QGN_EINVAL, "unknown scheme");
   if (p->precision != QGN_FP64 && p->precision != QGN_MIXED) return fail(QGN_EINVAL, "unknown precision");
   if (!(p->Lx > 0) || !(p->Ly > 0) || !(p->Htot > 0)) return fail(QGN_EINVAL, "Lx, Ly, Htot must be positive");
-  const int nz = p->nz;
+  const int nzm = p->nz;
+  const int nz = nzm * members;   // layers of the whole batch
+  c->nm = members;
   c->g = make_geom(p->nx, p->ny, nz, p->Lx, p->Ly);
+  c->g.nzm = nzm;
   make_plan(p->ny, &c->py);
   make_plan(p->nx / 2, &c->px);
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -951,19 +1085,29 @@ int create_impl(qgn_ctx* c, const qgn_params* p, const double* H, const double* 
   }
   CHK(make_tw<double>(p->nx, p->ny, &c->twd));
   CHK(make_tw<float>(p->nx, p->ny, &c->twf));
-  // constants
-  std::vector<double> hm((size_t)nz * nz), md((size_t)nz * nz);
-  for (int k = 0; k < nz; ++k)
-    for (int kk = 0; kk < nz; ++kk) {
-      hm[(size_t)k * nz + kk] = H[kk] * Modes[(size_t)k * nz + kk];  // H(kk)*Modes(kk,k), column-major
-      md[(size_t)k * nz + kk] = Modes[(size_t)kk * nz + k];          // Modes(k,kk)
-    }
+  // constants, member after member
+  const size_t nn2 = (size_t)nzm * nzm;
+  std::vector<double> hm(nn2 * members), md(nn2 * members), ekm(members), cdm(members);
+  for (int m = 0; m < members; ++m) {
+    const double* Hm = H + (size_t)m * nzm;
+    const double* Mm = Modes + nn2 * m;
+    for (int k = 0; k < nzm; ++k)
+      for (int kk = 0; kk < nzm; ++kk) {
+        hm[nn2 * m + (size_t)k * nzm + kk] = Hm[kk] * Mm[(size_t)k * nzm + kk];  // H(kk)*Modes(kk,k), column-major
+        md[nn2 * m + (size_t)k * nzm + kk] = Mm[(size_t)kk * nzm + k];           // Modes(k,kk)
+      }
+    const double hb = Hm[nzm - 1];
+    if (!(hb > 0)) return fail(QGN_EINVAL, "layer thicknesses must be positive");
+    ekm[m] = (r0 ? r0[m] : p->r0) * p->Htot / hb;   // QGN_Module.f90:282
+    cdm[m] = p->C_d * p->Htot / hb;                 // QGN_Module.f90:291-292
+  }
   CHK(upload(&c->HM, hm.data(), hm.size()));
   CHK(upload(&c->MD, md.data(), md.size()));
   CHK(upload(&c->evals, EVals, nz));
   CHK(upload(&c->ubar, uBar, nz));
   CHK(upload(&c->qybar, qyBar, nz));
-  c->Hnz = H[nz - 1];
+  CHK(upload(&c->ekm, ekm.data(), members));
+  CHK(upload(&c->cdm, cdm.data(), members));
   CHK(alloc((void**)&c->a8l, nz * sizeof(double)));
   {
     std::vector<double> a8(nz, p->A8);
@@ -997,10 +1141,18 @@ int create_impl(qgn_ctx* c, const qgn_params* p, const double* H, const double* 
   CHK(alloc(&c->QXm, mb));
   CHK(alloc(&c->QYm, mb));
   CHK(alloc(&c->Jm, mb));
-  CHK(alloc(&c->DUm, (size_t)g.ny * g.Kxp * rs));
-  CHK(alloc(&c->DVm, (size_t)g.ny * g.Kxp * rs));
+  CHK(alloc(&c->DUm, (size_t)members * g.ny * g.Kxp * rs));
+  CHK(alloc(&c->DVm, (size_t)members * g.ny * g.Kxp * rs));
   CHK(alloc((void**)&c->Em, (size_t)c->nmix * sizeof(double2)));
-  CHK(alloc((void**)&c->errslot, sizeof(unsigned long long)));
+  CHK(alloc((void**)&c->errslot, (size_t)members * sizeof(unsigned long long)));
+  CHK(alloc((void**)&c->bt, members * sizeof(double)));
+  CHK(alloc((void**)&c->bdt, members * sizeof(double)));
+  CHK(alloc((void**)&c->berr0, members * sizeof(double)));
+  CHK(alloc((void**)&c->blast, members * sizeof(double)));
+  CHK(alloc((void**)&c->bcount, 3 * (size_t)members * sizeof(long long)));
+  CHK(alloc((void**)&c->bint, 3 * (size_t)members * sizeof(int)));
+  CHK(alloc((void**)&c->brep, 8 * sizeof(double)));
+  HIPCHK(hipHostMalloc((void**)&c->h_rep, 8 * sizeof(double), hipHostMallocDefault));
   CHK(alloc((void**)&c->errpart, (size_t)g.ny * nz * sizeof(unsigned long long)));
   HIPCHK(hipHostMalloc((void**)&c->h_errslot, sizeof(unsigned long long), hipHostMallocDefault));
   const size_t fbytes = (size_t)nz * g.ny * g.nxh * sizeof(double2);
@@ -1112,10 +1264,15 @@ const char* qgn_version(void) { return "qgn-mi355x 0.1 (gfx950)"; }
 
 int qgn_create(qgn_ctx** ctx, const qgn_params* p, const double* H, const double* Modes, const double* EVals,
                const double* uBar, const double* qyBar) {
+  return qgn_create_batch(ctx, p, 1, H, Modes, EVals, uBar, qyBar, nullptr);
+}
+
+int qgn_create_batch(qgn_ctx** ctx, const qgn_params* p, int members, const double* H, const double* Modes,
+                     const double* EVals, const double* uBar, const double* qyBar, const double* r0) {
   if (!ctx || !p || !H || !Modes || !EVals || !uBar || !qyBar) return fail(QGN_EINVAL, "null argument");
   *ctx = nullptr;
   qgn_ctx* c = new qgn_ctx();
-  int r = create_impl(c, p, H, Modes, EVals, uBar, qyBar);
+  int r = create_impl(c, p, members, H, Modes, EVals, uBar, qyBar, r0);
   if (r != 0) {
     std::string msg = g_err;
     destroy_ctx(c);
@@ -1158,6 +1315,7 @@ int qgn_get_state(qgn_ctx* c, double* q_hat) {
 
 int qgn_time_step(qgn_ctx* c, double* t, double* dt) {
   if (!c || !t || !dt) return fail(QGN_EINVAL, "null argument");
+  if (c->nm > 1) return fail(QGN_EINVAL, "a batch of several members steps with qgn_batch_step");
   switch (c->p.scheme) {
     case QGN_AB3: return step_ab3(c, t, dt);
     case QGN_RK4: return step_rk4(c, t, dt);
@@ -1194,7 +1352,58 @@ int qgn_fields(qgn_ctx* c, double* q_phys, double* psi_phys) {
 
 int qgn_modal_fields(qgn_ctx* c, double* q_mode, double* psi_mode, double* j_mode) {
   if (!c) return fail(QGN_EINVAL, "null argument");
+  if (c->nm > 1) return fail(QGN_EINVAL, "qgn_modal_fields takes a single-member context");
   return modal_fields(c, q_mode, psi_mode, j_mode);
+}
+
+int qgn_batch_set_time(qgn_ctx* c, const double* t, const double* dt) {
+  if (!c || !t || !dt) return fail(QGN_EINVAL, "null argument");
+  const int nm = c->nm;
+  std::vector<double> e0(nm, c->p.TOL);   // FirstCall: err0 = TOL
+  std::vector<long long> z64(3 * (size_t)nm, 0);
+  std::vector<int> z32(3 * (size_t)nm, 0);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(c->bt, t, nm * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->bdt, dt, nm * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->berr0, e0.data(), nm * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->bcount, z64.data(), z64.size() * sizeof(long long), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->bint, z32.data(), z32.size() * sizeof(int), hipMemcpyHostToDevice));
+  c->batch_ready = true;
+  return 0;
+}
+
+static int batch_check(qgn_ctx* c) {
+  if (c->p.scheme != QGN_ARK43 && c->p.scheme != QGN_ARK43_QGLEITH)
+    return fail(QGN_EINVAL, "batch stepping supports ARK43 and ARK43_QGLeith");
+  if (!c->batch_ready) return fail(QGN_EINVAL, "call qgn_batch_set_time first");
+  CHK(build_graph(c));
+  return graph_state_sync(c);
+}
+
+int qgn_batch_step(qgn_ctx* c, int attempts) {
+  if (!c || attempts < 0) return fail(QGN_EINVAL, "bad argument");
+  CHK(batch_check(c));
+  for (int a = 0; a < attempts; ++a) HIPCHK(hipGraphLaunch(c->gexec, c->stream));
+  c->st.rhs_evals += 6LL * attempts * c->nm;
+  return 0;
+}
+
+int qgn_batch_get_time(qgn_ctx* c, double* t, double* dt, long long* steps, long long* rejects, int* status) {
+  if (!c) return fail(QGN_EINVAL, "null argument");
+  const int nm = c->nm;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (t) HIPCHK(hipMemcpy(t, c->bt, nm * sizeof(double), hipMemcpyDeviceToHost));
+  if (dt) HIPCHK(hipMemcpy(dt, c->bdt, nm * sizeof(double), hipMemcpyDeviceToHost));
+  if (steps) HIPCHK(hipMemcpy(steps, c->bcount, nm * sizeof(long long), hipMemcpyDeviceToHost));
+  if (rejects) HIPCHK(hipMemcpy(rejects, c->bcount + 2 * nm, nm * sizeof(long long), hipMemcpyDeviceToHost));
+  std::vector<int> st(nm);
+  HIPCHK(hipMemcpy(st.data(), c->bint + nm, nm * sizeof(int), hipMemcpyDeviceToHost));
+  if (status) std::memcpy(status, st.data(), nm * sizeof(int));
+  for (int m = 0; m < nm; ++m) {
+    if (st[m] == 1) return fail(QGN_ENONFINITE, "batch member " + std::to_string(m) + ": non-finite error norm");
+    if (st[m] == 2) return fail(QGN_EREJECT, "batch member " + std::to_string(m) + ": too many rejected steps");
+  }
+  return 0;
 }
 
 int qgn_spectra(qgn_ctx* c, int nbins, double* ke, double* ens) {
@@ -1397,6 +1606,72 @@ int qgn_dft_forward_device(qgn_dft* h, const void* grid, void* spec, void* strea
 int qgn_dft_backward_device(qgn_dft* h, const void* spec, void* grid, void* stream) {
   if (!h || !grid || !spec) return fail(QGN_EINVAL, "null argument");
   return dft_run(h, spec, grid, stream ? (hipStream_t)stream : h->stream, false);
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int qgn_ensemble_destroy(qgn_ens* e) {
+  if (!e) return 0;
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  if (e->exec) (void)hipGraphExecDestroy(e->exec);
+  if (e->graph) (void)hipGraphDestroy(e->graph);
+  for (hipEvent_t x : e->ready) (void)hipEventDestroy(x);
+  if (e->done) (void)hipEventDestroy(e->done);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  delete e;
+  return 0;
+}
+
+static int ensemble_create_impl(qgn_ens* e, qgn_ctx* const* ctxs, int n) {
+  HIPCHK(hipGraphCreate(&e->graph, 0));
+  for (int i = 0; i < n; ++i) {
+    if (!ctxs[i]) return fail(QGN_EINVAL, "null context");
+    CHK(batch_check(ctxs[i]));
+    e->ctx.push_back(ctxs[i]);
+    hipGraphNode_t node;
+    HIPCHK(hipGraphAddChildGraphNode(&node, e->graph, nullptr, 0, ctxs[i]->graph));
+    hipEvent_t ev;
+    HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    e->ready.push_back(ev);
+  }
+  HIPCHK(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
+  HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+  HIPCHK(hipEventCreateWithFlags(&e->done, hipEventDisableTiming));
+  return 0;
+}
+
+int qgn_ensemble_create(qgn_ens** out, qgn_ctx* const* ctxs, int n) {
+  if (!out || !ctxs || n < 1) return fail(QGN_EINVAL, "bad argument");
+  *out = nullptr;
+  qgn_ens* e = new qgn_ens();
+  const int rc = ensemble_create_impl(e, ctxs, n);
+  if (rc) {
+    std::string msg = g_err;
+    qgn_ensemble_destroy(e);
+    g_err = msg;
+    return rc;
+  }
+  *out = e;
+  return 0;
+}
+
+int qgn_ensemble_step(qgn_ens* e, int attempts) {
+  if (!e || attempts < 0) return fail(QGN_EINVAL, "bad argument");
+  // after whatever each context has queued, and before what it queues next
+  for (size_t i = 0; i < e->ctx.size(); ++i) {
+    CHK(graph_state_sync(e->ctx[i]));
+    HIPCHK(hipEventRecord(e->ready[i], e->ctx[i]->stream));
+    HIPCHK(hipStreamWaitEvent(e->stream, e->ready[i], 0));
+  }
+  for (int a = 0; a < attempts; ++a) HIPCHK(hipGraphLaunch(e->exec, e->stream));
+  HIPCHK(hipEventRecord(e->done, e->stream));
+  for (qgn_ctx* c : e->ctx) {
+    HIPCHK(hipStreamWaitEvent(c->stream, e->done, 0));
+    c->st.rhs_evals += 6LL * attempts * c->nm;
+  }
+  return 0;
 }
 
 }  // extern "C"

@@ -87,9 +87,12 @@ def spectra_bins(p: Params):
 class QGN:
     """One QGN model instance on the GPU (module state of QGN_Module.f90:11-41)."""
 
+    members = 1
+
     def __init__(self, p: Params, H, Modes, EVals, uBar, qyBar):
         self.lib = _lib.load()
         self.p = p
+        self.nzt = p.nz
         self.nxh = p.nx // 2 + 1
         self.H = np.ascontiguousarray(H, dtype=np.float64)
         modes_f = np.ascontiguousarray(np.asarray(Modes, dtype=np.float64).T)  # Fortran order
@@ -107,10 +110,10 @@ class QGN:
 
     # ---- shapes ----------------------------------------------------------
     def spec_shape(self):
-        return (self.p.nz, self.p.ny, self.nxh)
+        return (self.nzt, self.p.ny, self.nxh)
 
     def phys_shape(self):
-        return (self.p.nz, self.p.ny, self.p.nx)
+        return (self.nzt, self.p.ny, self.p.nx)
 
     def _spec_in(self, a):
         a = np.ascontiguousarray(a, dtype=np.complex128)
@@ -174,14 +177,14 @@ class QGN:
         p = self.p
         if nbins is None:
             nbins = spectra_bins(p)
-        ke = np.empty((p.nz, nbins))
-        ens = np.empty((p.nz, nbins))
+        ke = np.empty((self.nzt, nbins))
+        ens = np.empty((self.nzt, nbins))
         check(self.lib.qgn_spectra(self._ctx, int(nbins), dptr(ke), dptr(ens)))
         return ke, ens
 
     def leith(self):
         """A8L(nz) of the current state (ARK43_QGLeith.f90:62-73)."""
-        out = np.empty(self.p.nz)
+        out = np.empty(self.nzt)
         check(self.lib.qgn_leith(self._ctx, dptr(out)))
         return out
 
@@ -209,6 +212,77 @@ class QGN:
         if self._ctx:
             self.lib.qgn_destroy(self._ctx)
             self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch(QGN):
+    """Members of one grid size stacked along z, each with its own vertical
+    structure and Ekman coefficient, stepped by device-controlled ARK43
+    attempts captured in a HIP graph (no counterpart in the reference).
+
+    members: sequence of dicts with H, Modes, EVals, uBar, qyBar and
+    optionally r0.  State arrays hold members * nz layers, member after
+    member."""
+
+    def __init__(self, p: Params, members):
+        self.lib = _lib.load()
+        self.p = p
+        self.members = len(members)
+        self.nzt = p.nz * self.members
+        self.nxh = p.nx // 2 + 1
+        cat = lambda key: np.ascontiguousarray(np.concatenate([np.ravel(m[key]) for m in members]), dtype=np.float64)
+        modes_f = np.ascontiguousarray(np.concatenate(
+            [np.asarray(m["Modes"], dtype=np.float64).T.ravel() for m in members]))   # Fortran order each
+        r0 = np.array([m.get("r0", p.r0) for m in members], dtype=np.float64)
+        self.H = cat("H")
+        self._ctx = C.c_void_p()
+        cp = p.to_c()
+        check(self.lib.qgn_create_batch(C.byref(self._ctx), C.byref(cp), self.members, dptr(self.H), dptr(modes_f),
+                                        dptr(cat("EVals")), dptr(cat("uBar")), dptr(cat("qyBar")), dptr(r0)))
+
+    def set_time(self, t, dt):
+        t = np.ascontiguousarray(np.broadcast_to(np.asarray(t, dtype=np.float64), (self.members,)))
+        dt = np.ascontiguousarray(np.broadcast_to(np.asarray(dt, dtype=np.float64), (self.members,)))
+        check(self.lib.qgn_batch_set_time(self._ctx, dptr(t), dptr(dt)))
+
+    def step(self, attempts):
+        """Enqueue `attempts` ARK43 attempts of every member (asynchronous)."""
+        check(self.lib.qgn_batch_step(self._ctx, int(attempts)))
+
+    def times(self):
+        """Per-member t, dt, accepted steps, rejected attempts, status (waits)."""
+        n = self.members
+        t, dt = np.empty(n), np.empty(n)
+        steps, rej = np.empty(n, dtype=np.int64), np.empty(n, dtype=np.int64)
+        st = np.empty(n, dtype=np.int32)
+        check(self.lib.qgn_batch_get_time(self._ctx, dptr(t), dptr(dt), steps.ctypes.data_as(C.POINTER(C.c_longlong)),
+                                          rej.ctypes.data_as(C.POINTER(C.c_longlong)),
+                                          st.ctypes.data_as(C.POINTER(C.c_int))))
+        return dict(t=t, dt=dt, steps=steps, rejects=rej, status=st)
+
+
+class Ensemble:
+    """Several batches advanced by one graph launch per attempt."""
+
+    def __init__(self, batches):
+        self.lib = _lib.load()
+        self.batches = list(batches)
+        arr = (C.c_void_p * len(self.batches))(*[b._ctx.value for b in self.batches])
+        self._h = C.c_void_p()
+        check(self.lib.qgn_ensemble_create(C.byref(self._h), arr, len(self.batches)))
+
+    def step(self, attempts):
+        check(self.lib.qgn_ensemble_step(self._h, int(attempts)))
+
+    def close(self):
+        if self._h:
+            self.lib.qgn_ensemble_destroy(self._h)
+            self._h = C.c_void_p()
 
     def __del__(self):
         try:

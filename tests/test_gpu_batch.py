@@ -1,0 +1,157 @@
+"""Batches and ensembles: members stacked along z with their own vertical
+structure, Ekman drag and dt, stepped by device-controlled ARK43 attempts in
+a HIP graph.  Each member is checked against the same model run alone
+through qgn_time_step (whose parity with the oracle the other GPU tests
+establish), and one member directly against the oracle."""
+import time
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _single(p, m):
+    import dataclasses
+    from qgn_fortran_amd import QGN
+    return QGN(dataclasses.replace(p, r0=m["r0"]), m["H"], m["Modes"], m["EVals"], m["uBar"], m["qyBar"])
+
+
+def _check_members(batch, group, q0_hat, which, tol_state, tol_t):
+    """Run members `which` alone for as many accepted steps as the batch took."""
+    info = batch.times()
+    state = batch.get_state()
+    nz = group.params.nz
+    for m in which:
+        steps = int(info["steps"][m])
+        assert info["status"][m] == 0
+        single = _single(group.params, group.members[m])
+        single.set_state(q0_hat[m * nz:(m + 1) * nz])
+        t, dt = 0.0, group.dt0
+        for _ in range(steps):
+            t, dt = single.time_step(t, dt)
+        err = rel_l2(state[m * nz:(m + 1) * nz], single.get_state())
+        print(f"member {m}: {steps} steps, {int(info['rejects'][m])} rejects, t {info['t'][m]:.6e}, "
+              f"state rel L2 {err:.2e}", flush=True)
+        assert steps >= 1
+        assert abs(info["t"][m] - t) <= tol_t * t
+        assert err < tol_state
+        single.close()
+    return info
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_members_match_single_runs(precision):
+    """4 members at 96^2 x 4 with different deformation radii, Ekman drag
+    and initial states, 10 attempts from dt = 60000 s (rejections occur)."""
+    from qgn_fortran_amd import Batch, configs
+    print("batch", precision, flush=True)
+    group = configs.ensemble96(sizes=(96,), per_size=4, nz=4, precision=precision, seed=11)[0]
+    group.params.TOL = 1e-11
+    group.dt0 = 60000.0
+    group.q0 *= 10.0
+    b = Batch(group.params, group.members)
+    q0_hat = b.initialize(group.q0)
+    b.set_time(0.0, group.dt0)
+    b.step(10)
+    tol = 1e-10 if precision == "fp64" else 1e-6
+    info = _check_members(b, group, q0_hat, range(4), tol, 1e-12 if precision == "fp64" else 1e-6)
+    assert info["rejects"].sum() > 0       # the reject branch ran
+    if precision == "fp64":
+        # member 2 against the oracle directly
+        from oracle import qgn_oracle as orc
+        p, m = group.params, group.members[2]
+        po = orc.Params(nx=p.nx, ny=p.ny, nz=p.nz, Lx=p.Lx, Ly=p.Ly, Htot=p.Htot, beta=p.beta, r0=m["r0"],
+                        C_d=p.C_d, TOL=p.TOL)
+        om = orc.Model(po, m["H"], m["Modes"], m["EVals"], m["uBar"], m["qyBar"])
+        st = orc.ARK43(om, leith=True)
+        q, t, dt = q0_hat[8:12].copy(), 0.0, group.dt0
+        for _ in range(int(info["steps"][2])):
+            q, t, dt = st.step(q, t, dt)
+        err = rel_l2(b.get_state()[8:12], q)
+        print("member 2 vs oracle:", err, flush=True)
+        assert err < 1e-10 and abs(info["t"][2] - t) <= 1e-12 * t
+    b.close()
+
+
+def test_batch_set_time_restarts_controller():
+    """A second set_time resets counters and the controller memory: the same
+    attempts from the same state give the same result."""
+    from qgn_fortran_amd import Batch, configs
+    group = configs.ensemble96(sizes=(64,), per_size=2, nz=3, seed=5)[0]
+    b = Batch(group.params, group.members)
+    q0_hat = b.initialize(group.q0)
+    out = []
+    for _ in range(2):
+        b.set_state(q0_hat)
+        b.set_time([0.0, 100.0], [750.0, 500.0])
+        b.step(4)
+        info = b.times()
+        out.append((b.get_state(), info))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1]["t"], out[1][1]["t"]) and np.array_equal(out[0][1]["steps"], out[1][1]["steps"])
+    assert out[0][1]["t"][1] > 100.0
+    b.close()
+
+
+def test_time_step_then_batch_step_on_one_member():
+    """qgn_time_step and qgn_batch_step on the same one-member batch: the
+    graph reads the state wherever qgn_time_step's buffer swaps left it, so
+    the attempts that follow match a fresh batch started from that state."""
+    from qgn_fortran_amd import Batch, configs
+    group = configs.ensemble96(sizes=(64,), per_size=1, nz=3, seed=9)[0]
+    b = Batch(group.params, group.members)
+    b.initialize(group.q0)
+    b.set_time(0.0, 750.0)
+    b.step(1)                                   # captures the graph
+    info = b.times()
+    t, dt = b.time_step(float(info["t"][0]), float(info["dt"][0]))   # swaps the state buffers
+    s2 = b.get_state()
+    b.set_time(t, dt)
+    b.step(0)
+    assert np.array_equal(b.get_state(), s2)
+    b.step(3)
+    fresh = Batch(group.params, group.members)
+    fresh.set_state(s2)
+    fresh.set_time(t, dt)
+    fresh.step(3)
+    assert np.array_equal(b.get_state(), fresh.get_state())
+    assert np.array_equal(b.times()["t"], fresh.times()["t"])
+    b.close()
+    fresh.close()
+
+
+def test_ensemble96():
+    """The 96-member ensemble (32 members of 8 layers on each of 128^2,
+    256^2, 384^2): all members advance by one graph launch per attempt; one
+    member of each size is checked against its single run."""
+    from qgn_fortran_amd import Batch, Ensemble, configs
+    groups = configs.ensemble96()
+    batches, q0s = [], []
+    for g in groups:
+        b = Batch(g.params, g.members)
+        q0s.append(b.initialize(g.q0))
+        b.set_time(0.0, g.dt0)
+        batches.append(b)
+    ens = Ensemble(batches)
+    ens.step(2)                                  # warm-up
+    for b in batches:
+        b.times()
+    t0 = time.perf_counter()
+    attempts = 20
+    ens.step(attempts)
+    infos = [b.times() for b in batches]
+    wall = time.perf_counter() - t0
+    steps = sum(int(i["steps"].sum()) for i in infos)
+    days = sum(float(i["t"].sum()) for i in infos) / 86400.0
+    print(f"ensemble96: {attempts + 2} attempts, {steps} accepted member-steps, "
+          f"{attempts} attempts in {wall * 1e3:.1f} ms ({attempts / wall:.1f} attempts/s)", flush=True)
+    for g, b, q0, i in zip(groups, batches, q0s, infos):
+        assert np.all(i["status"] == 0) and np.all(i["steps"] >= 1)
+        _check_members(b, g, q0, [7], 1e-10, 1e-12)
+    assert days > 0
+    ens.close()
+    for b in batches:
+        b.close()
