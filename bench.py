@@ -4,8 +4,9 @@ Workload: expstrat_1024x16 (1024^2 x 16 layers, exponential stratification,
 surface-intensified shear, quadratic + Ekman drag, ARK43_QGLeith adaptive).
 A step is one accepted TimeStep (rejected attempts included in the time).
 Prints one JSON line; ``value`` is simulated days per wall-second of the
-mixed-precision path, with the FP64 path measured on the same GPU from the
-same spun-up state.
+mixed-precision path through the graph-captured attempts (qgn_batch_step),
+with the per-call qgn_time_step path and the FP64 path measured on the same
+GPU from the same spun-up state.
 """
 from __future__ import annotations
 
@@ -38,25 +39,47 @@ def kernel_bytes(kernel, g, rs):
     raise ValueError(kernel)
 
 
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` from the committed PMC summary
+    (profiles/traffic_<workload>.txt), or None."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = dict(kv.split("=", 1) for kv in line.split() if "=" in kv)
+        if f.get("kernel") == kernel:
+            return int(f["bytes_per_launch"])
+    return None
+
+
 def geom(p):
     ny = p.ny
     ya, yb = ny // 3, 2 * (ny // 3) + 1
     return dict(nx=p.nx, ny=ny, nz=p.nz, Kx=p.nx // 3, Ky=ya + (ny - yb), drag=p.C_d != 0.0)
 
 
-def run_gpu(wl, precision, q_spun, t0, dt0, steps, warmup, prof_kernel):
-    import torch
-    from qgn_fortran_amd import QGN
+def _model(wl, precision):
+    from qgn_fortran_amd import Batch
     from qgn_fortran_amd.model import vertical_modes
-
     p = wl.params
     p.precision = precision
     H, _L, Modes, EVals, qy = vertical_modes(wl.H, wl.S, wl.uBar, p.Htot)
-    m = QGN(p, H, Modes, EVals, wl.uBar, qy)
+    # a batch of one member: qgn_time_step and the graph path on one context
+    return Batch(p, [dict(H=H, Modes=Modes, EVals=EVals, uBar=wl.uBar, qyBar=qy)])
+
+
+def run_gpu(wl, precision, q_spun, t0, dt0, steps, warmup, prof_kernel):
+    """K accepted steps through qgn_time_step (one read-back per attempt)
+    with the roofline kernel timed by HIP events on the library's stream,
+    then K accepted steps from the same state through the graph-captured
+    attempts of qgn_batch_step (no host round trip inside the timed run)."""
+    import torch
+    m = _model(wl, precision)
     m.set_state(q_spun)
     t, dt = t0, dt0
     for _ in range(warmup):
         t, dt = m.time_step(t, dt)
+    q_w, t_w, dt_w = m.get_state(), t, dt
     st0 = m.stats()
     m.prof_enable(prof_kernel)
     torch.cuda.synchronize()
@@ -67,6 +90,7 @@ def run_gpu(wl, precision, q_spun, t0, dt0, steps, warmup, prof_kernel):
     torch.cuda.synchronize()
     wall = time.perf_counter() - tw0
     nlaunch, kms = m.prof_read()
+    m.prof_enable(-1)
     st1 = m.stats()
     res = dict(
         precision=precision, wall_s=wall, sim_days=(t - ts) / 86400.0,
@@ -76,30 +100,51 @@ def run_gpu(wl, precision, q_spun, t0, dt0, steps, warmup, prof_kernel):
         kernel_launches=nlaunch, kernel_ms=kms / max(nlaunch, 1), t_end=t, dt_end=dt,
     )
     res["ms_per_rhs"] = 1e3 * wall / max(res["rhs_evals"], 1)
+    # graph path from the same post-warm-up state
+    m.set_state(q_w)
+    m.set_time(t_w, dt_w)
+    m.step(warmup)                      # captures the graph; untimed
+    info = m.times()
+    n0, tg0 = int(info["steps"][0]), float(info["t"][0])
+    torch.cuda.synchronize()
+    tw0 = time.perf_counter()
+    done = 0
+    attempts = 0
+    while done < steps:                 # rejected attempts add to the timed work
+        m.step(steps - done)
+        attempts += steps - done
+        info = m.times()
+        done = int(info["steps"][0]) - n0
+    wall_g = time.perf_counter() - tw0
+    tg = float(info["t"][0])
+    res["graph"] = dict(wall_s=wall_g, steps=done, attempts=attempts, sim_days=(tg - tg0) / 86400.0,
+                        days_per_s=(tg - tg0) / 86400.0 / wall_g, ms_per_step=1e3 * wall_g / done,
+                        mean_dt=(tg - tg0) / done)
     q_end = m.get_state()
     m.close()
     return res, q_end
 
 
 def spin_up(wl, steps, precision="mixed"):
-    from qgn_fortran_amd import QGN, initial
-    from qgn_fortran_amd.model import vertical_modes
+    """`steps` accepted mixed-precision steps from the seeded initial state
+    (graph-captured attempts, 500 at a time)."""
+    from qgn_fortran_amd import initial
     p = wl.params
-    p.precision = precision
-    H, _L, Modes, EVals, qy = vertical_modes(wl.H, wl.S, wl.uBar, p.Htot)
-    m = QGN(p, H, Modes, EVals, wl.uBar, qy)
+    m = _model(wl, precision)
     q0 = initial.random_phase_pv(p.nx, p.ny, p.nz, wl.k_peak, wl.rms, wl.seed, p.Lx, p.Ly)
     m.initialize(q0)
-    t, dt = 0.0, wl.dt0
+    m.set_time(0.0, wl.dt0)
     tw = time.perf_counter()
-    for i in range(steps):
-        t, dt = m.time_step(t, dt)
-        if (i + 1) % 500 == 0:
-            print(f"# spin-up {i + 1}/{steps}: t = {t / 86400:.2f} days, dt = {dt:.1f} s, "
-                  f"{time.perf_counter() - tw:.1f} s", file=sys.stderr, flush=True)
+    done = 0
+    while done < steps:
+        m.step(min(500, steps - done))
+        info = m.times()
+        done = int(info["steps"][0])
+        print(f"# spin-up {done}/{steps}: t = {info['t'][0] / 86400:.2f} days, dt = {info['dt'][0]:.1f} s, "
+              f"{time.perf_counter() - tw:.1f} s", file=sys.stderr, flush=True)
     q = m.get_state()
     m.close()
-    return q, t, dt
+    return q, float(info["t"][0]), float(info["dt"][0])
 
 
 def cpu_baseline(wl, q_spun, t0, dt0, budget_s=30.0):
@@ -169,24 +214,24 @@ def main():
     kb = kernel_bytes(a.kernel, g, rs)
     achieved = kb / (mixed["kernel_ms"] * 1e-3) / 1e9
     roof = dict(bound="hbm", achieved=round(achieved, 1), peak=8000.0, unit="GB/s",
-                frac=round(achieved / 8000.0, 4), traffic=None, kernel=a.kernel,
+                frac=round(achieved / 8000.0, 4), traffic=measured_traffic(a.workload, a.kernel), kernel=a.kernel,
                 bytes_per_launch=kb, avg_launch_ms=mixed["kernel_ms"])
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(configs.WORKLOADS[a.workload](), q, t0, dt0, a.cpu_budget)
     out = {
         "metric": "simulated days/wall-s, 1024^2x16-layer ARK43_QGLeith, mixed precision",
-        "value": round(mixed["days_per_s"], 4),
+        "value": round(mixed["graph"]["days_per_s"], 4),
         "unit": "simulated days/s",
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
-        "ms_per_step": round(mixed["ms_per_step"], 4),
+        "ms_per_step": round(mixed["graph"]["ms_per_step"], 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 FFT/Jacobian, f64 state/stages/error",
         "data": "synthetic (seeded random-phase PV, spun up on the GPU)",
         "config": {"workload": a.workload, "nx": wl.params.nx, "ny": wl.params.ny, "nz": wl.params.nz,
                    "scheme": "ARK43_QGLeith", "spinup_steps": a.spinup, "parallelism": "single GPU"},
         "mixed": mixed, "fp64": fp64,
-        "mixed_vs_fp64": round(mixed["days_per_s"] / fp64["days_per_s"], 3),
+        "mixed_vs_fp64": round(mixed["graph"]["days_per_s"] / fp64["graph"]["days_per_s"], 3),
         "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
