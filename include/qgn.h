@@ -200,6 +200,44 @@ int qgn_dft_free(qgn_dft* h);
 int qgn_vertical_modes(int nz, double Htot, const double* H_in, const double* S, const double* uBar,
                        double* H_out, double* L, double* Modes, double* EVals, double* qyBar);
 
+/* ---- Learned subgrid closure: a 4-layer 3x3 CNN ------------------------
+ * No counterpart in the reference (whose hook for a closure is the QG-Leith
+ * term, ARK43_QGLeith.f90:62-79).  Layers cin -> 32 -> 32 -> 32 -> cout with
+ * ReLU between them, periodic padding; BF16 weights and activations with
+ * FP32 accumulation on the matrix cores, FP32 master weights and Adam.
+ * cin in {16, 32}, cout in {8, 16}.  Tensors are (n, H, W, C), channel
+ * fastest, with W % 16 == 0 and n*H*W % 64 == 0.
+ * params: FP32, for each layer W[co][tap][ci] then b[co], tap = 3 (dy+1) +
+ * (dx+1) for the input offset (dy, dx); qgn_cnn_nparams gives the length. */
+typedef struct qgn_cnn qgn_cnn;
+int qgn_cnn_create(qgn_cnn** h, int cin, int cout, const float* params);
+int qgn_cnn_destroy(qgn_cnn* h);
+int qgn_cnn_nparams(qgn_cnn* h, int* n);
+/* host copies of the parameters and of the last gradients (either may be NULL) */
+int qgn_cnn_get_params(qgn_cnn* h, float* params, float* grads);
+int qgn_cnn_set_params(qgn_cnn* h, const float* params);
+/* device pointers: x bf16 bit patterns (n, H, W, cin), y fp32 (n, H, W, cout) */
+int qgn_cnn_forward(qgn_cnn* h, const void* x, int n, int H, int W, float* y);
+/* gradients of the mean-squared error of the last forward's y against the
+ * device array target (n, H, W, cout); *loss receives the error */
+int qgn_cnn_backward(qgn_cnn* h, const float* target, double* loss);
+/* one Adam step (bias-corrected) with the last gradients */
+int qgn_cnn_adam(qgn_cnn* h, float lr, float beta1, float beta2, float eps);
+/* forward + backward + Adam(lr, 0.9, 0.999, 1e-8) */
+int qgn_cnn_train_step(qgn_cnn* h, const void* x, const float* target, int n, int H, int W, float* y, float lr,
+                       double* loss);
+const char* qgn_cnn_last_error(void);
+
+/* Attach the network as a learned subgrid PV forcing (NULL detaches): every
+ * RHS evaluation of the context (qgn_rhs and every stage of every stepper)
+ * adds r2c(F) on the kept modes, where F = out_scale * net(in_scale * (q,
+ * psi)) is evaluated on the physical fields of each member (input channels
+ * q_1..q_nz, psi_1..psi_nz; output one forcing per layer).  The network
+ * needs cin = 2 nz and cout = nz; nx % 16 == 0.  The context keeps the
+ * pointer: destroy the network only after detaching it.  An ensemble built
+ * before this call keeps the attempt it captured. */
+int qgn_set_closure(qgn_ctx* ctx, qgn_cnn* net, const double* in_scale, const double* out_scale);
+
 const char* qgn_last_error(void);
 const char* qgn_version(void);
 

@@ -23,6 +23,9 @@ EXPORTS = [
     "qgn_dft_forward_device", "qgn_dft_backward_device", "qgn_dft_free", "qgn_vertical_modes",
     "qgn_last_error", "qgn_version", "qgn_create_batch", "qgn_batch_set_time", "qgn_batch_step",
     "qgn_batch_get_time", "qgn_ensemble_create", "qgn_ensemble_step", "qgn_ensemble_destroy",
+    "qgn_cnn_create", "qgn_cnn_destroy", "qgn_cnn_nparams", "qgn_cnn_get_params", "qgn_cnn_set_params",
+    "qgn_cnn_forward", "qgn_cnn_backward", "qgn_cnn_adam", "qgn_cnn_train_step", "qgn_cnn_last_error",
+    "qgn_set_closure",
 ]
 
 
@@ -92,6 +95,17 @@ def load():
         "qgn_ensemble_create": (I, [C.POINTER(P), C.POINTER(P), I]),
         "qgn_ensemble_step": (I, [P, I]),
         "qgn_ensemble_destroy": (I, [P]),
+        "qgn_cnn_create": (I, [C.POINTER(P), I, I, C.POINTER(C.c_float)]),
+        "qgn_cnn_destroy": (I, [P]),
+        "qgn_cnn_nparams": (I, [P, C.POINTER(I)]),
+        "qgn_cnn_get_params": (I, [P, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+        "qgn_cnn_set_params": (I, [P, C.POINTER(C.c_float)]),
+        "qgn_cnn_forward": (I, [P, P, I, I, I, P]),
+        "qgn_cnn_backward": (I, [P, P, D]),
+        "qgn_cnn_adam": (I, [P, C.c_float, C.c_float, C.c_float, C.c_float]),
+        "qgn_cnn_train_step": (I, [P, P, P, I, I, I, P, C.c_float, D]),
+        "qgn_cnn_last_error": (C.c_char_p, []),
+        "qgn_set_closure": (I, [P, P, D, D]),
         "qgn_last_error": (C.c_char_p, []),
         "qgn_version": (C.c_char_p, []),
     }
@@ -108,6 +122,22 @@ def check(code):
         msg = _lib.qgn_last_error().decode() if _lib is not None else ""
         raise QgnError(code, msg)
     return code
+
+
+def check_cnn(code):
+    if code != 0:
+        msg = _lib.qgn_cnn_last_error().decode() if _lib is not None else ""
+        raise QgnError(code, msg)
+    return code
+
+
+def fptr(a):
+    """float* view of a C-contiguous float32 numpy array."""
+    if a is None:
+        return None
+    if a.dtype != "float32" or not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("array must be C-contiguous float32")
+    return a.ctypes.data_as(C.POINTER(C.c_float))
 
 
 def dptr(a):

@@ -1,0 +1,88 @@
+"""The learned subgrid closure network over the HIP library (include/qgn.h,
+qgn_cnn_*): a 4-layer 3x3 periodic CNN with BF16 MFMA forward/backward
+kernels and Adam.  Device tensors are torch CUDA tensors (used only as
+device memory): inputs as int16 bit patterns of bfloat16, outputs and
+targets float32, all (n, H, W, C) with the channel fastest."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import check_cnn, fptr
+
+HIDDEN = 32
+
+
+def nparams(cin, cout):
+    dims = [(cin, HIDDEN), (HIDDEN, HIDDEN), (HIDDEN, HIDDEN), (HIDDEN, cout)]
+    return sum(o * 9 * i + o for i, o in dims)
+
+
+def to_bf16_bits(a):
+    """bfloat16 bit patterns (int16) of a float array, round to nearest even."""
+    f = np.ascontiguousarray(a, dtype=np.float32)
+    u = f.view(np.uint32).astype(np.uint64)
+    return ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16).view(np.int16)
+
+
+class CNN:
+    def __init__(self, cin, cout, params):
+        self.lib = _lib.load()
+        self.cin, self.cout = cin, cout
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        if p.size != nparams(cin, cout):
+            raise ValueError(f"expected {nparams(cin, cout)} parameters, got {p.size}")
+        self._h = C.c_void_p()
+        check_cnn(self.lib.qgn_cnn_create(C.byref(self._h), cin, cout, fptr(p)))
+        self.np = p.size
+
+    def params(self):
+        out = np.empty(self.np, dtype=np.float32)
+        check_cnn(self.lib.qgn_cnn_get_params(self._h, fptr(out), None))
+        return out
+
+    def grads(self):
+        out = np.empty(self.np, dtype=np.float32)
+        check_cnn(self.lib.qgn_cnn_get_params(self._h, None, fptr(out)))
+        return out
+
+    def set_params(self, p):
+        check_cnn(self.lib.qgn_cnn_set_params(self._h, fptr(np.ascontiguousarray(p, dtype=np.float32))))
+
+    @staticmethod
+    def _shape(x):
+        n, H, W, _ = x.shape
+        return n, H, W
+
+    def forward(self, x, y):
+        """x: int16 CUDA tensor (n, H, W, cin); y: float32 CUDA tensor (n, H, W, cout)."""
+        n, H, W = self._shape(x)
+        check_cnn(self.lib.qgn_cnn_forward(self._h, C.c_void_p(x.data_ptr()), n, H, W, C.c_void_p(y.data_ptr())))
+
+    def backward(self, target):
+        loss = C.c_double()
+        check_cnn(self.lib.qgn_cnn_backward(self._h, C.c_void_p(target.data_ptr()), C.byref(loss)))
+        return loss.value
+
+    def adam(self, lr, b1=0.9, b2=0.999, eps=1e-8):
+        check_cnn(self.lib.qgn_cnn_adam(self._h, lr, b1, b2, eps))
+
+    def train_step(self, x, target, y, lr):
+        n, H, W = self._shape(x)
+        loss = C.c_double()
+        check_cnn(self.lib.qgn_cnn_train_step(self._h, C.c_void_p(x.data_ptr()), C.c_void_p(target.data_ptr()),
+                                              n, H, W, C.c_void_p(y.data_ptr()), lr, C.byref(loss)))
+        return loss.value
+
+    def close(self):
+        if self._h:
+            self.lib.qgn_cnn_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

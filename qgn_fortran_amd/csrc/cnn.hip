@@ -1,0 +1,585 @@
+// A 4-layer 3x3 convolutional network on doubly periodic grids, written for
+// the gfx950 matrix cores: BF16 weights and activations, FP32 accumulation
+// (v_mfma_f32_16x16x16bf16_1k), FP32 master weights trained with Adam.  It is
+// the learned subgrid PV closure of the coarse model.
+//
+// Layers: cin -> 32 -> 32 -> 32 -> cout, ReLU after the first three; the
+// padding is periodic (every tap wraps around the grid).  Tensors are
+// (n, H, W, C) with the channel fastest ("NHWC").
+//
+// Parameter vector (FP32, host and device): for each layer l, W_l[co][tap][ci]
+// then b_l[co], tap = 3 (dy + 1) + (dx + 1) for the input offset (dy, dx).
+//
+// Convolution as an implicit GEMM: per wave, D (16 output channels x 16
+// pixels) += A (16 output channels x 16 input channels of one tap) *
+// B (16 input channels x 16 pixels).  MFMA operand layouts on gfx950:
+// A[i][k] in lane l at i = l % 16, k = 4 (l / 16) + r; B[k][j] at j = l % 16,
+// k = 4 (l / 16) + r; D[i][j] at i = 4 (l / 16) + r, j = l % 16 (r = 0..3).
+// With channels fastest, each lane's A and B operands are 8 contiguous bytes,
+// and each lane's 4 results are 4 consecutive output channels of one pixel.
+// The backward pass for the data is the same convolution with the weights
+// transposed (ci <-> co) and the taps mirrored.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "cnn.hpp"
+#include "qgn.h"
+
+struct qgn_cnn;
+
+namespace qcnn {
+
+typedef short s4v __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef unsigned short u16;
+
+constexpr int kLayers = 4;
+constexpr int kHidden = 32;
+
+__host__ __device__ constexpr int pad16(int c) { return (c + 15) / 16 * 16; }
+
+// round to nearest even; NaN stays NaN
+__device__ __forceinline__ u16 f2bf(float x) {
+  unsigned u = __float_as_uint(x);
+  if ((u & 0x7F800000u) == 0x7F800000u) return (u16)((u >> 16) | ((u & 0xFFFFu) ? 0x40u : 0u));
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (u16)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(u16 h) { return __uint_as_float(((unsigned)h) << 16); }
+
+// MODE 0: bf16 out = relu(acc + bias)          (hidden layers, forward)
+// MODE 1: fp32 out = acc + bias                (last layer, forward)
+// MODE 2: bf16 out = acc * (mask > 0)          (backward data; mask = layer input)
+// in: (n, H, W, CI) bf16; wpk: (COP, 9, CI) bf16 with rows >= CO zero;
+// out: (n, H, W, CO).  Block: 4 waves, 16 consecutive pixels of one row each;
+// needs W % 16 == 0 and n H W % 64 == 0.
+template <int CI, int CO, int MODE>
+__global__ void __launch_bounds__(256) k_conv3x3(const u16* __restrict__ in, const u16* __restrict__ wpk,
+                                                 const float* __restrict__ bias, const u16* __restrict__ mask,
+                                                 void* __restrict__ out, int H, int W) {
+  constexpr int COP = pad16(CO);
+  constexpr int NW = COP * 9 * CI;
+  __shared__ __align__(16) u16 ws[NW];
+  for (int e = threadIdx.x * 8; e < NW; e += 256 * 8)
+    *reinterpret_cast<uint4*>(&ws[e]) = *reinterpret_cast<const uint4*>(&wpk[e]);
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int j = lane & 15, g = lane >> 4;
+  const long p0 = (long)blockIdx.x * 64 + w * 16;
+  const long row = p0 / W;                  // n H + y
+  const int x0 = (int)(p0 - row * W);
+  const int y = (int)(row % H);
+  const long img = (row - y) * W;           // first pixel of image n
+  f4v acc[COP / 16];
+#pragma unroll
+  for (int mt = 0; mt < COP / 16; ++mt) acc[mt] = f4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int tap = 0; tap < 9; ++tap) {
+    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+    int yy = y + dy;
+    yy += yy < 0 ? H : 0;
+    yy -= yy >= H ? H : 0;
+    int xx = x0 + j + dx;
+    xx += xx < 0 ? W : 0;
+    xx -= xx >= W ? W : 0;
+    const u16* src = in + (img + (long)yy * W + xx) * CI + 4 * g;
+#pragma unroll
+    for (int kc = 0; kc < CI / 16; ++kc) {
+      const s4v b = *reinterpret_cast<const s4v*>(src + kc * 16);
+#pragma unroll
+      for (int mt = 0; mt < COP / 16; ++mt) {
+        const s4v a = *reinterpret_cast<const s4v*>(&ws[((mt * 16 + j) * 9 + tap) * CI + kc * 16 + 4 * g]);
+        acc[mt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[mt], 0, 0, 0);
+      }
+    }
+  }
+  const long p = p0 + j;
+#pragma unroll
+  for (int mt = 0; mt < COP / 16; ++mt) {
+    const int co = mt * 16 + 4 * g;
+    if (co >= CO) continue;                 // CO is a multiple of 4
+    if constexpr (MODE == 1) {
+      float4 v;
+      v.x = acc[mt][0] + bias[co];
+      v.y = acc[mt][1] + bias[co + 1];
+      v.z = acc[mt][2] + bias[co + 2];
+      v.w = acc[mt][3] + bias[co + 3];
+      *reinterpret_cast<float4*>(reinterpret_cast<float*>(out) + p * CO + co) = v;
+    } else {
+      float v[4];
+      if constexpr (MODE == 0) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) v[r] = fmaxf(acc[mt][r] + bias[co + r], 0.f);
+      } else {
+        const ushort4 m = *reinterpret_cast<const ushort4*>(mask + p * CO + co);
+        v[0] = bf2f(m.x) > 0.f ? acc[mt][0] : 0.f;
+        v[1] = bf2f(m.y) > 0.f ? acc[mt][1] : 0.f;
+        v[2] = bf2f(m.z) > 0.f ? acc[mt][2] : 0.f;
+        v[3] = bf2f(m.w) > 0.f ? acc[mt][3] : 0.f;
+      }
+      ushort4 o;
+      o.x = f2bf(v[0]);
+      o.y = f2bf(v[1]);
+      o.z = f2bf(v[2]);
+      o.w = f2bf(v[3]);
+      *reinterpret_cast<ushort4*>(reinterpret_cast<u16*>(out) + p * CO + co) = o;
+    }
+  }
+}
+
+// Weight and bias gradients of one layer:
+//   dW[co][tap][ci] += sum_p dY[p][co] X[p + off(tap)][ci],  db[co] += sum_p dY[p][co]
+// X: (n, H, W, CI) bf16 (layer input), dY: (n, H, W, COP) bf16 (gradient of
+// the layer's pre-activation, channels >= CO zero).  Block: `rows` image
+// rows of image blockIdx.y; per row the three input rows and the gradient
+// row are staged in LDS, and each wave accumulates its share of the
+// 9 (COP/16) (CI/16) 16x16 tiles over 16-pixel chunks.
+template <int CI, int CO>
+__global__ void __launch_bounds__(256) k_conv_wgrad(const u16* __restrict__ x, const u16* __restrict__ dy,
+                                                    float* __restrict__ dw, float* __restrict__ db, int H, int W,
+                                                    int rows) {
+  constexpr int COP = pad16(CO);
+  constexpr int NMT = COP / 16, NNT = CI / 16, NT = 9 * NMT * NNT, TPW = (NT + 3) / 4;
+  extern __shared__ __align__(16) u16 lds[];
+  u16* xs = lds;                       // [3][W][CI]
+  u16* ds = lds + 3 * W * CI;          // [W][COP]
+  __shared__ float bsum[COP];
+  const int tid = threadIdx.x;
+  if (tid < COP) bsum[tid] = 0.f;
+  const int lane = tid & 63, w = tid >> 6;
+  const int j = lane & 15, g = lane >> 4;
+  const int n = blockIdx.y;
+  const int y0 = blockIdx.x * rows;
+  const int y1 = y0 + rows < H ? y0 + rows : H;
+  f4v acc[TPW];
+#pragma unroll
+  for (int t = 0; t < TPW; ++t) acc[t] = f4v{0.f, 0.f, 0.f, 0.f};
+  float bacc = 0.f;   // channel tid % COP (256 is a multiple of COP)
+  for (int y = y0; y < y1; ++y) {
+    __syncthreads();
+    for (int e = tid * 8; e < 3 * W * CI; e += 256 * 8) {
+      const int r = e / (W * CI), rem = e - r * (W * CI);
+      int yy = y + r - 1;
+      yy += yy < 0 ? H : 0;
+      yy -= yy >= H ? H : 0;
+      *reinterpret_cast<uint4*>(&xs[e]) =
+          *reinterpret_cast<const uint4*>(&x[((long)n * H + yy) * W * CI + rem]);
+    }
+    for (int e = tid * 8; e < W * COP; e += 256 * 8)
+      *reinterpret_cast<uint4*>(&ds[e]) = *reinterpret_cast<const uint4*>(&dy[((long)n * H + y) * W * COP + e]);
+    __syncthreads();
+    for (int e = tid; e < W * COP; e += 256) bacc += bf2f(ds[e]);
+    for (int ks = 0; ks < W / 16; ++ks) {
+#pragma unroll
+      for (int tt = 0; tt < TPW; ++tt) {
+        const int t = w + 4 * tt;
+        if (t >= NT) break;
+        const int tap = t / (NMT * NNT);
+        const int rem = t - tap * (NMT * NNT);
+        const int mt = rem / NNT, nt = rem - mt * NNT;
+        const int ddy = tap / 3 - 1, ddx = tap % 3 - 1;
+        s4v a, b;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int px = ks * 16 + 4 * g + r;
+          a[r] = (short)ds[px * COP + mt * 16 + j];                 // A[co][pixel]
+          int xx = px + ddx;
+          xx += xx < 0 ? W : 0;
+          xx -= xx >= W ? W : 0;
+          b[r] = (short)xs[((ddy + 1) * W + xx) * CI + nt * 16 + j];  // B[pixel][ci]
+        }
+        acc[tt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[tt], 0, 0, 0);
+      }
+    }
+  }
+#pragma unroll
+  for (int tt = 0; tt < TPW; ++tt) {
+    const int t = w + 4 * tt;
+    if (t >= NT) break;
+    const int tap = t / (NMT * NNT);
+    const int rem = t - tap * (NMT * NNT);
+    const int mt = rem / NNT, nt = rem - mt * NNT;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int co = mt * 16 + 4 * g + r;
+      if (co < CO) atomicAdd(&dw[(co * 9 + tap) * CI + nt * 16 + j], acc[tt][r]);
+    }
+  }
+  atomicAdd(&bsum[tid % COP], bacc);
+  __syncthreads();
+  if (tid < CO) atomicAdd(&db[tid], bsum[tid]);
+}
+
+// Mean-squared error of pred against target over n_pix * CO values; writes
+// dL/dpred as bf16 with COP channels (the padding channels zero) and adds
+// the sum of squares to *sq.
+__global__ void k_mse(const float* __restrict__ pred, const float* __restrict__ tgt, long npix, int CO, int COP,
+                      float scale, u16* __restrict__ dpred, double* __restrict__ sq) {
+  __shared__ double red[4];
+  double s = 0.0;
+  const long n = npix * COP;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    const long p = e / COP;
+    const int c = (int)(e - p * COP);
+    float d = 0.f;
+    if (c < CO) {
+      d = pred[p * CO + c] - tgt[p * CO + c];
+      s += (double)d * d;
+    }
+    dpred[e] = f2bf(scale * d);
+  }
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(sq, red[0] + red[1] + red[2] + red[3]);
+}
+
+// Adam on the FP32 parameter vector; c1 = 1/(1 - b1^t), c2 = 1/(1 - b2^t).
+__global__ void k_adam(float* __restrict__ w, const float* __restrict__ grad, float* __restrict__ m,
+                       float* __restrict__ v, int n, float lr, float b1, float b2, float eps, float c1, float c2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = grad[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  w[i] -= lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+}
+
+// bf16 weights of one layer from the FP32 W[co][tap][ci]: forward layout
+// wf[COP][9][CI] and backward-data layout wb[CI][9][COP] with mirrored taps.
+__global__ void k_pack_w(const float* __restrict__ W, int CI, int CO, int COP, u16* __restrict__ wf,
+                         u16* __restrict__ wb) {
+  const int n = COP * 9 * CI;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const int co = e / (9 * CI);
+  const int rem = e - co * 9 * CI;
+  const int tap = rem / CI, ci = rem - tap * CI;
+  const float v = co < CO ? W[(co * 9 + tap) * CI + ci] : 0.f;
+  const u16 h = f2bf(v);
+  wf[e] = h;
+  wb[(ci * 9 + (8 - tap)) * COP + co] = h;
+}
+
+struct Layer {
+  int ci, co, cop;
+  long woff, boff;   // offsets of W and b in the parameter vector
+};
+
+int allow_lds(const void* f, size_t bytes) {
+  if (bytes <= 65536) return 0;
+  return hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) == hipSuccess ? 0 : -1;
+}
+
+}  // namespace qcnn
+
+using namespace qcnn;
+
+struct qgn_cnn {
+  int cin = 16, cout = 8;
+  Layer L[kLayers];
+  int np = 0;
+  float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;   // FP32 parameters, gradients, Adam moments
+  u16 *wf[kLayers] = {}, *wb[kLayers] = {};
+  int t = 0;                   // Adam step
+  // activations of the last forward: act[0] = input (not owned), act[1..3] hidden
+  const u16* act[kLayers] = {};
+  u16* hid[3] = {};
+  u16 *dpred = nullptr, *dbuf[2] = {};
+  float* pred = nullptr;       // last forward output (not owned)
+  long cap = 0;                // pixels the buffers hold
+  int n = 0, H = 0, W = 0;
+  double* sq = nullptr;
+  double* h_sq = nullptr;
+  hipStream_t stream = nullptr;
+};
+
+namespace {
+
+thread_local std::string g_cnn_err;
+int cfail(int code, const std::string& m) {
+  g_cnn_err = m;
+  return code;
+}
+#define CCHK(x)                                                                  \
+  do {                                                                           \
+    hipError_t e_ = (x);                                                         \
+    if (e_ != hipSuccess) return cfail(QGN_EHIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+#define CRET(x)          \
+  do {                   \
+    int r_ = (x);        \
+    if (r_) return r_;   \
+  } while (0)
+
+int repack(qgn_cnn* h, hipStream_t s) {
+  for (int l = 0; l < kLayers; ++l) {
+    const Layer& L = h->L[l];
+    const int n = L.cop * 9 * L.ci;
+    hipLaunchKernelGGL(k_pack_w, dim3((n + 255) / 256), dim3(256), 0, s, h->w + L.woff, L.ci, L.co, L.cop, h->wf[l],
+                       h->wb[l]);
+  }
+  CCHK(hipGetLastError());
+  return 0;
+}
+
+int ensure(qgn_cnn* h, long npix) {
+  if (npix <= h->cap) return 0;
+  for (u16* p : h->hid)
+    if (p) (void)hipFree(p);
+  for (u16* p : h->dbuf)
+    if (p) (void)hipFree(p);
+  if (h->dpred) (void)hipFree(h->dpred);
+  for (int i = 0; i < 3; ++i) CCHK(hipMalloc((void**)&h->hid[i], npix * kHidden * sizeof(u16)));
+  for (int i = 0; i < 2; ++i) CCHK(hipMalloc((void**)&h->dbuf[i], npix * kHidden * sizeof(u16)));
+  CCHK(hipMalloc((void**)&h->dpred, npix * h->L[kLayers - 1].cop * sizeof(u16)));
+  h->cap = npix;
+  return 0;
+}
+
+template <int CI, int CO, int MODE>
+int conv(const u16* in, const u16* w, const float* b, const u16* mask, void* out, int n, int H, int W, hipStream_t s) {
+  const long npix = (long)n * H * W;
+  hipLaunchKernelGGL((k_conv3x3<CI, CO, MODE>), dim3((unsigned)(npix / 64)), dim3(256), 0, s, in, w, b, mask, out, H,
+                     W);
+  CCHK(hipGetLastError());
+  return 0;
+}
+
+template <int CI, int CO>
+int wgrad(const u16* x, const u16* dy, float* dw, float* db, int n, int H, int W, hipStream_t s) {
+  constexpr int COP = pad16(CO);
+  const size_t lds = (size_t)(3 * W * CI + W * COP) * sizeof(u16);
+  auto kern = k_conv_wgrad<CI, CO>;
+  if (allow_lds((const void*)kern, lds)) return cfail(QGN_EINVAL, "wgrad: grid row too wide for LDS");
+  const int rows = 16;
+  hipLaunchKernelGGL(kern, dim3((H + rows - 1) / rows, n), dim3(256), lds, s, x, dy, dw, db, H, W, rows);
+  CCHK(hipGetLastError());
+  return 0;
+}
+
+int check_shape(const qgn_cnn* h, int n, int H, int W) {
+  if (n < 1 || H < 3 || W < 16 || W % 16 || ((long)n * H * W) % 64)
+    return cfail(QGN_EINVAL, "cnn: needs W % 16 == 0, n*H*W % 64 == 0, H >= 3");
+  if (3L * W * 32 * 2 + (long)W * 32 * 2 > 160 * 1024) return cfail(QGN_EINVAL, "cnn: W too large");
+  return 0;
+}
+
+// layer l forward: input act[l] -> hid[l] or y
+int layer_fwd(qgn_cnn* h, int l, const u16* in, void* out, int n, int H, int W, hipStream_t s) {
+  const Layer& L = h->L[l];
+  const float* b = h->w + L.boff;
+  if (l < kLayers - 1) {
+    if (L.ci == 16) return conv<16, 32, 0>(in, h->wf[l], b, nullptr, out, n, H, W, s);
+    return conv<32, 32, 0>(in, h->wf[l], b, nullptr, out, n, H, W, s);
+  }
+  if (L.co == 8) return conv<32, 8, 1>(in, h->wf[l], b, nullptr, out, n, H, W, s);
+  return conv<32, 16, 1>(in, h->wf[l], b, nullptr, out, n, H, W, s);
+}
+
+}  // namespace
+
+namespace qcnn {
+
+int forward(qgn_cnn* h, const u16* x, int n, int H, int W, float* y, hipStream_t s) {
+  CRET(check_shape(h, n, H, W));
+  CRET(ensure(h, (long)n * H * W));
+  h->act[0] = x;
+  for (int l = 0; l < kLayers; ++l) {
+    void* out = l < kLayers - 1 ? (void*)h->hid[l] : (void*)y;
+    CRET(layer_fwd(h, l, h->act[l], out, n, H, W, s));
+    if (l < kLayers - 1) h->act[l + 1] = h->hid[l];
+  }
+  h->pred = y;
+  h->n = n;
+  h->H = H;
+  h->W = W;
+  return 0;
+}
+
+int reserve(qgn_cnn* h, long npix) { return ensure(h, npix); }
+
+const char* last_error() { return g_cnn_err.c_str(); }
+
+int channels(const qgn_cnn* h, int* cin, int* cout) {
+  *cin = h->cin;
+  *cout = h->cout;
+  return 0;
+}
+
+}  // namespace qcnn
+
+extern "C" {
+
+const char* qgn_cnn_last_error(void) { return g_cnn_err.c_str(); }
+
+int qgn_cnn_destroy(qgn_cnn* h) {
+  if (!h) return 0;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  void* ptrs[] = {h->w, h->g, h->m, h->v, h->dpred, h->sq, h->hid[0], h->hid[1], h->hid[2], h->dbuf[0], h->dbuf[1]};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  for (int l = 0; l < kLayers; ++l) {
+    if (h->wf[l]) (void)hipFree(h->wf[l]);
+    if (h->wb[l]) (void)hipFree(h->wb[l]);
+  }
+  if (h->h_sq) (void)hipHostFree(h->h_sq);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return 0;
+}
+
+static int cnn_create_impl(qgn_cnn* h, int cin, int cout, const float* params) {
+  if (cin != 16 && cin != 32) return cfail(QGN_EINVAL, "cnn: cin must be 16 or 32");
+  if (cout != 8 && cout != 16) return cfail(QGN_EINVAL, "cnn: cout must be 8 or 16");
+  h->cin = cin;
+  h->cout = cout;
+  const int ci[kLayers] = {cin, kHidden, kHidden, kHidden};
+  const int co[kLayers] = {kHidden, kHidden, kHidden, cout};
+  long off = 0;
+  for (int l = 0; l < kLayers; ++l) {
+    h->L[l] = Layer{ci[l], co[l], pad16(co[l]), off, off + (long)co[l] * 9 * ci[l]};
+    off = h->L[l].boff + co[l];
+  }
+  h->np = (int)off;
+  const size_t pb = (size_t)h->np * sizeof(float);
+  CCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  CCHK(hipMalloc((void**)&h->w, pb));
+  CCHK(hipMalloc((void**)&h->g, pb));
+  CCHK(hipMalloc((void**)&h->m, pb));
+  CCHK(hipMalloc((void**)&h->v, pb));
+  CCHK(hipMemset(h->m, 0, pb));
+  CCHK(hipMemset(h->v, 0, pb));
+  CCHK(hipMemset(h->g, 0, pb));
+  CCHK(hipMemcpy(h->w, params, pb, hipMemcpyHostToDevice));
+  for (int l = 0; l < kLayers; ++l) {
+    const size_t nb = (size_t)h->L[l].cop * 9 * h->L[l].ci * sizeof(u16);
+    CCHK(hipMalloc((void**)&h->wf[l], nb));
+    CCHK(hipMalloc((void**)&h->wb[l], nb));
+    CCHK(hipMemset(h->wb[l], 0, nb));
+  }
+  CCHK(hipMalloc((void**)&h->sq, sizeof(double)));
+  CCHK(hipHostMalloc((void**)&h->h_sq, sizeof(double), hipHostMallocDefault));
+  CRET(repack(h, h->stream));
+  CCHK(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int qgn_cnn_create(qgn_cnn** out, int cin, int cout, const float* params) {
+  if (!out || !params) return cfail(QGN_EINVAL, "null argument");
+  *out = nullptr;
+  qgn_cnn* h = new qgn_cnn();
+  const int rc = cnn_create_impl(h, cin, cout, params);
+  if (rc) {
+    std::string m = g_cnn_err;
+    qgn_cnn_destroy(h);
+    g_cnn_err = m;
+    return rc;
+  }
+  *out = h;
+  return 0;
+}
+
+int qgn_cnn_nparams(qgn_cnn* h, int* n) {
+  if (!h || !n) return cfail(QGN_EINVAL, "null argument");
+  *n = h->np;
+  return 0;
+}
+
+int qgn_cnn_get_params(qgn_cnn* h, float* params, float* grads) {
+  if (!h) return cfail(QGN_EINVAL, "null argument");
+  CCHK(hipStreamSynchronize(h->stream));
+  if (params) CCHK(hipMemcpy(params, h->w, (size_t)h->np * sizeof(float), hipMemcpyDeviceToHost));
+  if (grads) CCHK(hipMemcpy(grads, h->g, (size_t)h->np * sizeof(float), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int qgn_cnn_set_params(qgn_cnn* h, const float* params) {
+  if (!h || !params) return cfail(QGN_EINVAL, "null argument");
+  CCHK(hipStreamSynchronize(h->stream));
+  CCHK(hipMemcpy(h->w, params, (size_t)h->np * sizeof(float), hipMemcpyHostToDevice));
+  CRET(repack(h, h->stream));
+  CCHK(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int qgn_cnn_forward(qgn_cnn* h, const void* x, int n, int H, int W, float* y) {
+  if (!h || !x || !y) return cfail(QGN_EINVAL, "null argument");
+  CRET(qcnn::forward(h, reinterpret_cast<const u16*>(x), n, H, W, y, h->stream));
+  CCHK(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+// Backward of the last forward against `target`: gradients (overwritten)
+// and the mean-squared error.
+static int backward_impl(qgn_cnn* h, const float* target, double* loss) {
+  if (!h->pred) return cfail(QGN_EINVAL, "cnn: backward needs a forward first");
+  hipStream_t s = h->stream;
+  const int n = h->n, H = h->H, W = h->W;
+  const long npix = (long)n * H * W;
+  const Layer& L4 = h->L[kLayers - 1];
+  CCHK(hipMemsetAsync(h->g, 0, (size_t)h->np * sizeof(float), s));
+  CCHK(hipMemsetAsync(h->sq, 0, sizeof(double), s));
+  const float scale = (float)(2.0 / ((double)npix * L4.co));
+  hipLaunchKernelGGL(k_mse, dim3(1024), dim3(256), 0, s, h->pred, target, npix, L4.co, L4.cop, scale, h->dpred,
+                     h->sq);
+  CCHK(hipGetLastError());
+  // layer 4 (no activation): dY = dpred
+  float* g = h->g;
+  if (L4.co == 8) CRET((wgrad<32, 8>(h->act[3], h->dpred, g + L4.woff, g + L4.boff, n, H, W, s)));
+  else CRET((wgrad<32, 16>(h->act[3], h->dpred, g + L4.woff, g + L4.boff, n, H, W, s)));
+  const u16* dy = h->dpred;
+  int nb = 0;
+  for (int l = kLayers - 1; l >= 1; --l) {
+    // dX of layer l masked by the ReLU of layer l-1: the pre-activation gradient of layer l-1
+    u16* dx = h->dbuf[nb];
+    nb ^= 1;
+    if (l == kLayers - 1) {
+      if (L4.cop == 16) CRET((conv<16, 32, 2>(dy, h->wb[l], nullptr, h->act[l], dx, n, H, W, s)));
+    } else {
+      CRET((conv<32, 32, 2>(dy, h->wb[l], nullptr, h->act[l], dx, n, H, W, s)));
+    }
+    const Layer& Lp = h->L[l - 1];
+    if (Lp.ci == 16) CRET((wgrad<16, 32>(h->act[l - 1], dx, g + Lp.woff, g + Lp.boff, n, H, W, s)));
+    else CRET((wgrad<32, 32>(h->act[l - 1], dx, g + Lp.woff, g + Lp.boff, n, H, W, s)));
+    dy = dx;
+  }
+  CCHK(hipMemcpyAsync(h->h_sq, h->sq, sizeof(double), hipMemcpyDeviceToHost, s));
+  CCHK(hipStreamSynchronize(s));
+  if (loss) *loss = *h->h_sq / ((double)npix * L4.co);
+  return 0;
+}
+
+int qgn_cnn_backward(qgn_cnn* h, const float* target, double* loss) {
+  if (!h || !target) return cfail(QGN_EINVAL, "null argument");
+  return backward_impl(h, target, loss);
+}
+
+int qgn_cnn_adam(qgn_cnn* h, float lr, float b1, float b2, float eps) {
+  if (!h) return cfail(QGN_EINVAL, "null argument");
+  h->t += 1;
+  const float c1 = (float)(1.0 / (1.0 - std::pow((double)b1, h->t)));
+  const float c2 = (float)(1.0 / (1.0 - std::pow((double)b2, h->t)));
+  hipLaunchKernelGGL(k_adam, dim3((h->np + 255) / 256), dim3(256), 0, h->stream, h->w, h->g, h->m, h->v, h->np, lr,
+                     b1, b2, eps, c1, c2);
+  CCHK(hipGetLastError());
+  CRET(repack(h, h->stream));
+  CCHK(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int qgn_cnn_train_step(qgn_cnn* h, const void* x, const float* target, int n, int H, int W, float* y, float lr,
+                       double* loss) {
+  if (!h || !x || !target || !y) return cfail(QGN_EINVAL, "null argument");
+  CRET(qcnn::forward(h, reinterpret_cast<const u16*>(x), n, H, W, y, h->stream));
+  CRET(backward_impl(h, target, loss));
+  return qgn_cnn_adam(h, lr, 0.9f, 0.999f, 1e-8f);
+}
+
+}  // extern "C"

@@ -11,6 +11,7 @@
 #include <utility>
 #include <vector>
 
+#include "cnn.hpp"
 #include "kernels_ct.hpp"
 #include "qgn.h"
 
@@ -276,6 +277,13 @@ struct qgn_ctx {
   double* mp[5] = {};
   double* spec = nullptr;  // spectra accumulator, 2 nz nbins
   long spec_n = 0;
+  // learned closure: network, scales and work arrays
+  qgn_cnn* net = nullptr;
+  double *cl_in = nullptr, *cl_out = nullptr;   // 2 nzm and nzm scales
+  double2 *cl_F = nullptr, *cl_psi64 = nullptr;
+  double *cl_pq = nullptr, *cl_pp = nullptr;
+  short* cl_x = nullptr;
+  float* cl_y = nullptr;
   // device-controlled ARK43 attempts of a batch: per-member controller state
   double *bt = nullptr, *bdt = nullptr, *berr0 = nullptr, *blast = nullptr;
   long long* bcount = nullptr;  // steps, attempts, rejects per member (3 nm)
@@ -556,9 +564,55 @@ int rhs_impl(qgn_ctx* c, const double2* qin, void* out, bool psi_ready) {
 }
 
 // psi_ready: psi of qin was already written by k_stage_proj
+int lincomb(qgn_ctx* c, const void* X0, const void* X1, const void* X2, const void* X3, double c0, double c1,
+            double c2, double c3, int nxs, void* out);
+template <typename R>
+int dft_fwd(hipStream_t s, const Geom& g, const FftPlan& px, const FftPlan& py, const Tw<R>& tw, int howmany,
+            const R* grid, cplx<R>* spec, cplx<R>* tmp);
+template <typename R>
+int dft_bwd(hipStream_t s, const Geom& g, const FftPlan& px, const FftPlan& py, const Tw<R>& tw, int howmany,
+            const cplx<R>* spec, R* grid, cplx<R>* tmp, R scale);
+
+// Learned closure: out += r2c(net(q, psi) * scale) on the kept modes, with
+// q, psi the physical fields of qin and of the psi the RHS pass just formed.
+template <typename S>
+int closure_add(qgn_ctx* c, const double2* qin, void* out) {
+  const Geom& g = c->g;
+  hipStream_t s = c->stream;
+  const long nf = (long)g.nz * g.ny * g.nxh;
+  const long np = (long)g.nz * g.ny * g.nx;
+  const double nn = 1.0 / ((double)g.nx * (double)g.ny);
+  hipLaunchKernelGGL(k_unpack, dim3(grid1d(nf)), dim3(256), 0, s, qin, c->cl_F, g);
+  HIPCHK(hipGetLastError());
+  CHK(dft_bwd<double>(s, g, c->px, c->py, c->twd, g.nz, c->cl_F, c->cl_pq, c->fb, nn));
+  const double2* psi64 = c->psi64;
+  if (std::is_same<S, float>::value) {
+    CHK(lincomb(c, c->psi, nullptr, nullptr, nullptr, 1.0, 0, 0, 0, 1, c->cl_psi64));
+    psi64 = c->cl_psi64;
+  }
+  hipLaunchKernelGGL(k_unpack, dim3(grid1d(nf)), dim3(256), 0, s, psi64, c->cl_F, g);
+  HIPCHK(hipGetLastError());
+  CHK(dft_bwd<double>(s, g, c->px, c->py, c->twd, g.nz, c->cl_F, c->cl_pp, c->fb, nn));
+  hipLaunchKernelGGL(k_closure_in, dim3(grid1d(2 * np)), dim3(256), 0, s, c->cl_pq, c->cl_pp, c->cl_in, c->cl_x, g);
+  HIPCHK(hipGetLastError());
+  if (qcnn::forward(c->net, reinterpret_cast<const unsigned short*>(c->cl_x), c->nm, g.ny, g.nx, c->cl_y, s))
+    return fail(QGN_EINVAL, std::string("closure network: ") + qcnn::last_error());
+  hipLaunchKernelGGL(k_closure_out, dim3(grid1d(np)), dim3(256), 0, s, c->cl_y, c->cl_out, c->cl_pq, g);
+  HIPCHK(hipGetLastError());
+  CHK(dft_fwd<double>(s, g, c->px, c->py, c->twd, g.nz, c->cl_pq, c->cl_F, c->fb));
+  hipLaunchKernelGGL((k_closure_add<S>), dim3(grid1d(c->ncmp)), dim3(256), 0, s, c->cl_F,
+                     reinterpret_cast<cplx<S>*>(out), g);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
 int rhs(qgn_ctx* c, const double2* qin, void* out, bool psi_ready = false) {
-  if (c->p.precision == QGN_MIXED) return rhs_impl<float>(c, qin, out, psi_ready);
-  return rhs_impl<double>(c, qin, out, psi_ready);
+  if (c->p.precision == QGN_MIXED) {
+    CHK(rhs_impl<float>(c, qin, out, psi_ready));
+    return c->net ? closure_add<float>(c, qin, out) : 0;
+  }
+  CHK(rhs_impl<double>(c, qin, out, psi_ready));
+  return c->net ? closure_add<double>(c, qin, out) : 0;
 }
 
 // max over the physical grid of |c2r(E)| (unnormalised), FP64, per member
@@ -1031,6 +1085,9 @@ void destroy_ctx(qgn_ctx* c) {
   for (int i = 0; i < 5; ++i)
     if (c->mp[i]) (void)hipFree(c->mp[i]);
   if (c->spec) (void)hipFree(c->spec);
+  void* cl[] = {c->cl_in, c->cl_out, c->cl_F, c->cl_psi64, c->cl_pq, c->cl_pp, c->cl_x, c->cl_y};
+  for (void* x : cl)
+    if (x) (void)hipFree(x);
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   if (c->graph) (void)hipGraphDestroy(c->graph);
   void* bp[] = {c->ekm, c->cdm, c->bt, c->bdt, c->berr0, c->blast, c->bcount, c->bint, c->brep};
@@ -1354,6 +1411,46 @@ int qgn_modal_fields(qgn_ctx* c, double* q_mode, double* psi_mode, double* j_mod
   if (!c) return fail(QGN_EINVAL, "null argument");
   if (c->nm > 1) return fail(QGN_EINVAL, "qgn_modal_fields takes a single-member context");
   return modal_fields(c, q_mode, psi_mode, j_mode);
+}
+
+int qgn_set_closure(qgn_ctx* c, qgn_cnn* net, const double* in_scale, const double* out_scale) {
+  if (!c) return fail(QGN_EINVAL, "null argument");
+  if (c->gexec) {   // the captured attempt changes
+    (void)hipGraphExecDestroy(c->gexec);
+    (void)hipGraphDestroy(c->graph);
+    c->gexec = nullptr;
+    c->graph = nullptr;
+  }
+  if (!net) {
+    c->net = nullptr;
+    return 0;
+  }
+  if (!in_scale || !out_scale) return fail(QGN_EINVAL, "null scale");
+  const Geom& g = c->g;
+  int cin = 0, cout = 0;
+  qcnn::channels(net, &cin, &cout);
+  if (cin != 2 * g.nzm || cout != g.nzm)
+    return fail(QGN_EINVAL, "closure network needs 2*nz input and nz output channels");
+  const long pix = (long)c->nm * g.ny * g.nx;
+  if (g.nx % 16 || pix % 64) return fail(QGN_EINVAL, "closure network needs nx % 16 == 0");
+  if (!c->cl_F) {
+    const long nf = (long)g.nz * g.ny * g.nxh;
+    const long np = (long)g.nz * g.ny * g.nx;
+    CHK(alloc((void**)&c->cl_in, 2 * (size_t)g.nzm * sizeof(double)));
+    CHK(alloc((void**)&c->cl_out, (size_t)g.nzm * sizeof(double)));
+    CHK(alloc((void**)&c->cl_F, nf * sizeof(double2)));
+    CHK(alloc((void**)&c->cl_psi64, (size_t)c->ncmp * sizeof(double2)));
+    CHK(alloc((void**)&c->cl_pq, np * sizeof(double)));
+    CHK(alloc((void**)&c->cl_pp, np * sizeof(double)));
+    CHK(alloc((void**)&c->cl_x, pix * 2 * g.nzm * sizeof(short)));
+    CHK(alloc((void**)&c->cl_y, pix * g.nzm * sizeof(float)));
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(c->cl_in, in_scale, 2 * (size_t)g.nzm * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->cl_out, out_scale, (size_t)g.nzm * sizeof(double), hipMemcpyHostToDevice));
+  if (qcnn::reserve(net, pix)) return fail(QGN_ENOMEM, std::string("closure network: ") + qcnn::last_error());
+  c->net = net;
+  return 0;
 }
 
 int qgn_batch_set_time(qgn_ctx* c, const double* t, const double* dt) {

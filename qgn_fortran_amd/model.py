@@ -182,6 +182,18 @@ class QGN:
         check(self.lib.qgn_spectra(self._ctx, int(nbins), dptr(ke), dptr(ens)))
         return ke, ens
 
+    def set_closure(self, net, in_scale=None, out_scale=None):
+        """Attach a closure network (qgn_fortran_amd.cnn.CNN) as subgrid PV
+        forcing of every RHS evaluation; None detaches it."""
+        if net is None:
+            check(self.lib.qgn_set_closure(self._ctx, None, None, None))
+            self._net = None
+            return
+        ins = np.ascontiguousarray(in_scale, dtype=np.float64)
+        outs = np.ascontiguousarray(out_scale, dtype=np.float64)
+        check(self.lib.qgn_set_closure(self._ctx, net._h, dptr(ins), dptr(outs)))
+        self._net = net      # keep the network alive while attached
+
     def leith(self):
         """A8L(nz) of the current state (ARK43_QGLeith.f90:62-73)."""
         out = np.empty(self.nzt)

@@ -1,0 +1,136 @@
+"""The closure network's BF16 MFMA kernels (csrc/cnn.hip) through the C-ABI
+against the oracle (oracle/cnn_oracle.py).
+
+The bf16-emulating oracle rounds at the GPU's storage points, so only the
+FP32-vs-FP64 accumulation order differs: prediction and loss within
+relative L2 1e-3, gradients per layer within 5e-3.  Against the plain FP64
+oracle the bf16 storage itself shows: prediction within 3e-2, gradients
+within 6e-2."""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(a, dtype):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).to(device="cuda", dtype=dtype)
+
+
+def _case(cin, cout, n, H, W, seed):
+    from oracle import cnn_oracle as co
+    rng = np.random.default_rng(seed)
+    p = co.init_params(cin, cout, seed, last_scale=1.0)
+    p[co.nparams(cin, cout) - cout:] = 0.1 * rng.standard_normal(cout)       # last-layer biases
+    x = rng.standard_normal((n, H, W, cin)).astype(np.float32)
+    t = rng.standard_normal((n, H, W, cout)).astype(np.float32)
+    return p, x, t
+
+
+def _blocks(v, cin, cout):
+    from oracle import cnn_oracle as co
+    out, off = [], 0
+    for i, o in co.shapes(cin, cout):
+        out.append(v[off:off + o * 9 * i])
+        out.append(v[off + o * 9 * i:off + o * 9 * i + o])
+        off += o * 9 * i + o
+    return out
+
+
+@pytest.mark.parametrize("cin,cout,n,H,W", [(16, 8, 2, 32, 48), (32, 16, 1, 24, 64), (16, 8, 1, 256, 256)])
+def test_forward_backward_adam_vs_oracle(cin, cout, n, H, W):
+    import torch
+    from oracle import cnn_oracle as co
+    from qgn_fortran_amd.cnn import CNN, to_bf16_bits
+    print("cnn", cin, cout, n, H, W, flush=True)
+    p, x, t = _case(cin, cout, n, H, W, seed=cin + H)
+    net = CNN(cin, cout, p)
+    xd = _dev(to_bf16_bits(x), torch.int16)
+    td = _dev(t, torch.float32)
+    yd = torch.empty((n, H, W, cout), device="cuda", dtype=torch.float32)
+    net.forward(xd, yd)
+    y = yd.cpu().numpy()
+    loss_e, g_e, pred_e = co.loss_and_grads(p, x, t, cin, cout, bf16_points=True)
+    loss_f, g_f, pred_f = co.loss_and_grads(p, x, t, cin, cout, bf16_points=False)
+    e1, e2 = rel_l2(y, pred_e), rel_l2(y, pred_f)
+    print(f"pred rel L2 vs bf16 oracle {e1:.2e}, vs fp64 oracle {e2:.2e}", flush=True)
+    assert e1 < 1e-3 and e2 < 3e-2
+    loss = net.backward(td)
+    assert abs(loss - loss_e) < 1e-3 * loss_e
+    g = net.grads()
+    for k, (a, b, c) in enumerate(zip(_blocks(g, cin, cout), _blocks(g_e, cin, cout), _blocks(g_f, cin, cout))):
+        ea, eb = rel_l2(a, b), rel_l2(a, c)
+        print(f"grad block {k}: rel L2 vs bf16 oracle {ea:.2e}, vs fp64 {eb:.2e}", flush=True)
+        assert ea < 5e-3 and eb < 6e-2, k
+    # one Adam step against the oracle's Adam with the GPU's gradients
+    net.adam(1e-3)
+    ref, _, _ = co.adam(p.astype(np.float64), g.astype(np.float64), np.zeros(p.size), np.zeros(p.size), 1, 1e-3)
+    assert np.max(np.abs(net.params() - ref)) < 1e-6
+    net.close()
+
+
+def test_training_reduces_loss_on_a_teacher():
+    """A student network learns a random teacher network's map (targets from
+    the FP64 oracle): 300 Adam steps on the GPU cut the loss 5-fold."""
+    import torch
+    from oracle import cnn_oracle as co
+    from qgn_fortran_amd.cnn import CNN, to_bf16_bits
+    cin, cout, n, H, W = 16, 8, 4, 32, 32
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal((n, H, W, cin)).astype(np.float32)
+    teacher = co.init_params(cin, cout, 101, last_scale=1.0)
+    t, _ = co.forward(teacher, x, cin, cout)
+    net = CNN(cin, cout, co.init_params(cin, cout, 5, last_scale=0.1))
+    xd = _dev(to_bf16_bits(x), torch.int16)
+    td = _dev(t.astype(np.float32), torch.float32)
+    yd = torch.empty((n, H, W, cout), device="cuda", dtype=torch.float32)
+    losses = [net.train_step(xd, td, yd, 2e-3) for _ in range(300)]
+    print(f"loss {losses[0]:.4e} -> {losses[-1]:.4e}", flush=True)
+    assert losses[-1] < 0.2 * losses[0]
+    net.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_closure_forcing_in_rhs(precision):
+    """qgn_set_closure: GetRHS with the network attached minus GetRHS without
+    it is r2c(out_scale * net(in_scale * (q, psi))) on the kept modes, with q
+    and psi the physical fields of the state (the oracle's WriteQP fields,
+    QGN_Module.f90:353-361) and the network the bf16-emulating oracle."""
+    from conftest import make_case
+    from oracle import cnn_oracle as co
+    from oracle import qgn_oracle as orc
+    from qgn_fortran_amd import QGN
+    from qgn_fortran_amd.cnn import CNN
+    case = make_case(64, 64, 8, k_peak=6, precision=precision)
+    om = orc.Model(case["po"], case["H"], case["Modes"], case["EVals"], case["uBar"], case["qyBar"])
+    qh = om.initialize(case["q0"])
+    qp, pp = om.get_psi_phys(qh)
+    s_in = np.concatenate([np.full(8, 1.0 / np.std(qp)), np.full(8, 1.0 / np.std(pp))])
+    base = om.get_rhs(qh)
+    s_out = np.full(8, 0.3 * np.sqrt(np.mean(om.c2r(base) ** 2)) / (64 * 64))
+    params = co.init_params(16, 8, seed=3, last_scale=1.0)
+    x = np.concatenate([qp, pp], axis=0).transpose(1, 2, 0)[None] * s_in      # (1, H, W, 16)
+    y, _ = co.forward(params, x.astype(np.float32), 16, 8, bf16_points=True)
+    F = orc.dealias(orc.rfft2(y[0].transpose(2, 0, 1) * s_out[:, None, None]), 64, 64)
+    gm = QGN(case["pp"], case["H"], case["Modes"], case["EVals"], case["uBar"], case["qyBar"])
+    r0 = gm.get_rhs(qh)
+    net = CNN(16, 8, params)
+    gm.set_closure(net, s_in, s_out)
+    r1 = gm.get_rhs(qh)
+    err = rel_l2(r1 - r0, F)
+    print(precision, "closure forcing rel L2", err, "forcing/RHS", np.linalg.norm(F) / np.linalg.norm(base), flush=True)
+    assert err < (2e-3 if precision == "fp64" else 5e-3)
+    # the closure inside the steppers: a step with it differs from a step without
+    gm.set_state(qh)
+    t, dt = gm.time_step(0.0, 600.0)
+    s1 = gm.get_state()
+    gm.set_closure(None)
+    g2 = QGN(case["pp"], case["H"], case["Modes"], case["EVals"], case["uBar"], case["qyBar"])
+    g2.set_state(qh)
+    g2.time_step(0.0, 600.0)
+    assert rel_l2(s1, g2.get_state()) > 1e-9
+    gm.close()
+    g2.close()
+    net.close()
