@@ -187,8 +187,8 @@ def main():
                     help="first run a 1 GiB read + 1 GiB write multiply three times (PMC byte-counter reference)")
     a = ap.parse_args()
     if a.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path (members/slabs would "
-              "shard across GPUs, see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path (ensemble members or kx/y slabs "
+              "would shard across GPUs); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     import torch
     from qgn_fortran_amd import configs
