@@ -194,129 +194,16 @@ __global__ void __launch_bounds__(NTHR) k_row_jac_ct(RowJacArgs a, Geom g, const
 }
 
 // P2, register-fused variant for nx/2 whose plan starts and ends with a
-// radix-8 pass (64 | nx/2): the inverse transforms' first pass reads global
-// memory into registers; their last pass leaves u, v, q_x, q_y in registers
-// at positions t + m N/8, which are the inputs of the forward transform's
-// first pass, so the products and that pass skip the LDS round trip.  One
-// thread per (row, t), handling all four fields; ROWS rows per workgroup.
+// radix-8 pass (64 | nx/2), ROWS rows per workgroup, nx/16 threads per row,
+// one padded LDS sequence per row.  Each inverse transform's first pass reads
+// global memory into registers and its last pass leaves the physical values
+// in registers, at the positions where the forward transform's first pass
+// takes them, so the products skip an LDS round trip.  u and v are
+// transformed first (the bottom layer's |u| u and |u| v go out right away),
+// then q_x and q_y are folded into J = u q_x + v q_y one at a time, so at
+// most two fields and the accumulator are live.
 template <typename R, int N2, int ROWS>
-__global__ void __launch_bounds__(ROWS * (N2 / 8)) k_row_jac_fused(RowJacArgs a, Geom g,
-                                                                  const cplx<R>* __restrict__ twx,
-                                                                  const cplx<R>* __restrict__ twr, R scale) {
-  using Cx = cplx<R>;
-  constexpr CtPlan PL = make_ct_plan(N2);
-  static_assert(N2 % 64 == 0 && PL.rad[0] == 8 && PL.rad[PL.np - 1] == 8, "needs radix-8 end passes");
-  constexpr int NB = N2 / 8;
-  constexpr int NTHR = ROWS * NB;
-  constexpr int PS = row_pad_shift<R>();
-  constexpr int LAST = PL.np - 1;
-  constexpr int OFF_LAST = ct_tw_offset(N2, LAST);
-  using L4 = CtLayout<4 * ROWS, N2, 1, false, PS>;
-  using L1 = CtLayout<ROWS, 4 * N2, 1, false, PS>;
-  extern __shared__ __align__(16) unsigned char lds_raw[];
-  Cx* buf = reinterpret_cast<Cx*>(lds_raw);
-  Cx* ts = buf + row_lds_slots<R>(ROWS * 4 * N2);
-  Cx* tr = ts + ct_tw_size(N2);
-  load_table<R>(ts, twx, ct_tw_size(N2));
-  load_table<R>(tr, twr, N2 + 1);
-  const int r = threadIdx.x / NB;
-  const int t = threadIdx.x - r * NB;
-  const int row0 = blockIdx.x * ROWS;
-  const int z = row0 / g.ny;
-  const int j0 = row0 - z * g.ny;
-  const bool bottom = a.drag && (z % g.nzm == g.nzm - 1);
-  const int Kx = g.Kx;
-  const long Kxp = g.Kxp;
-  const long roff = (long)(row0 + r) * Kxp;
-  const Cx zero = cmk<Cx>((R)0, (R)0);
-  const Cx* srcs[4] = {reinterpret_cast<const Cx*>(a.U), reinterpret_cast<const Cx*>(a.V),
-                       reinterpret_cast<const Cx*>(a.QX), reinterpret_cast<const Cx*>(a.QY)};
-  __syncthreads();
-  // inverse, first pass (P = 1, no twiddles): inputs t + j N2/8 straight from global memory
-#pragma unroll
-  for (int s = 0; s < 4; ++s) {
-    const Cx* p = srcs[s] + roff;
-    Cx v[8];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int k = t + j * NB;
-      const Cx x = k < Kx ? p[k] : zero;
-      const Cx y = (N2 - k) < Kx ? p[N2 - k] : zero;
-      v[j] = c2r_pre_ab<R>(x, y, k, tr, scale);
-    }
-    Dft<Cx, 8>::run(v, +1);
-#pragma unroll
-    for (int m = 0; m < 8; ++m) buf[padded<PS>((4 * r + s) * N2 + 8 * t + m)] = v[m];
-  }
-  __syncthreads();
-  ct_passes_range<R, N2, 1, LAST, 8, L4, NTHR>(buf, ts, +1);
-  // inverse, last pass (P = N2/8, twiddle index k = t) into registers
-  Cx f[4][8];
-#pragma unroll
-  for (int s = 0; s < 4; ++s) {
-#pragma unroll
-    for (int j = 0; j < 8; ++j) f[s][j] = buf[padded<PS>((4 * r + s) * N2 + t + j * NB)];
-#pragma unroll
-    for (int j = 1; j < 8; ++j) {
-      Cx w = ts[OFF_LAST + (j - 1) * NB + t];
-      w.y = -w.y;
-      f[s][j] = cmul(f[s][j], w);
-    }
-    Dft<Cx, 8>::run(f[s], +1);
-  }
-  __syncthreads();   // every read of buf above precedes the writes below
-  // products at positions t + m N2/8 (element m of f[s]: x[2p] + i x[2p+1])
-  Cx jv[8];
-#pragma unroll
-  for (int m = 0; m < 8; ++m) {
-    const Cx u = f[0][m], v = f[1][m], qx = f[2][m], qy = f[3][m];
-    jv[m] = cmk<Cx>(u.x * qx.x + v.x * qy.x, u.y * qx.y + v.y * qy.y);
-  }
-  Dft<Cx, 8>::run(jv, -1);   // forward, first pass (inputs t + j N2/8)
-#pragma unroll
-  for (int m = 0; m < 8; ++m) buf[padded<PS>((4 * r) * N2 + 8 * t + m)] = jv[m];
-  if (bottom) {   // URMS = sqrt(u^2 + v^2) (QGN_Module.f90:285-289)
-    Cx du[8], dv[8];
-#pragma unroll
-    for (int m = 0; m < 8; ++m) {
-      const Cx u = f[0][m], v = f[1][m];
-      const R r0 = sqrt(u.x * u.x + v.x * v.x), r1 = sqrt(u.y * u.y + v.y * v.y);
-      du[m] = cmk<Cx>(r0 * u.x, r1 * u.y);
-      dv[m] = cmk<Cx>(r0 * v.x, r1 * v.y);
-    }
-    Dft<Cx, 8>::run(du, -1);
-    Dft<Cx, 8>::run(dv, -1);
-#pragma unroll
-    for (int m = 0; m < 8; ++m) {
-      buf[padded<PS>((4 * r + 1) * N2 + 8 * t + m)] = du[m];
-      buf[padded<PS>((4 * r + 2) * N2 + 8 * t + m)] = dv[m];
-    }
-  }
-  __syncthreads();
-  if (bottom) ct_passes_range<R, N2, 1, PL.np, 8, L4, NTHR>(buf, ts, -1);
-  else ct_passes_range<R, N2, 1, PL.np, 8, L1, NTHR>(buf, ts, -1);
-  Cx* jo = reinterpret_cast<Cx*>(a.J) + (long)row0 * Kxp;
-  for (int e = threadIdx.x; e < ROWS * Kx; e += NTHR) {
-    const int rr = e / Kx, k = e - rr * Kx;
-    jo[(long)rr * Kxp + k] = r2c_post_pad<R, PS>(buf, 4 * rr * N2, k, N2, tr);
-  }
-  if (bottom) {
-    Cx* du = reinterpret_cast<Cx*>(a.DU) + (long)((z / g.nzm) * g.ny + j0) * Kxp;
-    Cx* dv = reinterpret_cast<Cx*>(a.DV) + (long)((z / g.nzm) * g.ny + j0) * Kxp;
-    for (int e = threadIdx.x; e < ROWS * Kx; e += NTHR) {
-      const int rr = e / Kx, k = e - rr * Kx;
-      du[(long)rr * Kxp + k] = r2c_post_pad<R, PS>(buf, (4 * rr + 1) * N2, k, N2, tr);
-      dv[(long)rr * Kxp + k] = r2c_post_pad<R, PS>(buf, (4 * rr + 2) * N2, k, N2, tr);
-    }
-  }
-}
-
-// P2, register-fused with one LDS sequence per row: the four inverse
-// transforms run one after another through the same buffer (their results
-// stay in registers), so a row needs N2 LDS slots instead of 4 N2 and more
-// rows fit on a CU.  Same plan requirement as k_row_jac_fused.
-template <typename R, int N2, int ROWS>
-__global__ void __launch_bounds__(ROWS * (N2 / 8)) k_row_jac_fused1(RowJacArgs a, Geom g,
+__global__ void __launch_bounds__(ROWS * (N2 / 8)) k_row_jac_fused2(RowJacArgs a, Geom g,
                                                                    const cplx<R>* __restrict__ twx,
                                                                    const cplx<R>* __restrict__ twr, R scale) {
   using Cx = cplx<R>;
@@ -344,13 +231,9 @@ __global__ void __launch_bounds__(ROWS * (N2 / 8)) k_row_jac_fused1(RowJacArgs a
   const long Kxp = g.Kxp;
   const long roff = (long)(row0 + r) * Kxp;
   const Cx zero = cmk<Cx>((R)0, (R)0);
-  const Cx* srcs[4] = {reinterpret_cast<const Cx*>(a.U), reinterpret_cast<const Cx*>(a.V),
-                       reinterpret_cast<const Cx*>(a.QX), reinterpret_cast<const Cx*>(a.QY)};
-  Cx f[4][8];
-  __syncthreads();
-#pragma unroll
-  for (int s = 0; s < 4; ++s) {
-    const Cx* p = srcs[s] + roff;
+  // x-c2r of one mixed row into registers: o[j] holds x[2m], x[2m+1] at m = t + j NB
+  auto inverse = [&](const void* src, Cx(&o)[8]) {
+    const Cx* p = reinterpret_cast<const Cx*>(src) + roff;
     Cx v[8];
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
@@ -360,51 +243,65 @@ __global__ void __launch_bounds__(ROWS * (N2 / 8)) k_row_jac_fused1(RowJacArgs a
       v[j] = c2r_pre_ab<R>(x, y, k, tr, scale);
     }
     Dft<Cx, 8>::run(v, +1);
-    if (s > 0) __syncthreads();   // the previous field's last-pass reads are done
+    __syncthreads();   // earlier readers of buf are done
 #pragma unroll
     for (int m = 0; m < 8; ++m) buf[padded<PS>(r * N2 + 8 * t + m)] = v[m];
     __syncthreads();
     ct_passes_range<R, N2, 1, LAST, 8, L1, NTHR>(buf, ts, +1);
 #pragma unroll
-    for (int j = 0; j < 8; ++j) f[s][j] = buf[padded<PS>(r * N2 + t + j * NB)];
+    for (int j = 0; j < 8; ++j) o[j] = buf[padded<PS>(r * N2 + t + j * NB)];
 #pragma unroll
     for (int j = 1; j < 8; ++j) {
       Cx w = ts[OFF_LAST + (j - 1) * NB + t];
       w.y = -w.y;
-      f[s][j] = cmul(f[s][j], w);
+      o[j] = cmul(o[j], w);
     }
-    Dft<Cx, 8>::run(f[s], +1);
-  }
-  const long drow = (long)((z / g.nzm) * g.ny + j0) * Kxp;   // bottom layer of member z / nzm
-  Cx* outs[3] = {reinterpret_cast<Cx*>(a.J) + (long)row0 * Kxp, reinterpret_cast<Cx*>(a.DU) + drow,
-                 reinterpret_cast<Cx*>(a.DV) + drow};
-  const int nout = bottom ? 3 : 1;
-  for (int o = 0; o < nout; ++o) {
-    Cx w8[8];
-#pragma unroll
-    for (int m = 0; m < 8; ++m) {
-      const Cx u = f[0][m], v = f[1][m];
-      if (o == 0) {
-        const Cx qx = f[2][m], qy = f[3][m];
-        w8[m] = cmk<Cx>(u.x * qx.x + v.x * qy.x, u.y * qx.y + v.y * qy.y);
-      } else {   // URMS = sqrt(u^2 + v^2) (QGN_Module.f90:285-289)
-        const R r0 = sqrt(u.x * u.x + v.x * v.x), r1 = sqrt(u.y * u.y + v.y * v.y);
-        const Cx c = o == 1 ? u : v;
-        w8[m] = cmk<Cx>(r0 * c.x, r1 * c.y);
-      }
-    }
+    Dft<Cx, 8>::run(o, +1);
+  };
+  // x-r2c of the products in w8 (same positions) to the Kx kept bins of ROWS rows at out
+  auto forward = [&](Cx(&w8)[8], Cx* out) {
     Dft<Cx, 8>::run(w8, -1);
-    __syncthreads();   // previous reads of buf are done
+    __syncthreads();
 #pragma unroll
     for (int m = 0; m < 8; ++m) buf[padded<PS>(r * N2 + 8 * t + m)] = w8[m];
     __syncthreads();
     ct_passes_range<R, N2, 1, PL.np, 8, L1, NTHR>(buf, ts, -1);
-    Cx* out = outs[o];
     for (int e = threadIdx.x; e < ROWS * Kx; e += NTHR) {
       const int rr = e / Kx, k = e - rr * Kx;
       out[(long)rr * Kxp + k] = r2c_post_pad<R, PS>(buf, rr * N2, k, N2, tr);
     }
+  };
+  __syncthreads();
+  Cx fu[8], fv[8];
+  inverse(a.U, fu);
+  inverse(a.V, fv);
+  if (bottom) {   // URMS = sqrt(u^2 + v^2) (QGN_Module.f90:285-289)
+    const long drow = (long)((z / g.nzm) * g.ny + j0) * Kxp;   // bottom layer of member z / nzm
+    Cx w8[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const R r0 = sqrt(fu[m].x * fu[m].x + fv[m].x * fv[m].x), r1 = sqrt(fu[m].y * fu[m].y + fv[m].y * fv[m].y);
+      w8[m] = cmk<Cx>(r0 * fu[m].x, r1 * fu[m].y);
+    }
+    forward(w8, reinterpret_cast<Cx*>(a.DU) + drow);
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const R r0 = sqrt(fu[m].x * fu[m].x + fv[m].x * fv[m].x), r1 = sqrt(fu[m].y * fu[m].y + fv[m].y * fv[m].y);
+      w8[m] = cmk<Cx>(r0 * fv[m].x, r1 * fv[m].y);
+    }
+    forward(w8, reinterpret_cast<Cx*>(a.DV) + drow);
   }
+  Cx acc[8];
+  inverse(a.QX, acc);
+#pragma unroll
+  for (int m = 0; m < 8; ++m) acc[m] = cmk<Cx>(fu[m].x * acc[m].x, fu[m].y * acc[m].y);
+  {
+    Cx qy[8];
+    inverse(a.QY, qy);
+#pragma unroll
+    for (int m = 0; m < 8; ++m) acc[m] = cmk<Cx>(acc[m].x + fv[m].x * qy[m].x, acc[m].y + fv[m].y * qy[m].y);
+  }
+  forward(acc, reinterpret_cast<Cx*>(a.J) + (long)row0 * Kxp);
 }
 
 // P3: y-forward FFT of J, then

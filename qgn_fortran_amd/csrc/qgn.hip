@@ -503,7 +503,7 @@ int rhs_impl(qgn_ctx* c, const double2* qin, void* out, bool psi_ready) {
       // FP32: register-fused row pass, 4 rows per workgroup (measured best)
       constexpr int RW = 4, NT = RW * N2 / 8;
       const size_t lds = (size_t)(row_lds_slots<R>(RW * N2) + ct_tw_size(N2) + N2 + 1) * sizeof(cplx<R>);
-      auto kern = k_row_jac_fused1<R, N2, RW>;
+      auto kern = k_row_jac_fused2<R, N2, RW>;
       rc = allow_lds(kern, lds);
       if (rc) return;
       rc = timed(c, QGN_K_ROW_JAC, [&] {
