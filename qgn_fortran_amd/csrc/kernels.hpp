@@ -864,11 +864,11 @@ __global__ void k_batch_commit(double2* __restrict__ q, const double2* __restric
 // ---------------------------------------------------------------------------
 // Learned closure plumbing: physical q and psi (planar, layers of all
 // members) -> network input (member, y, x, channel) in bf16, channels
-// q_1..q_nzm then psi_1..psi_nzm, each times its scale; network output
-// (member, y, x, layer) times its scale -> planar physical forcing; and the
-// forcing's spectrum added to an RHS on the kept modes.
+// q_1..q_nzm then psi_1..psi_nzm, each times its scale; and the network
+// output (member, y, x, layer) times its scale -> planar physical forcing.
 // ---------------------------------------------------------------------------
-__global__ void k_closure_in(const double* __restrict__ q, const double* __restrict__ psi,
+template <typename R>
+__global__ void k_closure_in(const R* __restrict__ q, const R* __restrict__ psi,
                              const double* __restrict__ scale, short* __restrict__ x, Geom g) {
   const int C = 2 * g.nzm;
   const long plane = (long)g.ny * g.nx;
@@ -878,38 +878,20 @@ __global__ void k_closure_in(const double* __restrict__ q, const double* __restr
     const long pm = e / C;               // member * plane + pixel
     const long m = pm / plane, pix = pm - m * plane;
     const int layer = (int)m * g.nzm + (c < g.nzm ? c : c - g.nzm);
-    const double v = (c < g.nzm ? q : psi)[layer * plane + pix] * scale[c];
+    const double v = (double)(c < g.nzm ? q : psi)[layer * plane + pix] * scale[c];
     x[e] = bf16_rne((float)v);
   }
 }
 
+template <typename R>
 __global__ void k_closure_out(const float* __restrict__ y, const double* __restrict__ scale,
-                              double* __restrict__ f, Geom g) {
+                              R* __restrict__ f, Geom g) {
   const long plane = (long)g.ny * g.nx;
   const long n = plane * g.nz;
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
     const long layer = e / plane, pix = e - layer * plane;
     const int m = (int)(layer / g.nzm), c = (int)(layer - (long)m * g.nzm);
-    f[e] = (double)y[((long)m * plane + pix) * g.nzm + c] * scale[c];
-  }
-}
-
-// out (compact) += full spectrum F on the kept modes, (0,0) excluded
-template <typename S>
-__global__ void k_closure_add(const double2* __restrict__ F, cplx<S>* __restrict__ out, Geom g) {
-  const long n = (long)g.nz * g.Ky * g.Kxp;
-  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
-    const int i = (int)(e % g.Kxp);
-    const long r = e / g.Kxp;
-    const int jc = (int)(r % g.Ky);
-    const int z = (int)(r / g.Ky);
-    const int j = compact_row_to_full(jc, g);
-    if (i >= g.Kx || (i == 0 && j == 0)) continue;
-    const double2 v = F[((long)z * g.ny + j) * g.nxh + i];
-    cplx<S> o = out[e];
-    o.x += (S)v.x;
-    o.y += (S)v.y;
-    out[e] = o;
+    f[e] = (R)((double)y[((long)m * plane + pix) * g.nzm + c] * scale[c]);
   }
 }
 

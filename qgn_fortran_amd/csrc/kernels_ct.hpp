@@ -457,6 +457,92 @@ __global__ void __launch_bounds__(NTHR) k_row_maxabs_ct(const cplx<R>* __restric
   }
 }
 
+// Rows of mixed arrays to physical rows: out[row][x] = c2r(row)(x) * scale
+// (the c2r of k_row_maxabs_ct, stored).  out holds nx reals per row.
+template <typename R, int N2, int ROWS, int NTHR>
+__global__ void __launch_bounds__(NTHR) k_row_c2r_ct(const cplx<R>* __restrict__ mixed, int Kin, int stride,
+                                                     R* __restrict__ out, R scale, Geom g,
+                                                     const cplx<R>* __restrict__ twx, const cplx<R>* __restrict__ twr) {
+  using Cx = cplx<R>;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  constexpr int PS = row_pad_shift<R>();
+  Cx* buf = reinterpret_cast<Cx*>(lds_raw);
+  Cx* ts = buf + row_lds_slots<R>(ROWS * N2);
+  Cx* tr = ts + ct_tw_size(N2);
+  load_table<R>(ts, twx, ct_tw_size(N2));
+  load_table<R>(tr, twr, N2 + 1);
+  __syncthreads();
+  const long row0 = (long)blockIdx.x * ROWS;
+  for (int e = threadIdx.x; e < ROWS * N2; e += NTHR) {
+    const int r = e / N2, k = e - r * N2;
+    buf[padded<PS>(e)] = c2r_pre<R>(mixed + (row0 + r) * stride, Kin, k, N2, tr, scale);
+  }
+  __syncthreads();
+  ct_fft<R, N2, CtLayout<ROWS, N2, 1, false, PS>, NTHR>(buf, ts, +1);
+  Cx* o = reinterpret_cast<Cx*>(out) + row0 * N2;
+  for (int e = threadIdx.x; e < ROWS * N2; e += NTHR) o[e] = buf[padded<PS>(e)];
+}
+
+// Physical rows to mixed rows: out[row][k] = r2c(row)[k] for k < Kx.
+template <typename R, int N2, int ROWS, int NTHR>
+__global__ void __launch_bounds__(NTHR) k_row_r2c_ct(const R* __restrict__ in, cplx<R>* __restrict__ mixed, Geom g,
+                                                     const cplx<R>* __restrict__ twx, const cplx<R>* __restrict__ twr) {
+  using Cx = cplx<R>;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  constexpr int PS = row_pad_shift<R>();
+  Cx* buf = reinterpret_cast<Cx*>(lds_raw);
+  Cx* ts = buf + row_lds_slots<R>(ROWS * N2);
+  Cx* tr = ts + ct_tw_size(N2);
+  load_table<R>(ts, twx, ct_tw_size(N2));
+  load_table<R>(tr, twr, N2 + 1);
+  const long row0 = (long)blockIdx.x * ROWS;
+  const Cx* src = reinterpret_cast<const Cx*>(in) + row0 * N2;
+  for (int e = threadIdx.x; e < ROWS * N2; e += NTHR) buf[padded<PS>(e)] = src[e];
+  __syncthreads();
+  ct_fft<R, N2, CtLayout<ROWS, N2, 1, false, PS>, NTHR>(buf, ts, -1);
+  const int Kx = g.Kx;
+  for (int e = threadIdx.x; e < ROWS * Kx; e += NTHR) {
+    const int r = e / Kx, k = e - r * Kx;
+    mixed[(row0 + r) * g.Kxp + k] = r2c_post_pad<R, PS>(buf, r * N2, k, N2, tr);
+  }
+}
+
+// y-forward FFT of mixed columns, added to a compact array on the kept
+// modes ((0,0) excluded): out += FFT_y(mixed).
+template <typename R, typename S, int NY, int C, int NTHR>
+__global__ void __launch_bounds__(NTHR) k_col_fwd_add_ct(const cplx<R>* __restrict__ mixed, cplx<S>* __restrict__ out,
+                                                         Geom g, const cplx<R>* __restrict__ twy) {
+  using Cx = cplx<R>;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  Cx* buf = reinterpret_cast<Cx*>(lds_raw);
+  Cx* tw = buf + C * NY;
+  using L = CtLayout<C, 1, C, true>;
+  constexpr int TOT = C * NY;
+  const int ntile = g.Kxp / C;
+  const int w = xcd_tile(blockIdx.x, gridDim.x);   // grid = ntile * nz
+  const int i0 = (w % ntile) * C;
+  const int z = w / ntile;
+  load_table<R>(tw, twy, ct_tw_size(NY));
+  for (int e = threadIdx.x; e < TOT; e += NTHR) {
+    const int c = e % C, j = e / C;
+    buf[e] = mixed[((long)z * NY + j) * g.Kxp + i0 + c];
+  }
+  __syncthreads();
+  ct_fft<R, NY, L, NTHR>(buf, tw, -1);
+  const int nown = C * g.Ky;
+  for (int o = threadIdx.x; o < nown; o += NTHR) {
+    const int c = o % C, jc = o / C, i = i0 + c;
+    const int j = compact_row_to_full(jc, g);
+    if (i >= g.Kx || (i == 0 && j == 0)) continue;
+    const long idx = ((long)z * g.Ky + jc) * g.Kxp + i;
+    const Cx v = buf[j * C + c];
+    cplx<S> r = out[idx];
+    r.x += (S)v.x;
+    r.y += (S)v.y;
+    out[idx] = r;
+  }
+}
+
 // Final max over the per-block partials: workgroup b reduces partials
 // [b n, (b+1) n) into slot[b] (one per member of a batch).
 __global__ void __launch_bounds__(1024) k_max_final(const unsigned long long* __restrict__ partial, int n,

@@ -280,7 +280,6 @@ struct qgn_ctx {
   // learned closure: network, scales and work arrays
   qgn_cnn* net = nullptr;
   double *cl_in = nullptr, *cl_out = nullptr;   // 2 nzm and nzm scales
-  double2 *cl_F = nullptr, *cl_psi64 = nullptr;
   double *cl_pq = nullptr, *cl_pp = nullptr;
   short* cl_x = nullptr;
   float* cl_y = nullptr;
@@ -563,47 +562,91 @@ int rhs_impl(qgn_ctx* c, const double2* qin, void* out, bool psi_ready) {
   return 0;
 }
 
-// psi_ready: psi of qin was already written by k_stage_proj
-int lincomb(qgn_ctx* c, const void* X0, const void* X1, const void* X2, const void* X3, double c0, double c1,
-            double c2, double c3, int nxs, void* out);
-template <typename R>
-int dft_fwd(hipStream_t s, const Geom& g, const FftPlan& px, const FftPlan& py, const Tw<R>& tw, int howmany,
-            const R* grid, cplx<R>* spec, cplx<R>* tmp);
-template <typename R>
-int dft_bwd(hipStream_t s, const Geom& g, const FftPlan& px, const FftPlan& py, const Tw<R>& tw, int howmany,
-            const cplx<R>* spec, R* grid, cplx<R>* tmp, R scale);
+// Row passes of the closure in the RHS type R: c2r of mixed rows to
+// physical rows (times scale), and r2c of physical rows to the Kx bins of
+// mixed rows; RW rows per workgroup (RW divides ny).
+template <typename R, bool INVERSE>
+int launch_row_tx(qgn_ctx* c, const void* src, void* dst, R scale) {
+  const Geom& g = c->g;
+  const Tw<R>& tw = tw_of<R>(c);
+  int rc = 0;
+  bool hit = with_size(g.N2, [&](auto nc) {
+    constexpr int N2 = decltype(nc)::value;
+    using Cfg = RowCfg<float2, N2, 4>;
+    constexpr int RW = Cfg::ROWS * 4 > 16 ? 16 : Cfg::ROWS * 4;
+    constexpr int NT0 = RW * N2 / 8;
+    constexpr int NT = NT0 < 64 ? 64 : (NT0 > 1024 ? 1024 : ((NT0 + 63) / 64) * 64);
+    const size_t lds = (size_t)(row_lds_slots<R>(RW * N2) + ct_tw_size(N2) + N2 + 1) * sizeof(cplx<R>);
+    const dim3 grid(g.ny * g.nz / RW);
+    if constexpr (INVERSE) {
+      auto kern = k_row_c2r_ct<R, N2, RW, NT>;
+      rc = allow_lds(kern, lds);
+      if (rc) return;
+      hipLaunchKernelGGL(kern, grid, dim3(NT), lds, c->stream, reinterpret_cast<const cplx<R>*>(src), g.Kx, g.Kxp,
+                         reinterpret_cast<R*>(dst), scale, g, tw.sx, tw.r);
+    } else {
+      auto kern = k_row_r2c_ct<R, N2, RW, NT>;
+      rc = allow_lds(kern, lds);
+      if (rc) return;
+      hipLaunchKernelGGL(kern, grid, dim3(NT), lds, c->stream, reinterpret_cast<const R*>(src),
+                         reinterpret_cast<cplx<R>*>(dst), g, tw.sx, tw.r);
+    }
+    rc = hipGetLastError() == hipSuccess ? 0 : fail(QGN_EHIP, "row transform launch failed");
+  }, CtSizes{});
+  if (!hit) return fail(QGN_EINVAL, "nx/2 not supported by the row transforms");
+  return rc;
+}
+
+// out (compact, type S) += y-forward FFT of mixed (type R) on the kept modes
+template <typename R, typename S>
+int launch_col_fwd_add(qgn_ctx* c, const void* mixed, void* out) {
+  const Geom& g = c->g;
+  const Tw<R>& tw = tw_of<R>(c);
+  int rc = 0;
+  bool hit = with_size(g.ny, [&](auto nyc) {
+    constexpr int NY = decltype(nyc)::value;
+    using Cfg = ColCfg<R, NY>;
+    auto kern = k_col_fwd_add_ct<R, S, NY, Cfg::C, Cfg::NTHR>;
+    rc = allow_lds(kern, Cfg::LDS);
+    if (rc) return;
+    hipLaunchKernelGGL(kern, dim3(g.Kxp / Cfg::C * g.nz), dim3(Cfg::NTHR), Cfg::LDS, c->stream,
+                       reinterpret_cast<const cplx<R>*>(mixed), reinterpret_cast<cplx<S>*>(out), g, tw.sy);
+    rc = hipGetLastError() == hipSuccess ? 0 : fail(QGN_EHIP, "column transform launch failed");
+  }, CtSizes{});
+  if (!hit) return fail(QGN_EINVAL, "ny not supported by the column transforms");
+  return rc;
+}
 
 // Learned closure: out += r2c(net(q, psi) * scale) on the kept modes, with
-// q, psi the physical fields of qin and of the psi the RHS pass just formed.
+// q, psi the physical fields of qin and of the psi the RHS pass just formed,
+// all in the RHS's arithmetic type (the network itself takes bf16).  Reuses
+// the RHS pass's mixed buffers, which it has finished with.
 template <typename S>
 int closure_add(qgn_ctx* c, const double2* qin, void* out) {
+  using R = S;
   const Geom& g = c->g;
   hipStream_t s = c->stream;
-  const long nf = (long)g.nz * g.ny * g.nxh;
   const long np = (long)g.nz * g.ny * g.nx;
-  const double nn = 1.0 / ((double)g.nx * (double)g.ny);
-  hipLaunchKernelGGL(k_unpack, dim3(grid1d(nf)), dim3(256), 0, s, qin, c->cl_F, g);
-  HIPCHK(hipGetLastError());
-  CHK(dft_bwd<double>(s, g, c->px, c->py, c->twd, g.nz, c->cl_F, c->cl_pq, c->fb, nn));
-  const double2* psi64 = c->psi64;
-  if (std::is_same<S, float>::value) {
-    CHK(lincomb(c, c->psi, nullptr, nullptr, nullptr, 1.0, 0, 0, 0, 1, c->cl_psi64));
-    psi64 = c->cl_psi64;
-  }
-  hipLaunchKernelGGL(k_unpack, dim3(grid1d(nf)), dim3(256), 0, s, psi64, c->cl_F, g);
-  HIPCHK(hipGetLastError());
-  CHK(dft_bwd<double>(s, g, c->px, c->py, c->twd, g.nz, c->cl_F, c->cl_pp, c->fb, nn));
-  hipLaunchKernelGGL(k_closure_in, dim3(grid1d(2 * np)), dim3(256), 0, s, c->cl_pq, c->cl_pp, c->cl_in, c->cl_x, g);
+  const int isd = std::is_same<R, double>::value;
+  ColInvArgs ca{};
+  ca.nf = 2;
+  ca.src[0] = qin; ca.src_is_double[0] = 1; ca.mult[0] = 0; ca.dst[0] = c->Um;
+  ca.src[1] = isd ? (const void*)c->psi64 : (const void*)c->psi; ca.src_is_double[1] = isd; ca.mult[1] = 0;
+  ca.dst[1] = c->Vm;
+  CHK(launch_col_inv<R>(c, ca));
+  const R nn = (R)(1.0 / ((double)g.nx * (double)g.ny));
+  R* pq = reinterpret_cast<R*>(c->cl_pq);
+  R* pp = reinterpret_cast<R*>(c->cl_pp);
+  CHK((launch_row_tx<R, true>(c, c->Um, pq, nn)));
+  CHK((launch_row_tx<R, true>(c, c->Vm, pp, nn)));
+  hipLaunchKernelGGL((k_closure_in<R>), dim3(grid1d(2 * np)), dim3(256), 0, s, pq, pp, c->cl_in, c->cl_x, g);
   HIPCHK(hipGetLastError());
   if (qcnn::forward(c->net, reinterpret_cast<const unsigned short*>(c->cl_x), c->nm, g.ny, g.nx, c->cl_y, s))
     return fail(QGN_EINVAL, std::string("closure network: ") + qcnn::last_error());
-  hipLaunchKernelGGL(k_closure_out, dim3(grid1d(np)), dim3(256), 0, s, c->cl_y, c->cl_out, c->cl_pq, g);
+  hipLaunchKernelGGL((k_closure_out<R>), dim3(grid1d(np)), dim3(256), 0, s, c->cl_y, c->cl_out, pq, g);
   HIPCHK(hipGetLastError());
-  CHK(dft_fwd<double>(s, g, c->px, c->py, c->twd, g.nz, c->cl_pq, c->cl_F, c->fb));
-  hipLaunchKernelGGL((k_closure_add<S>), dim3(grid1d(c->ncmp)), dim3(256), 0, s, c->cl_F,
-                     reinterpret_cast<cplx<S>*>(out), g);
-  HIPCHK(hipGetLastError());
-  return 0;
+  CHK((launch_row_tx<R, false>(c, pq, c->Jm, (R)1)));
+  return launch_col_fwd_add<R, S>(c, c->Jm, out);
 }
 
 int rhs(qgn_ctx* c, const double2* qin, void* out, bool psi_ready = false) {
@@ -1085,7 +1128,7 @@ void destroy_ctx(qgn_ctx* c) {
   for (int i = 0; i < 5; ++i)
     if (c->mp[i]) (void)hipFree(c->mp[i]);
   if (c->spec) (void)hipFree(c->spec);
-  void* cl[] = {c->cl_in, c->cl_out, c->cl_F, c->cl_psi64, c->cl_pq, c->cl_pp, c->cl_x, c->cl_y};
+  void* cl[] = {c->cl_in, c->cl_out, c->cl_pq, c->cl_pp, c->cl_x, c->cl_y};
   for (void* x : cl)
     if (x) (void)hipFree(x);
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
@@ -1433,13 +1476,10 @@ int qgn_set_closure(qgn_ctx* c, qgn_cnn* net, const double* in_scale, const doub
     return fail(QGN_EINVAL, "closure network needs 2*nz input and nz output channels");
   const long pix = (long)c->nm * g.ny * g.nx;
   if (g.nx % 16 || pix % 64) return fail(QGN_EINVAL, "closure network needs nx % 16 == 0");
-  if (!c->cl_F) {
-    const long nf = (long)g.nz * g.ny * g.nxh;
+  if (!c->cl_in) {
     const long np = (long)g.nz * g.ny * g.nx;
     CHK(alloc((void**)&c->cl_in, 2 * (size_t)g.nzm * sizeof(double)));
     CHK(alloc((void**)&c->cl_out, (size_t)g.nzm * sizeof(double)));
-    CHK(alloc((void**)&c->cl_F, nf * sizeof(double2)));
-    CHK(alloc((void**)&c->cl_psi64, (size_t)c->ncmp * sizeof(double2)));
     CHK(alloc((void**)&c->cl_pq, np * sizeof(double)));
     CHK(alloc((void**)&c->cl_pp, np * sizeof(double)));
     CHK(alloc((void**)&c->cl_x, pix * 2 * g.nzm * sizeof(short)));
