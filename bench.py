@@ -173,6 +173,41 @@ def cpu_baseline(wl, q_spun, t0, dt0, budget_s=30.0):
                        f"oracle, {wall:.1f} s")
 
 
+def run_cnn(steps, warmup):
+    """The closure network's BF16 MFMA kernels at the closure's shapes: train
+    steps on 10 images of 256^2 x 16 channels (the training batch) and
+    forward passes on 4 images (the 4-member coarse ensemble), with HIP-event
+    timings and the dense-MFMA flops of each."""
+    import torch
+    from oracle import cnn_oracle as co
+    from qgn_fortran_amd.cnn import CNN, to_bf16_bits
+    cin, cout, H, W = 16, 8, 256, 256
+    rng = np.random.default_rng(0)
+    net = CNN(cin, cout, co.init_params(cin, cout, 1))
+    macs_px = sum(o * 9 * i for i, o in co.shapes(cin, cout))        # per pixel, forward
+    out = {}
+    for name, n, train in (("train_step", 10, True), ("forward", 4, False)):
+        x = torch.from_numpy(to_bf16_bits(rng.standard_normal((n, H, W, cin)))).cuda()
+        t = torch.from_numpy(rng.standard_normal((n, H, W, cout)).astype(np.float32)).cuda()
+        y = torch.empty((n, H, W, cout), device="cuda", dtype=torch.float32)
+        call = (lambda: net.train_step(x, t, y, 1e-4)) if train else (lambda: net.forward(x, y))
+        for _ in range(warmup):
+            call()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            call()
+        torch.cuda.synchronize()
+        ms = 1e3 * (time.perf_counter() - t0) / steps
+        # forward 2 flops per MAC; a train step adds the data and weight gradients (2x forward, the
+        # first layer's data gradient skipped)
+        flops = 2.0 * n * H * W * macs_px * (3.0 if train else 1.0)
+        out[name] = dict(images=n, ms=ms, tflops=flops / (ms * 1e-3) / 1e12,
+                         frac_bf16_dense_peak=flops / (ms * 1e-3) / 2.5e15)
+    net.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -183,6 +218,8 @@ def main():
     ap.add_argument("--kernel", default="row_jac", help="kernel timed for the roofline figure")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=30.0)
+    ap.add_argument("--cnn", action="store_true",
+                    help="time the closure network's MFMA kernels instead of the stepper")
     ap.add_argument("--calib-copy", action="store_true",
                     help="first run a 1 GiB read + 1 GiB write multiply three times (PMC byte-counter reference)")
     a = ap.parse_args()
@@ -205,6 +242,9 @@ def main():
             torch.mul(src, 2.0, out=dst)
         torch.cuda.synchronize()
         del src, dst
+    if a.cnn:
+        print(json.dumps({"cnn": run_cnn(a.steps, a.warmup)}), flush=True)
+        return
     wl = configs.WORKLOADS[a.workload]()
     q, t0, dt0 = spin_up(wl, a.spinup)
     mixed, _ = run_gpu(configs.WORKLOADS[a.workload](), "mixed", q, t0, dt0, a.steps, a.warmup, a.kernel)

@@ -55,12 +55,14 @@ __device__ __forceinline__ float bf2f(u16 h) { return __uint_as_float(((unsigned
 // MODE 1: fp32 out = acc + bias                (last layer, forward)
 // MODE 2: bf16 out = acc * (mask > 0)          (backward data; mask = layer input)
 // in: (n, H, W, CI) bf16; wpk: (COP, 9, CI) bf16 with rows >= CO zero;
-// out: (n, H, W, CO).  Block: 4 waves, 16 consecutive pixels of one row each;
-// needs W % 16 == 0 and n H W % 64 == 0.
-template <int CI, int CO, int MODE>
+// out: (n, H, W, CO).  Block: 4 waves; each wave takes TP tiles of 16
+// consecutive pixels of one row (wave w of block b: tiles TP (4 b + w) ..),
+// so every weight operand read from LDS feeds TP independent MFMA chains.
+// Needs W % 16 == 0 and n H W % 16 == 0.
+template <int CI, int CO, int MODE, int TP>
 __global__ void __launch_bounds__(256) k_conv3x3(const u16* __restrict__ in, const u16* __restrict__ wpk,
                                                  const float* __restrict__ bias, const u16* __restrict__ mask,
-                                                 void* __restrict__ out, int H, int W) {
+                                                 void* __restrict__ out, int H, int W, long ntiles) {
   constexpr int COP = pad16(CO);
   constexpr int NW = COP * 9 * CI;
   __shared__ __align__(16) u16 ws[NW];
@@ -69,64 +71,85 @@ __global__ void __launch_bounds__(256) k_conv3x3(const u16* __restrict__ in, con
   __syncthreads();
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int j = lane & 15, g = lane >> 4;
-  const long p0 = (long)blockIdx.x * 64 + w * 16;
-  const long row = p0 / W;                  // n H + y
-  const int x0 = (int)(p0 - row * W);
-  const int y = (int)(row % H);
-  const long img = (row - y) * W;           // first pixel of image n
-  f4v acc[COP / 16];
+  const long t0 = ((long)blockIdx.x * 4 + w) * TP;
+  long img[TP];
+  int y[TP], x0[TP];
+  bool on[TP];
 #pragma unroll
-  for (int mt = 0; mt < COP / 16; ++mt) acc[mt] = f4v{0.f, 0.f, 0.f, 0.f};
+  for (int u = 0; u < TP; ++u) {
+    on[u] = t0 + u < ntiles;
+    const long p0 = on[u] ? (t0 + u) * 16 : 0;
+    const long row = p0 / W;                // n H + y
+    x0[u] = (int)(p0 - row * W);
+    y[u] = (int)(row % H);
+    img[u] = (row - y[u]) * W;               // first pixel of image n
+  }
+  f4v acc[TP][COP / 16];
+#pragma unroll
+  for (int u = 0; u < TP; ++u)
+#pragma unroll
+    for (int mt = 0; mt < COP / 16; ++mt) acc[u][mt] = f4v{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int tap = 0; tap < 9; ++tap) {
     const int dy = tap / 3 - 1, dx = tap % 3 - 1;
-    int yy = y + dy;
-    yy += yy < 0 ? H : 0;
-    yy -= yy >= H ? H : 0;
-    int xx = x0 + j + dx;
-    xx += xx < 0 ? W : 0;
-    xx -= xx >= W ? W : 0;
-    const u16* src = in + (img + (long)yy * W + xx) * CI + 4 * g;
+    const u16* src[TP];
+#pragma unroll
+    for (int u = 0; u < TP; ++u) {
+      int yy = y[u] + dy;
+      yy += yy < 0 ? H : 0;
+      yy -= yy >= H ? H : 0;
+      int xx = x0[u] + j + dx;
+      xx += xx < 0 ? W : 0;
+      xx -= xx >= W ? W : 0;
+      src[u] = in + (img[u] + (long)yy * W + xx) * CI + 4 * g;
+    }
 #pragma unroll
     for (int kc = 0; kc < CI / 16; ++kc) {
-      const s4v b = *reinterpret_cast<const s4v*>(src + kc * 16);
+      s4v b[TP];
+#pragma unroll
+      for (int u = 0; u < TP; ++u) b[u] = *reinterpret_cast<const s4v*>(src[u] + kc * 16);
 #pragma unroll
       for (int mt = 0; mt < COP / 16; ++mt) {
         const s4v a = *reinterpret_cast<const s4v*>(&ws[((mt * 16 + j) * 9 + tap) * CI + kc * 16 + 4 * g]);
-        acc[mt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[mt], 0, 0, 0);
+#pragma unroll
+        for (int u = 0; u < TP; ++u) acc[u][mt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[u], acc[u][mt], 0, 0, 0);
       }
     }
   }
-  const long p = p0 + j;
 #pragma unroll
-  for (int mt = 0; mt < COP / 16; ++mt) {
-    const int co = mt * 16 + 4 * g;
-    if (co >= CO) continue;                 // CO is a multiple of 4
-    if constexpr (MODE == 1) {
-      float4 v;
-      v.x = acc[mt][0] + bias[co];
-      v.y = acc[mt][1] + bias[co + 1];
-      v.z = acc[mt][2] + bias[co + 2];
-      v.w = acc[mt][3] + bias[co + 3];
-      *reinterpret_cast<float4*>(reinterpret_cast<float*>(out) + p * CO + co) = v;
-    } else {
-      float v[4];
-      if constexpr (MODE == 0) {
+  for (int u = 0; u < TP; ++u) {
+    if (!on[u]) continue;
+    const long p = (t0 + u) * 16 + j;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) v[r] = fmaxf(acc[mt][r] + bias[co + r], 0.f);
+    for (int mt = 0; mt < COP / 16; ++mt) {
+      const int co = mt * 16 + 4 * g;
+      if (co >= CO) continue;               // CO is a multiple of 4
+      if constexpr (MODE == 1) {
+        float4 v;
+        v.x = acc[u][mt][0] + bias[co];
+        v.y = acc[u][mt][1] + bias[co + 1];
+        v.z = acc[u][mt][2] + bias[co + 2];
+        v.w = acc[u][mt][3] + bias[co + 3];
+        *reinterpret_cast<float4*>(reinterpret_cast<float*>(out) + p * CO + co) = v;
       } else {
-        const ushort4 m = *reinterpret_cast<const ushort4*>(mask + p * CO + co);
-        v[0] = bf2f(m.x) > 0.f ? acc[mt][0] : 0.f;
-        v[1] = bf2f(m.y) > 0.f ? acc[mt][1] : 0.f;
-        v[2] = bf2f(m.z) > 0.f ? acc[mt][2] : 0.f;
-        v[3] = bf2f(m.w) > 0.f ? acc[mt][3] : 0.f;
+        float v[4];
+        if constexpr (MODE == 0) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) v[r] = fmaxf(acc[u][mt][r] + bias[co + r], 0.f);
+        } else {
+          const ushort4 m = *reinterpret_cast<const ushort4*>(mask + p * CO + co);
+          v[0] = bf2f(m.x) > 0.f ? acc[u][mt][0] : 0.f;
+          v[1] = bf2f(m.y) > 0.f ? acc[u][mt][1] : 0.f;
+          v[2] = bf2f(m.z) > 0.f ? acc[u][mt][2] : 0.f;
+          v[3] = bf2f(m.w) > 0.f ? acc[u][mt][3] : 0.f;
+        }
+        ushort4 o;
+        o.x = f2bf(v[0]);
+        o.y = f2bf(v[1]);
+        o.z = f2bf(v[2]);
+        o.w = f2bf(v[3]);
+        *reinterpret_cast<ushort4*>(reinterpret_cast<u16*>(out) + p * CO + co) = o;
       }
-      ushort4 o;
-      o.x = f2bf(v[0]);
-      o.y = f2bf(v[1]);
-      o.z = f2bf(v[2]);
-      o.w = f2bf(v[3]);
-      *reinterpret_cast<ushort4*>(reinterpret_cast<u16*>(out) + p * CO + co) = o;
     }
   }
 }
@@ -345,9 +368,11 @@ int ensure(qgn_cnn* h, long npix) {
 
 template <int CI, int CO, int MODE>
 int conv(const u16* in, const u16* w, const float* b, const u16* mask, void* out, int n, int H, int W, hipStream_t s) {
-  const long npix = (long)n * H * W;
-  hipLaunchKernelGGL((k_conv3x3<CI, CO, MODE>), dim3((unsigned)(npix / 64)), dim3(256), 0, s, in, w, b, mask, out, H,
-                     W);
+  constexpr int TP = 4;   // pixel tiles per wave
+  const long ntiles = (long)n * H * W / 16;
+  const long blocks = (ntiles + 4 * TP - 1) / (4 * TP);
+  hipLaunchKernelGGL((k_conv3x3<CI, CO, MODE, TP>), dim3((unsigned)blocks), dim3(256), 0, s, in, w, b, mask, out, H,
+                     W, ntiles);
   CCHK(hipGetLastError());
   return 0;
 }
