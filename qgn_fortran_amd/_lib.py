@@ -25,7 +25,7 @@ EXPORTS = [
     "qgn_batch_get_time", "qgn_ensemble_create", "qgn_ensemble_step", "qgn_ensemble_destroy",
     "qgn_cnn_create", "qgn_cnn_destroy", "qgn_cnn_nparams", "qgn_cnn_get_params", "qgn_cnn_set_params",
     "qgn_cnn_forward", "qgn_cnn_backward", "qgn_cnn_adam", "qgn_cnn_train_step", "qgn_cnn_last_error",
-    "qgn_set_closure",
+    "qgn_set_closure", "qgn_get_tuning", "qgn_set_tuning",
 ]
 
 
@@ -106,6 +106,8 @@ def load():
         "qgn_cnn_train_step": (I, [P, P, P, I, I, I, P, C.c_float, D]),
         "qgn_cnn_last_error": (C.c_char_p, []),
         "qgn_set_closure": (I, [P, P, D, D]),
+        "qgn_get_tuning": (I, [P, C.POINTER(I), C.POINTER(I), C.POINTER(I), C.POINTER(C.c_float)]),
+        "qgn_set_tuning": (I, [P, I, I, I]),
         "qgn_last_error": (C.c_char_p, []),
         "qgn_version": (C.c_char_p, []),
     }

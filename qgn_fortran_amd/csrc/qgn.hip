@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <type_traits>
@@ -283,6 +284,10 @@ struct qgn_ctx {
   double *cl_pq = nullptr, *cl_pp = nullptr;
   short* cl_x = nullptr;
   float* cl_y = nullptr;
+  // launch choices of the FP32 passes (0: the fixed rule), picked by
+  // autotune() at creation or set by qgn_set_tuning; the measured times (us)
+  int tune_col_inv = 0, tune_col_rhs = 0, tune_row = 0;
+  float tune_ms[3][4] = {};
   // device-controlled ARK43 attempts of a batch: per-member controller state
   double *bt = nullptr, *bdt = nullptr, *berr0 = nullptr, *blast = nullptr;
   long long* bcount = nullptr;  // steps, attempts, rejects per member (3 nm)
@@ -381,14 +386,36 @@ constexpr int pow2_floor(int x) {
 // Column tiles: C adjacent kx columns per workgroup, 8 elements per thread
 // (measured best on MI355X at 1024 points: 8 FP32 / 4 FP64 columns, i.e.
 // 64-byte row segments, two workgroups per CU).
-template <typename R, int NY> struct ColCfg {
-  static constexpr bool f32 = std::is_same<R, float>::value;
-  // a power of two, so it divides the 16-column padding of Kxp
-  static constexpr int C = f32 ? (NY <= 1024 ? 8 : pow2_floor(8192 / NY)) : (NY <= 2048 ? 4 : 2);
+// A column tile of C columns (a power of two, so it divides the 16-column
+// padding of Kxp), 8 elements per thread.
+template <typename R, int NY, int C> struct ColTile {
   static constexpr int NT0 = C * NY / 8;
   static constexpr int NTHR = NT0 < 64 ? 64 : ((NT0 + 63) / 64) * 64;
   static constexpr size_t LDS = (size_t)(C * NY + ct_tw_size(NY)) * sizeof(cplx<R>);
+  static constexpr bool ok = NT0 <= 1024 && LDS <= 160 * 1024;
 };
+bool col_tile_ok(int C, int ny, size_t rsz) {
+  return (C == 2 || C == 4 || C == 8 || C == 16) && C * ny / 8 <= 1024 &&
+         (size_t)(C * ny + ct_tw_size(ny)) * rsz <= 160 * 1024;
+}
+// the fixed rule: 8 FP32 / 4 FP64 columns while a tile fits 1024 threads
+template <typename R, int NY> struct ColCfg {
+  static constexpr bool f32 = std::is_same<R, float>::value;
+  static constexpr int C = f32 ? (NY <= 1024 ? 8 : pow2_floor(8192 / NY)) : (NY <= 2048 ? 4 : 2);
+  static constexpr int NTHR = ColTile<R, NY, C>::NTHR;
+  static constexpr size_t LDS = ColTile<R, NY, C>::LDS;
+};
+// runtime tile width -> compile-time constant
+template <typename F>
+bool with_tile(int C, F&& f) {
+  switch (C) {
+    case 2: f(std::integral_constant<int, 2>{}); return true;
+    case 4: f(std::integral_constant<int, 4>{}); return true;
+    case 8: f(std::integral_constant<int, 8>{}); return true;
+    case 16: f(std::integral_constant<int, 16>{}); return true;
+    default: return false;
+  }
+}
 // Row passes: ROWS rows per workgroup (every compiled ny is a multiple of
 // 16, so a workgroup stays inside one layer), 8 elements per thread.
 template <typename R, int N2, int NSEQ> struct RowCfg {
@@ -441,14 +468,27 @@ int launch_col_inv(qgn_ctx* c, const ColInvArgs& a) {
   int rc = 0;
   bool hit = with_size(g.ny, [&](auto nyc) {
     constexpr int NY = decltype(nyc)::value;
-    using Cfg = ColCfg<R, NY>;
-    auto kern = k_col_inv_ct<R, NY, Cfg::C, Cfg::NTHR>;
-    rc = allow_lds(kern, Cfg::LDS);
-    if (rc) return;
-    dim3 grid(g.Kxp / Cfg::C * g.nz * a.nf);
-    rc = timed(c, QGN_K_COL_INV, [&] {
-      hipLaunchKernelGGL(kern, grid, dim3(Cfg::NTHR), Cfg::LDS, c->stream, a, g, tw.sy);
-    });
+    auto go = [&](auto cc) {
+      constexpr int C = decltype(cc)::value;
+      using T = ColTile<R, NY, C>;
+      if constexpr (!T::ok) {
+        rc = fail(QGN_EINVAL, "column tile does not fit");
+      } else {
+        auto kern = k_col_inv_ct<R, NY, C, T::NTHR>;
+        rc = allow_lds(kern, T::LDS);
+        if (rc) return;
+        dim3 grid(g.Kxp / C * g.nz * a.nf);
+        rc = timed(c, QGN_K_COL_INV, [&] {
+          hipLaunchKernelGGL(kern, grid, dim3(T::NTHR), T::LDS, c->stream, a, g, tw.sy);
+        });
+      }
+    };
+    if constexpr (std::is_same<R, float>::value) {
+      if (!with_tile(c->tune_col_inv ? c->tune_col_inv : ColCfg<R, NY>::C, go))
+        rc = fail(QGN_EINVAL, "bad column tile");
+    } else {
+      go(std::integral_constant<int, ColCfg<R, NY>::C>{});
+    }
   }, CtSizes{});
   if (!hit) return fail(QGN_EINVAL, "ny not supported by the RHS passes");
   return rc;
@@ -465,11 +505,108 @@ int launch_stage_proj_mfma(qgn_ctx* c, const StageArgs& a, int kernel_id) {
 }
 bool use_mfma_projection(const qgn_ctx* c) { return c->p.precision == QGN_MIXED && c->g.nzm <= 16; }
 
+// P2: the Jacobian row pass.  FP32 with 64 | nx/2 uses the register-fused
+// kernel with c->tune_row rows per workgroup (default 4); otherwise the
+// persistent-grid kernel.
+template <typename R>
+int launch_row_jac(qgn_ctx* c, const RowJacArgs& ra) {
+  const Geom& g = c->g;
+  const Tw<R>& tw = tw_of<R>(c);
+  const R scale = (R)(1.0 / ((double)g.nx * (double)g.ny));
+  int rc = 0;
+  bool hit = with_size(g.N2, [&](auto nc) {
+    constexpr int N2 = decltype(nc)::value;
+    if constexpr (std::is_same<R, float>::value && N2 % 64 == 0 && N2 <= 1024) {
+      auto go = [&](auto rr) {
+        constexpr int RW = decltype(rr)::value, NT = RW * N2 / 8;
+        if constexpr (NT > 1024) {
+          rc = fail(QGN_EINVAL, "row group does not fit");
+        } else {
+          const size_t lds = (size_t)(row_lds_slots<R>(RW * N2) + ct_tw_size(N2) + N2 + 1) * sizeof(cplx<R>);
+          auto kern = k_row_jac_fused2<R, N2, RW>;
+          rc = allow_lds(kern, lds);
+          if (rc) return;
+          rc = timed(c, QGN_K_ROW_JAC, [&] {
+            hipLaunchKernelGGL(kern, dim3(g.ny * g.nz / RW), dim3(NT), lds, c->stream, ra, g, tw.sx, tw.r, scale);
+          });
+        }
+      };
+      switch (c->tune_row ? c->tune_row : 4) {
+        case 2: go(std::integral_constant<int, 2>{}); break;
+        case 4: go(std::integral_constant<int, 4>{}); break;
+        case 8: go(std::integral_constant<int, 8>{}); break;
+        default: rc = fail(QGN_EINVAL, "bad row group");
+      }
+      return;
+    }
+    using Cfg = RowCfg<cplx<R>, N2, 4>;
+    constexpr int NT = Cfg::NTHR, RW = Cfg::ROWS;
+    const size_t lds = (size_t)(row_lds_slots<R>(RW * 4 * N2) + ct_tw_size(N2) + N2 + 1) * sizeof(cplx<R>);
+    auto kern = k_row_jac_ct<R, N2, RW, NT>;
+    rc = allow_lds(kern, lds);
+    if (rc) return;
+    const int ngroups = g.ny * g.nz / RW;
+    int per_cu = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, NT, lds) != hipSuccess || per_cu < 1)
+      per_cu = 1;
+    const int grid = std::min(ngroups, per_cu * c->ncu);
+    rc = timed(c, QGN_K_ROW_JAC, [&] {
+      hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), lds, c->stream, ra, g, tw.sx, tw.r, scale, ngroups);
+    });
+  }, CtSizes{});
+  if (!hit) return fail(QGN_EINVAL, "nx/2 not supported by the RHS passes");
+  return rc;
+}
+
+// P3 (FP32: c->tune_col_rhs columns per tile, default the fixed rule) and
+// the bottom-layer drag pass.
+template <typename R>
+int launch_col_rhs(qgn_ctx* c, const ColRhsArgs& ka, bool drag) {
+  const Geom& g = c->g;
+  const Tw<R>& tw = tw_of<R>(c);
+  DragArgs da{};
+  da.DU = c->DUm; da.DV = c->DVm; da.out = ka.out;
+  da.cdrag = c->cdm;
+  int rc = 0;
+  with_size(g.ny, [&](auto nyc) {
+    constexpr int NY = decltype(nyc)::value;
+    using Cfg = ColCfg<R, NY>;
+    auto go = [&](auto cc) {
+      constexpr int C = decltype(cc)::value;
+      using T = ColTile<R, NY, C>;
+      if constexpr (!T::ok) {
+        rc = fail(QGN_EINVAL, "column tile does not fit");
+      } else {
+        auto kern = k_col_rhs_ct<R, R, NY, C, T::NTHR>;
+        rc = allow_lds(kern, T::LDS);
+        if (rc) return;
+        rc = timed(c, QGN_K_COL_RHS, [&] {
+          hipLaunchKernelGGL(kern, dim3(g.Kxp / C * g.nz), dim3(T::NTHR), T::LDS, c->stream, ka, g, tw.sy);
+        });
+      }
+    };
+    if constexpr (std::is_same<R, float>::value) {
+      if (!with_tile(c->tune_col_rhs ? c->tune_col_rhs : Cfg::C, go)) rc = fail(QGN_EINVAL, "bad column tile");
+    } else {
+      go(std::integral_constant<int, Cfg::C>{});
+    }
+    if (rc || !drag) return;
+    constexpr int CD = Cfg::C / 2 > 0 ? Cfg::C / 2 : 1;
+    constexpr int NTD0 = 2 * CD * NY / 8;
+    constexpr int NTD = NTD0 < 64 ? 64 : ((NTD0 + 63) / 64) * 64;
+    constexpr size_t LDSD = (size_t)(2 * CD * NY + ct_tw_size(NY)) * sizeof(cplx<R>);
+    auto dk = k_col_drag_ct<R, R, NY, CD, NTD>;
+    rc = allow_lds(dk, LDSD);
+    if (rc) return;
+    hipLaunchKernelGGL(dk, dim3(g.Kxp / CD, c->nm), dim3(NTD), LDSD, c->stream, da, g, tw.sy);
+    rc = hipGetLastError() == hipSuccess ? 0 : fail(QGN_EHIP, "drag kernel launch failed");
+  }, CtSizes{});
+  return rc;
+}
+
 // GetRHS on compact arrays: out = RHS(qin)
 template <typename R>
 int rhs_impl(qgn_ctx* c, const double2* qin, void* out, bool psi_ready) {
-  const Geom& g = c->g;
-  const Tw<R>& tw = tw_of<R>(c);
   void* psi = std::is_same<R, double>::value ? (void*)c->psi64 : c->psi;
   const int isd = std::is_same<R, double>::value;
   if (!psi_ready) {
@@ -494,39 +631,7 @@ int rhs_impl(qgn_ctx* c, const double2* qin, void* out, bool psi_ready) {
   RowJacArgs ra{};
   ra.U = c->Um; ra.V = c->Vm; ra.QX = c->QXm; ra.QY = c->QYm; ra.J = c->Jm; ra.DU = c->DUm; ra.DV = c->DVm;
   ra.drag = c->p.C_d != 0.0;
-  const R scale = (R)(1.0 / ((double)g.nx * (double)g.ny));
-  int rc = 0;
-  bool hit = with_size(g.N2, [&](auto nc) {
-    constexpr int N2 = decltype(nc)::value;
-    if constexpr (std::is_same<R, float>::value && N2 % 64 == 0 && N2 <= 1024) {
-      // FP32: register-fused row pass, 4 rows per workgroup (measured best)
-      constexpr int RW = 4, NT = RW * N2 / 8;
-      const size_t lds = (size_t)(row_lds_slots<R>(RW * N2) + ct_tw_size(N2) + N2 + 1) * sizeof(cplx<R>);
-      auto kern = k_row_jac_fused2<R, N2, RW>;
-      rc = allow_lds(kern, lds);
-      if (rc) return;
-      rc = timed(c, QGN_K_ROW_JAC, [&] {
-        hipLaunchKernelGGL(kern, dim3(g.ny * g.nz / RW), dim3(NT), lds, c->stream, ra, g, tw.sx, tw.r, scale);
-      });
-      return;
-    }
-    using Cfg = RowCfg<cplx<R>, N2, 4>;
-    constexpr int NT = Cfg::NTHR, RW = Cfg::ROWS;
-    const size_t lds = (size_t)(row_lds_slots<R>(RW * 4 * N2) + ct_tw_size(N2) + N2 + 1) * sizeof(cplx<R>);
-    auto kern = k_row_jac_ct<R, N2, RW, NT>;
-    rc = allow_lds(kern, lds);
-    if (rc) return;
-    const int ngroups = g.ny * g.nz / RW;
-    int per_cu = 1;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, NT, lds) != hipSuccess || per_cu < 1)
-      per_cu = 1;
-    const int grid = std::min(ngroups, per_cu * c->ncu);
-    rc = timed(c, QGN_K_ROW_JAC, [&] {
-      hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), lds, c->stream, ra, g, tw.sx, tw.r, scale, ngroups);
-    });
-  }, CtSizes{});
-  if (!hit) return fail(QGN_EINVAL, "nx/2 not supported by the RHS passes");
-  CHK(rc);
+  CHK(launch_row_jac<R>(c, ra));
   ColRhsArgs ka{};
   ka.J = c->Jm; ka.q = qin; ka.psi = psi; ka.out = out;
   ka.ubar = c->ubar; ka.qybar = c->qybar;
@@ -534,30 +639,7 @@ int rhs_impl(qgn_ctx* c, const double2* qin, void* out, bool psi_ready) {
   ka.ekman = c->ekm;
   ka.A2 = c->p.A2; ka.A8 = c->p.A8;
   ka.explicit_visc = c->p.is_explicit;
-  DragArgs da{};
-  da.DU = c->DUm; da.DV = c->DVm; da.out = out;
-  da.cdrag = c->cdm;
-  with_size(g.ny, [&](auto nyc) {
-    constexpr int NY = decltype(nyc)::value;
-    using Cfg = ColCfg<R, NY>;
-    auto kern = k_col_rhs_ct<R, R, NY, Cfg::C, Cfg::NTHR>;
-    rc = allow_lds(kern, Cfg::LDS);
-    if (rc) return;
-    rc = timed(c, QGN_K_COL_RHS, [&] {
-      hipLaunchKernelGGL(kern, dim3(g.Kxp / Cfg::C * g.nz), dim3(Cfg::NTHR), Cfg::LDS, c->stream, ka, g, tw.sy);
-    });
-    if (rc || !ra.drag) return;
-    constexpr int CD = Cfg::C / 2 > 0 ? Cfg::C / 2 : 1;
-    constexpr int NTD0 = 2 * CD * NY / 8;
-    constexpr int NTD = NTD0 < 64 ? 64 : ((NTD0 + 63) / 64) * 64;
-    constexpr size_t LDSD = (size_t)(2 * CD * NY + ct_tw_size(NY)) * sizeof(cplx<R>);
-    auto dk = k_col_drag_ct<R, R, NY, CD, NTD>;
-    rc = allow_lds(dk, LDSD);
-    if (rc) return;
-    hipLaunchKernelGGL(dk, dim3(g.Kxp / CD, c->nm), dim3(NTD), LDSD, c->stream, da, g, tw.sy);
-    rc = hipGetLastError() == hipSuccess ? 0 : fail(QGN_EHIP, "drag kernel launch failed");
-  }, CtSizes{});
-  CHK(rc);
+  CHK(launch_col_rhs<R>(c, ka, ra.drag));
   c->st.rhs_evals++;
   return 0;
 }
@@ -1154,6 +1236,78 @@ int upload(double** dst, const double* src, size_t n) {
   return 0;
 }
 
+// The autotuner of the FP32 RHS passes: on the context's own buffers, time
+// every compiled tile width of the column passes (2, 4, 8, 16 columns, those
+// that fit) and row group of the fused row pass (2, 4, 8 rows), 3 launches
+// each after one warm-up, and keep the fastest.  Kernel time does not depend
+// on the data.  QGN_AUTOTUNE=0 in the environment keeps the fixed rules.
+int autotune(qgn_ctx* c) {
+  c->tune_col_inv = c->tune_col_rhs = c->tune_row = 0;
+  if (c->p.precision != QGN_MIXED) return 0;
+  const char* env = std::getenv("QGN_AUTOTUNE");
+  if (env && env[0] == '0') return 0;
+  const Geom& g = c->g;
+  hipStream_t s = c->stream;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  auto time3 = [&](auto&& launch, float* ms) -> int {
+    CHK(launch());
+    HIPCHK(hipEventRecord(e0, s));
+    for (int i = 0; i < 3; ++i) CHK(launch());
+    HIPCHK(hipEventRecord(e1, s));
+    HIPCHK(hipEventSynchronize(e1));
+    HIPCHK(hipEventElapsedTime(ms, e0, e1));
+    *ms /= 3.f;
+    return 0;
+  };
+  ColInvArgs ca{};
+  ca.nf = 4;
+  for (int f = 0; f < 4; ++f) {   // as the RHS: u, v from psi (FP32); q_x, q_y from q (FP64)
+    ca.src[f] = f < 2 ? (const void*)c->psi : (const void*)c->q;
+    ca.src_is_double[f] = f < 2 ? 0 : 1;
+  }
+  ca.mult[0] = 1; ca.mult[1] = 2; ca.mult[2] = 2; ca.mult[3] = 3;
+  ca.dst[0] = c->Um; ca.dst[1] = c->Vm; ca.dst[2] = c->QXm; ca.dst[3] = c->QYm;
+  RowJacArgs ra{};
+  ra.U = c->Um; ra.V = c->Vm; ra.QX = c->QXm; ra.QY = c->QYm; ra.J = c->Jm; ra.DU = c->DUm; ra.DV = c->DVm;
+  ra.drag = 0;
+  ColRhsArgs ka{};
+  ka.J = c->Jm; ka.q = c->q; ka.psi = c->psi; ka.out = c->N[5];
+  ka.ubar = c->ubar; ka.qybar = c->qybar; ka.ekman = c->ekm;
+  const int tiles[4] = {2, 4, 8, 16}, rows[3] = {2, 4, 8};
+  int rc = 0;
+  float best[3] = {1e30f, 1e30f, 1e30f};
+  int pick[3] = {0, 0, 0};
+  for (int k = 0; k < 4 && !rc; ++k) {
+    if (!col_tile_ok(tiles[k], g.ny, sizeof(float2))) continue;
+    float ms = 0.f;
+    c->tune_col_inv = tiles[k];
+    rc = time3([&] { return launch_col_inv<float>(c, ca); }, &ms);
+    c->tune_ms[0][k] = ms * 1e3f;
+    if (!rc && ms < best[0]) { best[0] = ms; pick[0] = tiles[k]; }
+    c->tune_col_rhs = tiles[k];
+    rc = rc ? rc : time3([&] { return launch_col_rhs<float>(c, ka, false); }, &ms);
+    c->tune_ms[1][k] = ms * 1e3f;
+    if (!rc && ms < best[1]) { best[1] = ms; pick[1] = tiles[k]; }
+  }
+  const bool fused_row = g.N2 % 64 == 0 && g.N2 <= 1024;
+  for (int k = 0; k < 3 && !rc && fused_row; ++k) {
+    if (rows[k] * g.N2 / 8 > 1024) continue;
+    float ms = 0.f;
+    c->tune_row = rows[k];
+    rc = time3([&] { return launch_row_jac<float>(c, ra); }, &ms);
+    c->tune_ms[2][k] = ms * 1e3f;
+    if (!rc && ms < best[2]) { best[2] = ms; pick[2] = rows[k]; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  c->tune_col_inv = pick[0];
+  c->tune_col_rhs = pick[1];
+  c->tune_row = pick[2];
+  return rc;
+}
+
 // members > 1: a batch; H, EVals, uBar, qyBar hold nz values per member and
 // Modes nz*nz per member, member after member; r0 (may be null) one each.
 int create_impl(qgn_ctx* c, const qgn_params* p, int members, const double* H, const double* Modes,
@@ -1255,6 +1409,7 @@ int create_impl(qgn_ctx* c, const qgn_params* p, int members, const double* H, c
   HIPCHK(hipHostMalloc((void**)&c->h_rep, 8 * sizeof(double), hipHostMallocDefault));
   CHK(alloc((void**)&c->errpart, (size_t)g.ny * nz * sizeof(unsigned long long)));
   HIPCHK(hipHostMalloc((void**)&c->h_errslot, sizeof(unsigned long long), hipHostMallocDefault));
+  CHK(autotune(c));
   const size_t fbytes = (size_t)nz * g.ny * g.nxh * sizeof(double2);
   CHK(alloc((void**)&c->fa, fbytes));
   CHK(alloc((void**)&c->fb, fbytes));
@@ -1539,6 +1694,36 @@ int qgn_batch_get_time(qgn_ctx* c, double* t, double* dt, long long* steps, long
   for (int m = 0; m < nm; ++m) {
     if (st[m] == 1) return fail(QGN_ENONFINITE, "batch member " + std::to_string(m) + ": non-finite error norm");
     if (st[m] == 2) return fail(QGN_EREJECT, "batch member " + std::to_string(m) + ": too many rejected steps");
+  }
+  return 0;
+}
+
+int qgn_get_tuning(qgn_ctx* c, int* col_inv, int* col_rhs, int* row, float* times_us) {
+  if (!c) return fail(QGN_EINVAL, "null argument");
+  if (col_inv) *col_inv = c->tune_col_inv;
+  if (col_rhs) *col_rhs = c->tune_col_rhs;
+  if (row) *row = c->tune_row;
+  if (times_us) std::memcpy(times_us, c->tune_ms, sizeof(c->tune_ms));
+  return 0;
+}
+
+int qgn_set_tuning(qgn_ctx* c, int col_inv, int col_rhs, int row) {
+  if (!c) return fail(QGN_EINVAL, "null argument");
+  const Geom& g = c->g;
+  if (c->p.precision != QGN_MIXED) return fail(QGN_EINVAL, "the tuned passes are those of the mixed path");
+  if ((col_inv && !col_tile_ok(col_inv, g.ny, sizeof(float2))) || (col_rhs && !col_tile_ok(col_rhs, g.ny, sizeof(float2))))
+    return fail(QGN_EINVAL, "column tile must be 2, 4, 8 or 16 and fit 1024 threads and 160 KB of LDS");
+  if (row && (row != 2 && row != 4 && row != 8)) return fail(QGN_EINVAL, "row group must be 2, 4 or 8");
+  if (row && row * g.N2 / 8 > 1024) return fail(QGN_EINVAL, "row group does not fit 1024 threads");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->tune_col_inv = col_inv;
+  c->tune_col_rhs = col_rhs;
+  c->tune_row = row;
+  if (c->gexec) {   // the captured attempt has the old launches
+    (void)hipGraphExecDestroy(c->gexec);
+    (void)hipGraphDestroy(c->graph);
+    c->gexec = nullptr;
+    c->graph = nullptr;
   }
   return 0;
 }

@@ -182,6 +182,18 @@ class QGN:
         check(self.lib.qgn_spectra(self._ctx, int(nbins), dptr(ke), dptr(ens)))
         return ke, ens
 
+    def tuning(self):
+        """Launch choices of the mixed path (column inverse tile, column
+        forward tile, row group; 0 = fixed rule) and the measured times (us)."""
+        ci, cr, rw = C.c_int(), C.c_int(), C.c_int()
+        t = np.zeros((3, 4), dtype=np.float32)
+        check(self.lib.qgn_get_tuning(self._ctx, C.byref(ci), C.byref(cr), C.byref(rw),
+                                      t.ctypes.data_as(C.POINTER(C.c_float))))
+        return dict(col_inv=ci.value, col_rhs=cr.value, row=rw.value, times_us=t)
+
+    def set_tuning(self, col_inv=0, col_rhs=0, row=0):
+        check(self.lib.qgn_set_tuning(self._ctx, int(col_inv), int(col_rhs), int(row)))
+
     def set_closure(self, net, in_scale=None, out_scale=None):
         """Attach a closure network (qgn_fortran_amd.cnn.CNN) as subgrid PV
         forcing of every RHS evaluation; None detaches it."""

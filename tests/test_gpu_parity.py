@@ -200,3 +200,29 @@ def test_mixed_precision_rhs():
     print("mixed per-layer rel L2", err)
     assert np.all(err < 1e-5)
     gm.close()
+
+
+def test_autotuned_launches_and_every_variant():
+    """The autotuner's choices are valid tiles/groups with measured times, and
+    every compiled variant of the mixed path's column tiles and row groups
+    gives the same RHS (relative L2 <= 1e-5 against FP64, as the fixed rule)."""
+    case = make_case(256, 256, 8, k_peak=16, C_d=1e-5)
+    om = _oracle_model(case)
+    q_hat = om.initialize(case["q0"])
+    ref = om.get_rhs(q_hat)
+    case["pp"].precision = "mixed"
+    gm = _gpu_model(case)
+    t = gm.tuning()
+    print("tuning", t, flush=True)
+    assert t["col_inv"] in (2, 4, 8, 16) and t["col_rhs"] in (2, 4, 8, 16) and t["row"] in (2, 4, 8)
+    assert np.all(t["times_us"][0][t["times_us"][0] > 0] > 0)
+    base = gm.get_rhs(q_hat)
+    assert np.all(rel_l2(base, ref, axis=(1, 2)) < 1e-5)
+    for ci, cr, rw in [(2, 16, 2), (4, 8, 8), (16, 2, 4), (8, 4, 2)]:
+        gm.set_tuning(ci, cr, rw)
+        out = gm.get_rhs(q_hat)
+        err = rel_l2(out, ref, axis=(1, 2))
+        print(ci, cr, rw, "max rel L2", err.max(), flush=True)
+        assert np.all(err < 1e-5)
+        assert rel_l2(out, base) < 1e-6
+    gm.close()
