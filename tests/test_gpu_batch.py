@@ -155,3 +155,40 @@ def test_ensemble96():
     ens.close()
     for b in batches:
         b.close()
+
+
+def test_controller_error_paths():
+    """The device step control's failure reports: a NaN in the state gives
+    QGN_ENONFINITE (the reference would accept the step) and a dt that keeps
+    being rejected gives QGN_EREJECT after max_rejects attempts, for
+    qgn_time_step and, per member, for a batch (the other member goes on)."""
+    import dataclasses
+    from qgn_fortran_amd import Batch, configs
+    from qgn_fortran_amd._lib import QGN_ENONFINITE, QGN_EREJECT, QgnError
+    group = configs.ensemble96(sizes=(64,), per_size=2, nz=3, seed=4)[0]
+    p = dataclasses.replace(group.params, max_rejects=2, TOL=1e-14)
+    b = Batch(p, group.members)
+    q0 = b.initialize(group.q0)
+    bad = q0.copy()
+    bad[1, 3, 2] = np.nan                       # member 0, layer 1
+    one = Batch(p, group.members[:1])
+    one.set_state(bad[:3])
+    with pytest.raises(QgnError) as e:
+        one.time_step(0.0, 750.0)
+    assert e.value.code == QGN_ENONFINITE
+    one.set_state(q0[:3])
+    with pytest.raises(QgnError) as e:
+        one.time_step(0.0, 1.0e7)               # TOL 1e-14 and dt 1e7 s: every attempt is rejected
+    assert e.value.code == QGN_EREJECT
+    b.set_state(bad)
+    b.set_time(0.0, 750.0)
+    b.step(3)
+    with pytest.raises(QgnError) as e:
+        b.times()
+    assert e.value.code == QGN_ENONFINITE
+    st = np.empty(2, dtype=np.int32)
+    import ctypes as C
+    b.lib.qgn_batch_get_time(b._ctx, None, None, None, None, st.ctypes.data_as(C.POINTER(C.c_int)))
+    assert st[0] == 1 and st[1] == 0
+    one.close()
+    b.close()
