@@ -381,7 +381,7 @@ __global__ void __launch_bounds__(256) k_stage_proj(StageArgs a, Geom g, const d
 }
 
 // ---------------------------------------------------------------------------
-// Split-BF16 MFMA projection (mixed precision, nz <= 16).
+// Split-BF16 MFMA projection (mixed precision, nz <= 32).
 // psi = MD diag(1/(-k2 + EVals)) (HM / Htot) q as two 16x16x16 contractions
 // on v_mfma_f32_16x16x16bf16_1k.  Each FP32 operand x is split into
 // hi = bf16(x) and lo = bf16(x - hi); a product is accumulated in FP32 as
@@ -410,11 +410,13 @@ __device__ __forceinline__ qgn_float4 mfma3(const qgn_short4& ah, const qgn_shor
 }
 
 // Stage combination (or, with a.ns = 0 and a.out = nullptr, a plain copy of
-// q) followed by the MFMA projection.  Workgroup: 16 points x 16 layers, one
-// thread per (point, layer) for the FP64 stage sums; waves 0 and 1 then
-// project 8 points each (16 columns: re, im).
+// q) followed by the MFMA projection, nz <= 16.  Workgroup: 16 points x 16
+// layers, one thread per (point, layer) for the FP64 stage sums; waves 0
+// and 1 then project 8 points each (16 columns: re, im), building the split
+// matrix operands as they go (measured faster at 16 layers than holding
+// them across tiles, which needs twice the registers).
 template <typename S>
-__global__ void __launch_bounds__(256) k_stage_proj_mfma(StageArgs a, Geom g, const double* __restrict__ HM,
+__global__ void __launch_bounds__(256) k_stage_proj_mfma16(StageArgs a, Geom g, const double* __restrict__ HM,
                                                          const double* __restrict__ MD,
                                                          const double* __restrict__ evals, double inv_htot,
                                                          float2* __restrict__ psi) {
@@ -509,6 +511,156 @@ __global__ void __launch_bounds__(256) k_stage_proj_mfma(StageArgs a, Geom g, co
     for (int r = 0; r < 4; ++r) {
       const int layer = kb + r;
       if (layer < nz) out[2 * (moff + layer * plane + pt) + (row & 1)] = p[r];
+    }
+  }
+}
+
+// Stage combination (or, with a.ns = 0 and a.out = nullptr, a plain copy of
+// q) followed by the MFMA projection, for nz <= NZT layers (NZT = 16 or 32:
+// the nz x nz matrices as (NZT/16)^2 tiles of 16 x 16).  A workgroup walks
+// tiles of 16 points x NZT layers (grid-stride); per tile each thread forms
+// the FP64 stage sums of NZT/16 (point, layer) elements, then waves 0 and 1
+// project 8 points each (16 columns: re, im).  The split-BF16 matrix
+// operands are built once per workgroup and stay in registers while the
+// workgroup walks its tiles (the 32-layer matrices are 4x those of 16).
+// The tile buffer alternates between two halves, so one barrier per tile
+// separates its writers from its readers (each half is rewritten only after
+// the next tile's barrier, by which time its readers are done).
+template <typename S, int NZT>
+__global__ void __launch_bounds__(256) k_stage_proj_mfma(StageArgs a, Geom g, const double* __restrict__ HM,
+                                                         const double* __restrict__ MD,
+                                                         const double* __restrict__ evals, double inv_htot,
+                                                         float2* __restrict__ psi) {
+  constexpr int NT = NZT / 16;          // 16-row tiles of modes and of layers
+  __shared__ double2 sq[2][NZT][16];    // [half][layer][point]
+  const int nz = g.nzm;
+  const int mb = blockIdx.y;            // member
+  HM += mb * nz * nz;
+  MD += mb * nz * nz;
+  evals += mb * nz;
+  const long moff = (long)mb * nz * g.Ky * g.Kxp;   // first element of the member
+  const double dt = a.dtv ? a.dtv[mb] : a.dt;
+  const int tid = threadIdx.x;
+  const long plane = (long)g.Ky * g.Kxp;
+  const long ntiles = (plane + 15) / 16;
+  const int l = tid & 63;
+  const int wv = tid >> 6;                     // waves 0, 1: points 8*wv .. 8*wv+7 of a tile
+  const int row = l & 15, kb = 4 * (l >> 4);   // A: i = row; B, D: j = row
+  qgn_short4 a1h[NT][NT], a1l[NT][NT], a2h[NT][NT], a2l[NT][NT];
+  double ev[NT][4];
+  auto build_ops = [&]() {
+#pragma unroll
+    for (int ti = 0; ti < NT; ++ti)
+#pragma unroll
+      for (int tk = 0; tk < NT; ++tk)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = 16 * ti + row, k = 16 * tk + kb + r;
+          const bool in = i < nz && k < nz;
+          const float m1 = in ? (float)(HM[i * nz + k] * inv_htot) : 0.f;   // (HM/Htot)[mode i][layer k]
+          const float m2 = in ? (float)MD[i * nz + k] : 0.f;                // Modes[layer i][mode k]
+          a1h[ti][tk][r] = bf16_rne(m1);
+          a1l[ti][tk][r] = bf16_lo(m1, a1h[ti][tk][r]);
+          a2h[ti][tk][r] = bf16_rne(m2);
+          a2l[ti][tk][r] = bf16_lo(m2, a2h[ti][tk][r]);
+        }
+#pragma unroll
+    for (int mi = 0; mi < NT; ++mi)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) ev[mi][r] = 16 * mi + kb + r < nz ? evals[16 * mi + kb + r] : 1.0;
+  };
+  if (tid < 128) build_ops();
+  int half = 0;
+  for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x, half ^= 1) {
+#pragma unroll
+    for (int h = 0; h < NT; ++h) {
+      const int pp = tid & 15, k = (tid >> 4) + 16 * h;
+      const long pt = tile * 16 + pp;
+      double2 qt = make_double2(0.0, 0.0);
+      if (k < nz && pt < plane) {
+        const long e = moff + k * plane + pt;
+        const double2 q = a.q[e];
+        double sx = 0.0, sy = 0.0;
+        for (int jj = 0; jj < a.ns; ++jj) {
+          const cplx<S> v = reinterpret_cast<const cplx<S>*>(a.N[jj])[e];
+          sx += a.ae[jj] * (double)v.x;
+          sy += a.ae[jj] * (double)v.y;
+        }
+        double mq = 1.0;
+        if (a.implicit_visc) {
+          const int i = (int)(pt % g.Kxp);
+          const int j = compact_row_to_full((int)(pt / g.Kxp), g);
+          const double kx = kx_of(i, g), ky = ky_of(j, g);
+          const double k2 = kx * kx + ky * ky;
+          const double k4 = k2 * k2;
+          const double a8 = a.a8l ? a.a8l[mb * nz + k] : a.A8;
+          const double lam = a.A2 * k2 + a8 * (k4 * k4);
+          for (int jj = 0; jj < a.ns; ++jj) {
+            const double2 y = jj == 0 ? q : a.Y[jj][e];
+            sx += a.ai[jj] * (-lam * y.x);
+            sy += a.ai[jj] * (-lam * y.y);
+          }
+          mq = 1.0 / (1.0 + 0.25 * dt * lam);
+        }
+        qt = make_double2(mq * (q.x + dt * sx), mq * (q.y + dt * sy));
+        if (a.out) a.out[e] = qt;
+      }
+      sq[half][k][pp] = qt;
+    }
+    __syncthreads();
+    if (tid >= 128) continue;
+    // first product: modal amplitudes D1[mode][column] = (HM/Htot)[mode][layer] Q[layer][column]
+    qgn_float4 d[NT];
+#pragma unroll
+    for (int mi = 0; mi < NT; ++mi) d[mi] = qgn_float4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kt = 0; kt < NT; ++kt) {
+      qgn_short4 bh, bl;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const double2 v = sq[half][16 * kt + kb + r][8 * wv + (row >> 1)];
+        const float x = (float)((row & 1) ? v.y : v.x);
+        bh[r] = bf16_rne(x);
+        bl[r] = bf16_lo(x, bh[r]);
+      }
+#pragma unroll
+      for (int mi = 0; mi < NT; ++mi) d[mi] = mfma3(a1h[mi][kt], a1l[mi][kt], bh, bl, d[mi]);
+    }
+    const long pt = tile * 16 + 8 * wv + (row >> 1);
+    bool zero_mode = pt >= plane;
+    double k2 = 0.0;
+    if (!zero_mode) {
+      const int i = (int)(pt % g.Kxp);
+      const int j = compact_row_to_full((int)(pt / g.Kxp), g);
+      const double kx = kx_of(i, g), ky = ky_of(j, g);
+      k2 = kx * kx + ky * ky;
+      zero_mode = (i == 0 && j == 0) || i >= g.Kx;
+    }
+    // modal Helmholtz inversion; D1 as it stands is the B operand of the second product
+    qgn_short4 ph[NT], pl[NT];
+#pragma unroll
+    for (int mi = 0; mi < NT; ++mi)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int mode = 16 * mi + kb + r;
+        const float pm = (zero_mode || mode >= nz) ? 0.f : (float)((double)d[mi][r] / (-k2 + ev[mi][r]));
+        ph[mi][r] = bf16_rne(pm);
+        pl[mi][r] = bf16_lo(pm, ph[mi][r]);
+      }
+    // second product: psi[layer][column] = Modes[layer][mode] pm[mode][column]
+#pragma unroll
+    for (int li = 0; li < NT; ++li) {
+      qgn_float4 p = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int mt = 0; mt < NT; ++mt) p = mfma3(a2h[li][mt], a2l[li][mt], ph[mt], pl[mt], p);
+      if (pt < plane) {
+        float* out = reinterpret_cast<float*>(psi);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int layer = 16 * li + kb + r;
+          if (layer < nz) out[2 * (moff + layer * plane + pt) + (row & 1)] = p[r];
+        }
+      }
     }
   }
 }

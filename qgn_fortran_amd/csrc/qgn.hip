@@ -494,16 +494,25 @@ int launch_col_inv(qgn_ctx* c, const ColInvArgs& a) {
   return rc;
 }
 
-// Mixed precision with nz <= 16: the stage + split-BF16 MFMA projection.
+// Mixed precision with nz <= 32: the stage + split-BF16 MFMA projection.
 int launch_stage_proj_mfma(qgn_ctx* c, const StageArgs& a, int kernel_id) {
   const long plane = (long)c->g.Ky * c->g.Kxp;
-  const int blocks = (int)((plane + 15) / 16);
+  const long tiles = (plane + 15) / 16;
+  // nz <= 16: one tile per workgroup; nz <= 32: workgroups walk tiles, 8 per
+  // CU over all members, so the 4x larger split matrices are built once per
+  // workgroup
+  const long want = c->g.nzm <= 16 ? tiles : std::max(1L, (8L * c->ncu + c->nm - 1) / c->nm);
+  const int blocks = (int)std::min(tiles, want);
   return timed(c, kernel_id, [&] {
-    hipLaunchKernelGGL((k_stage_proj_mfma<float>), dim3(blocks, c->nm), dim3(256), 0, c->stream, a, c->g, c->HM, c->MD,
-                       c->evals, 1.0 / c->p.Htot, reinterpret_cast<float2*>(c->psi));
+    if (c->g.nzm <= 16)
+      hipLaunchKernelGGL((k_stage_proj_mfma16<float>), dim3(blocks, c->nm), dim3(256), 0, c->stream, a, c->g, c->HM,
+                         c->MD, c->evals, 1.0 / c->p.Htot, reinterpret_cast<float2*>(c->psi));
+    else
+      hipLaunchKernelGGL((k_stage_proj_mfma<float, 32>), dim3(blocks, c->nm), dim3(256), 0, c->stream, a, c->g, c->HM,
+                         c->MD, c->evals, 1.0 / c->p.Htot, reinterpret_cast<float2*>(c->psi));
   });
 }
-bool use_mfma_projection(const qgn_ctx* c) { return c->p.precision == QGN_MIXED && c->g.nzm <= 16; }
+bool use_mfma_projection(const qgn_ctx* c) { return c->p.precision == QGN_MIXED && c->g.nzm <= 32; }
 
 // P2: the Jacobian row pass.  FP32 with 64 | nx/2 uses the register-fused
 // kernel with c->tune_row rows per workgroup (default 4); otherwise the

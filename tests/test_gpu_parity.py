@@ -187,9 +187,12 @@ def test_stepper_trajectory(scheme, kw, dt0, nsteps):
     gm.close()
 
 
-def test_mixed_precision_rhs():
-    """FP32 FFT/Jacobian path: RHS relative L2 vs the FP64 RHS <= 1e-5."""
-    case = make_case(256, 256, 8, k_peak=16, C_d=1e-5)
+@pytest.mark.parametrize("n,nz", [(256, 8), (128, 24), (128, 32)])
+def test_mixed_precision_rhs(n, nz):
+    """FP32 FFT/Jacobian path with the split-BF16 MFMA projection (16-layer
+    tiles; nz = 24 leaves a partial tile): RHS relative L2 vs the FP64 RHS
+    <= 1e-5."""
+    case = make_case(n, n, nz, k_peak=16, C_d=1e-5)
     om = _oracle_model(case)
     q_hat = om.initialize(case["q0"])
     ref = om.get_rhs(q_hat)
