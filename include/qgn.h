@@ -94,7 +94,8 @@ int qgn_batch_step(qgn_ctx* ctx, int attempts);
 int qgn_batch_get_time(qgn_ctx* ctx, double* t, double* dt, long long* steps, long long* rejects, int* status);
 
 /* Several batches (e.g. one per grid size) advanced by one graph launch per
- * attempt, the batches' attempt graphs as independent children.  Each
+ * attempt, the batches' attempt graphs as independent children that may run
+ * concurrently (so no two batches may share a closure network).  Each
  * context needs qgn_batch_set_time first; qgn_ensemble_step is ordered
  * after the work already queued on every context and before what is
  * queued on them afterwards, and returns without waiting. */

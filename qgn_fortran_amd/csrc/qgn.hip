@@ -1959,6 +1959,12 @@ static int ensemble_create_impl(qgn_ens* e, qgn_ctx* const* ctxs, int n) {
   HIPCHK(hipGraphCreate(&e->graph, 0));
   for (int i = 0; i < n; ++i) {
     if (!ctxs[i]) return fail(QGN_EINVAL, "null context");
+    for (int k = 0; k < i; ++k) {
+      if (ctxs[k] == ctxs[i]) return fail(QGN_EINVAL, "a context appears twice in the ensemble");
+      // the batches' graphs run concurrently; one network's activation buffers cannot serve two
+      if (ctxs[i]->net && ctxs[k]->net == ctxs[i]->net)
+        return fail(QGN_EINVAL, "two batches of an ensemble share one closure network");
+    }
     CHK(batch_check(ctxs[i]));
     e->ctx.push_back(ctxs[i]);
     hipGraphNode_t node;

@@ -192,3 +192,50 @@ def test_controller_error_paths():
     assert st[0] == 1 and st[1] == 0
     one.close()
     b.close()
+
+
+def test_closure_in_batch_graph_matches_single_runs():
+    """The closure network inside the graph-captured batch attempt: each member
+    of a 2-member batch steps exactly as the same model alone with the same
+    network (qgn_time_step); an ensemble whose batches share one network is
+    refused."""
+    from oracle import cnn_oracle as co
+    from qgn_fortran_amd import Batch, Ensemble, configs
+    from qgn_fortran_amd._lib import QGN_EINVAL, QgnError
+    from qgn_fortran_amd.cnn import CNN
+    group = configs.ensemble96(sizes=(64,), per_size=2, nz=8, seed=7, scheme="ARK43")[0]
+    group.q0 *= 10.0
+    net = CNN(16, 8, co.init_params(16, 8, seed=2, last_scale=0.05))
+    s_in = np.full(16, 1.0)
+    s_in[:8] = 1.0 / np.std(group.q0)
+    s_in[8:] = 1.0 / 3.0e3
+    s_out = np.full(8, 1e-12)
+    b = Batch(group.params, group.members)
+    q0 = b.initialize(group.q0)
+    b.set_closure(net, s_in, s_out)
+    b.set_time(0.0, 2000.0)
+    b.step(4)
+    info = b.times()
+    state = b.get_state()
+    for m in range(2):
+        single = Batch(group.params, group.members[m:m + 1])
+        single.set_closure(net, s_in, s_out)
+        single.set_state(q0[8 * m:8 * (m + 1)])
+        t, dt = 0.0, 2000.0
+        for _ in range(int(info["steps"][m])):
+            t, dt = single.time_step(t, dt)
+        err = rel_l2(state[8 * m:8 * (m + 1)], single.get_state())
+        print(f"member {m}: {int(info['steps'][m])} steps, state rel L2 {err:.2e}", flush=True)
+        assert err == 0.0 and t == info["t"][m]
+        single.set_closure(None)
+        single.close()
+    other = Batch(group.params, group.members)
+    other.set_closure(net, s_in, s_out)
+    other.set_time(0.0, 2000.0)
+    with pytest.raises(QgnError) as e:
+        Ensemble([b, other])
+    assert e.value.code == QGN_EINVAL
+    for x in (b, other):
+        x.set_closure(None)
+        x.close()
+    net.close()
