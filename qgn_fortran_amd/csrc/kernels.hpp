@@ -172,6 +172,8 @@ struct RowJacArgs {
 };
 
 // Column pass P3: y-forward FFT of J and the rest of GetRHS.
+// With drag set, the bottom layer of each member also gets the quadratic
+// drag cdrag (i ky du_hat - i kx dv_hat) of the row pass's |u| u, |u| v.
 struct ColRhsArgs {
   const void* J;         // mixed cplx<R>
   const double2* q;      // compact
@@ -179,15 +181,11 @@ struct ColRhsArgs {
   void* out;             // compact cplx<S>
   const double* ubar; const double* qybar;   // per layer
   const double* ekman;   // per member: r0 Htot / H_nz, applied on its bottom layer
+  const void* DU; const void* DV;   // (members, ny, Kxp) cplx<R>
+  const double* cdrag;   // per member: C_d Htot / H_nz
   double beta, A2, A8;
   int explicit_visc;
-};
-
-// Bottom-layer quadratic drag: out(nz) += cdrag (i ky du_hat - i kx dv_hat).
-struct DragArgs {
-  const void* DU; const void* DV;   // (members, ny, Kxp) cplx<R>
-  void* out;                        // compact cplx<S>, each member's bottom layer updated in place
-  const double* cdrag;              // per member: C_d Htot / H_nz
+  int drag;
 };
 
 // ---------------------------------------------------------------------------

@@ -304,22 +304,58 @@ __global__ void __launch_bounds__(ROWS * (N2 / 8)) k_row_jac_fused2(RowJacArgs a
   forward(acc, reinterpret_cast<Cx*>(a.J) + (long)row0 * Kxp);
 }
 
+// Tile of P3 block b of total = ntile * nz: (layer z, column tile t).  With
+// drag, bottom-layer tiles do three column FFTs instead of one; each XCD's
+// range of xcd_tile positions then starts with an equal share of them
+// (heavy tiles first, then the other layers in order), so no XCD ends with
+// a run of heavy tiles.  Without drag, or when the shares do not fit, tiles
+// follow xcd_tile in layer order.
+__device__ __forceinline__ void rhs_tile(int b, int total, int ntile, int nzm, int drag, int* z, int* t) {
+  const int w = xcd_tile(b, total);
+  const int per = total >> 3;
+  const int heavy = total / nzm;          // one bottom layer per member
+  if (!drag || nzm < 2 || (heavy + 7) / 8 > per) {
+    *z = w / ntile; *t = w % ntile;
+    return;
+  }
+  int h = -1, l;
+  if (w < (per << 3)) {
+    const int x = w / per, p = w - x * per;
+    const int hs = x * heavy / 8, hx = (x + 1) * heavy / 8 - hs;
+    if (p < hx) h = hs + p;
+    else l = x * per - hs + (p - hx);
+  } else {
+    l = w - heavy;
+  }
+  if (h >= 0) {
+    *z = (h / ntile) * nzm + nzm - 1; *t = h % ntile;
+  } else {
+    const int lz = l / ntile;             // over the members' upper layers
+    *z = (lz / (nzm - 1)) * nzm + lz % (nzm - 1); *t = l % ntile;
+  }
+}
+
 // P3: y-forward FFT of J, then
 //   RHS = -J_hat - Ubar_k i kx q - (beta + Qy_k) i kx psi
 //         [- (A2 k2 + A8 k8) q  if is_explicit]  [+ r0 Htot/H_nz k2 psi on layer nz]
+//         [+ C_d Htot/H_nz (i ky du - i kx dv) on layer nz, QGN_Module.f90:284-292]
 // on the kept modes (DeAlias, QGN_Module.f90:294), (0,0) and padding zero.
+// A bottom-layer tile with drag keeps its values in registers (each thread
+// owns at most C Ky / NTHR <= 8 Ky / NY < 6 of them) while |u| u and |u| v
+// go through the same LDS buffer.
 template <typename R, typename S, int NY, int C, int NTHR>
 __global__ void __launch_bounds__(NTHR) k_col_rhs_ct(ColRhsArgs a, Geom g, const cplx<R>* __restrict__ twy) {
   using Cx = cplx<R>;
+  constexpr int MAXO = 6;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   Cx* buf = reinterpret_cast<Cx*>(lds_raw);
   Cx* tw = buf + C * NY;
   using L = CtLayout<C, 1, C, true>;
   constexpr int TOT = C * NY;
   const int ntile = g.Kxp / C;
-  const int w = xcd_tile(blockIdx.x, gridDim.x);   // grid = ntile * nz
-  const int i0 = (w % ntile) * C;
-  const int z = w / ntile;
+  int z, i0;
+  rhs_tile(blockIdx.x, gridDim.x, ntile, g.nzm, a.drag, &z, &i0);   // grid = ntile * nz
+  i0 *= C;
   load_table<R>(tw, twy, ct_tw_size(NY));
   const Cx* src = reinterpret_cast<const Cx*>(a.J);
   for (int e = threadIdx.x; e < TOT; e += NTHR) {
@@ -335,7 +371,7 @@ __global__ void __launch_bounds__(NTHR) k_col_rhs_ct(ColRhsArgs a, Geom g, const
   const bool bottom = (z % g.nzm == g.nzm - 1);
   const R ek = bottom ? (R)a.ekman[z / g.nzm] : (R)0;
   const int nown = C * g.Ky;
-  for (int o = threadIdx.x; o < nown; o += NTHR) {
+  auto rhs_at = [&](int o) {
     const int c = o % C, jc = o / C, i = i0 + c;
     const int j = compact_row_to_full(jc, g);
     const long idx = ((long)z * g.Ky + jc) * g.Kxp + i;
@@ -358,53 +394,62 @@ __global__ void __launch_bounds__(NTHR) k_col_rhs_ct(ColRhsArgs a, Geom g, const
       r.x += ek * k2 * ps.x;
       r.y += ek * k2 * ps.y;
     }
-    if ((i == 0 && j == 0) || i >= g.Kx) r = cmk<Cx>((R)0, (R)0);
-    out[idx] = cmk<cplx<S>>((S)r.x, (S)r.y);
+    return r;
+  };
+  if (!(bottom && a.drag)) {
+    for (int o = threadIdx.x; o < nown; o += NTHR) {
+      const int c = o % C, jc = o / C, i = i0 + c;
+      const int j = compact_row_to_full(jc, g);
+      Cx r = rhs_at(o);
+      if ((i == 0 && j == 0) || i >= g.Kx) r = cmk<Cx>((R)0, (R)0);
+      out[((long)z * g.Ky + jc) * g.Kxp + i] = cmk<cplx<S>>((S)r.x, (S)r.y);
+    }
+    return;
   }
-}
-
-// Bottom-layer quadratic drag (QGN_Module.f90:284-292): y-forward FFT of
-// |u| u and |u| v, then out(nz) += C_d Htot/H_nz (i ky du - i kx dv).
-template <typename R, typename S, int NY, int C, int NTHR>
-__global__ void __launch_bounds__(NTHR) k_col_drag_ct(DragArgs a, Geom g, const cplx<R>* __restrict__ twy) {
-  using Cx = cplx<R>;
-  extern __shared__ __align__(16) unsigned char lds_raw[];
-  Cx* bu = reinterpret_cast<Cx*>(lds_raw);
-  Cx* bv = bu + C * NY;
-  Cx* tw = bv + C * NY;
-  using L = CtLayout<C, 1, C, true>;
-  constexpr int TOT = C * NY;
-  const int i0 = xcd_tile(blockIdx.x, gridDim.x) * C;   // grid = (Kxp / C, members)
-  const int mb = blockIdx.y;
-  const int z = mb * g.nzm + g.nzm - 1;                  // bottom layer of member mb
-  load_table<R>(tw, twy, ct_tw_size(NY));
-  const Cx* du = reinterpret_cast<const Cx*>(a.DU) + (long)mb * NY * g.Kxp;
-  const Cx* dv = reinterpret_cast<const Cx*>(a.DV) + (long)mb * NY * g.Kxp;
-  for (int e = threadIdx.x; e < TOT; e += NTHR) {
-    const int c = e % C, j = e / C;
-    bu[e] = du[(long)j * g.Kxp + i0 + c];
-    bv[e] = dv[(long)j * g.Kxp + i0 + c];
+  Cx keep[MAXO];
+#pragma unroll
+  for (int u = 0; u < MAXO; ++u) {
+    const int o = threadIdx.x + u * NTHR;
+    if (o < nown) keep[u] = rhs_at(o);
   }
-  __syncthreads();
-  ct_fft<R, NY, L, NTHR>(bu, tw, -1);
-  ct_fft<R, NY, L, NTHR>(bv, tw, -1);
-  cplx<S>* out = reinterpret_cast<cplx<S>*>(a.out);
+  const int mb = z / g.nzm;
   const R cd = (R)a.cdrag[mb];
-  const int nown = C * g.Ky;
-  for (int o = threadIdx.x; o < nown; o += NTHR) {
+#pragma unroll
+  for (int pass = 0; pass < 2; ++pass) {   // |u| u, then |u| v
+    const Cx* d = reinterpret_cast<const Cx*>(pass ? a.DV : a.DU) + (long)mb * NY * g.Kxp;
+    __syncthreads();   // the previous contents of buf are read
+    for (int e = threadIdx.x; e < TOT; e += NTHR) {
+      const int c = e % C, j = e / C;
+      buf[e] = d[(long)j * g.Kxp + i0 + c];
+    }
+    __syncthreads();
+    ct_fft<R, NY, L, NTHR>(buf, tw, -1);
+#pragma unroll
+    for (int u = 0; u < MAXO; ++u) {
+      const int o = threadIdx.x + u * NTHR;
+      if (o >= nown) continue;
+      const int c = o % C, jc = o / C, i = i0 + c;
+      const int j = compact_row_to_full(jc, g);
+      const Cx x = buf[j * C + c];
+      const R kx = (R)kx_of(i, g), ky = (R)ky_of(j, g);
+      if (pass == 0) {   // + cd i ky du
+        keep[u].x -= cd * ky * x.y;
+        keep[u].y += cd * ky * x.x;
+      } else {           // - cd i kx dv
+        keep[u].x += cd * kx * x.y;
+        keep[u].y -= cd * kx * x.x;
+      }
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < MAXO; ++u) {
+    const int o = threadIdx.x + u * NTHR;
+    if (o >= nown) continue;
     const int c = o % C, jc = o / C, i = i0 + c;
-    if (i >= g.Kx) continue;
     const int j = compact_row_to_full(jc, g);
-    if (i == 0 && j == 0) continue;
-    const long idx = ((long)z * g.Ky + jc) * g.Kxp + i;
-    const Cx x = bu[j * C + c], y = bv[j * C + c];
-    const R kx = (R)kx_of(i, g), ky = (R)ky_of(j, g);
-    // i ky x - i kx y = i (ky x - kx y)
-    const R px = ky * x.x - kx * y.x, py = ky * x.y - kx * y.y;
-    cplx<S> r = out[idx];
-    r.x += (S)(-cd * py);
-    r.y += (S)(cd * px);
-    out[idx] = r;
+    Cx r = keep[u];
+    if ((i == 0 && j == 0) || i >= g.Kx) r = cmk<Cx>((R)0, (R)0);
+    out[((long)z * g.Ky + jc) * g.Kxp + i] = cmk<cplx<S>>((S)r.x, (S)r.y);
   }
 }
 

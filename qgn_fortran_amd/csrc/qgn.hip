@@ -567,15 +567,18 @@ int launch_row_jac(qgn_ctx* c, const RowJacArgs& ra) {
   return rc;
 }
 
-// P3 (FP32: c->tune_col_rhs columns per tile, default the fixed rule) and
-// the bottom-layer drag pass.
+// P3 with the bottom-layer drag (FP32: c->tune_col_rhs columns per tile,
+// default the fixed rule).
 template <typename R>
-int launch_col_rhs(qgn_ctx* c, const ColRhsArgs& ka, bool drag) {
+int launch_col_rhs(qgn_ctx* c, const ColRhsArgs& ka0, bool drag) {
   const Geom& g = c->g;
   const Tw<R>& tw = tw_of<R>(c);
-  DragArgs da{};
-  da.DU = c->DUm; da.DV = c->DVm; da.out = ka.out;
-  da.cdrag = c->cdm;
+  ColRhsArgs ka = ka0;
+  ka.DU = c->DUm; ka.DV = c->DVm;
+  ka.cdrag = c->cdm;
+  ka.drag = drag;
+  // each thread of a drag tile holds at most C Ky / NTHR values, NTHR >= C ny / 8
+  if (drag && 8L * g.Ky > 6L * g.ny) return fail(QGN_EINVAL, "drag tile: too many kept rows");
   int rc = 0;
   with_size(g.ny, [&](auto nyc) {
     constexpr int NY = decltype(nyc)::value;
@@ -599,16 +602,6 @@ int launch_col_rhs(qgn_ctx* c, const ColRhsArgs& ka, bool drag) {
     } else {
       go(std::integral_constant<int, Cfg::C>{});
     }
-    if (rc || !drag) return;
-    constexpr int CD = Cfg::C / 2 > 0 ? Cfg::C / 2 : 1;
-    constexpr int NTD0 = 2 * CD * NY / 8;
-    constexpr int NTD = NTD0 < 64 ? 64 : ((NTD0 + 63) / 64) * 64;
-    constexpr size_t LDSD = (size_t)(2 * CD * NY + ct_tw_size(NY)) * sizeof(cplx<R>);
-    auto dk = k_col_drag_ct<R, R, NY, CD, NTD>;
-    rc = allow_lds(dk, LDSD);
-    if (rc) return;
-    hipLaunchKernelGGL(dk, dim3(g.Kxp / CD, c->nm), dim3(NTD), LDSD, c->stream, da, g, tw.sy);
-    rc = hipGetLastError() == hipSuccess ? 0 : fail(QGN_EHIP, "drag kernel launch failed");
   }, CtSizes{});
   return rc;
 }
@@ -1280,7 +1273,7 @@ int autotune(qgn_ctx* c) {
   ca.dst[0] = c->Um; ca.dst[1] = c->Vm; ca.dst[2] = c->QXm; ca.dst[3] = c->QYm;
   RowJacArgs ra{};
   ra.U = c->Um; ra.V = c->Vm; ra.QX = c->QXm; ra.QY = c->QYm; ra.J = c->Jm; ra.DU = c->DUm; ra.DV = c->DVm;
-  ra.drag = 0;
+  ra.drag = c->p.C_d != 0.0;   // the bottom-layer drag rides on the row and P3 passes
   ColRhsArgs ka{};
   ka.J = c->Jm; ka.q = c->q; ka.psi = c->psi; ka.out = c->N[5];
   ka.ubar = c->ubar; ka.qybar = c->qybar; ka.ekman = c->ekm;
@@ -1296,7 +1289,7 @@ int autotune(qgn_ctx* c) {
     c->tune_ms[0][k] = ms * 1e3f;
     if (!rc && ms < best[0]) { best[0] = ms; pick[0] = tiles[k]; }
     c->tune_col_rhs = tiles[k];
-    rc = rc ? rc : time3([&] { return launch_col_rhs<float>(c, ka, false); }, &ms);
+    rc = rc ? rc : time3([&] { return launch_col_rhs<float>(c, ka, ra.drag); }, &ms);
     c->tune_ms[1][k] = ms * 1e3f;
     if (!rc && ms < best[1]) { best[1] = ms; pick[1] = tiles[k]; }
   }
