@@ -30,6 +30,13 @@ template <typename C> __device__ __forceinline__ C cmul(C a, C b) {
   return cmk<C>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 template <typename C> __device__ __forceinline__ C cconj(C a) { return cmk<C>(a.x, -a.y); }
+// p[k] if ok, else zero.  The load is unconditional, from index 0 when !ok
+// (p must point at one valid element), so it stays one vector load with a
+// select after it instead of a branch around each component.
+template <typename C> __device__ __forceinline__ C ld_or_zero(const C* p, int k, bool ok) {
+  const C v = p[ok ? k : 0];
+  return ok ? v : cmk<C>(0, 0);
+}
 template <typename C> __device__ __forceinline__ C cscale(C a, decltype(C::x) s) { return cmk<C>(a.x * s, a.y * s); }
 // multiply by sign*i  (sign = +1 or -1)
 template <typename C> __device__ __forceinline__ C cmul_si(C a, int sign) {

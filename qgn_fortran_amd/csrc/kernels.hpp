@@ -212,9 +212,9 @@ template <typename R>
 __device__ __forceinline__ cplx<R> c2r_pre(const cplx<R>* __restrict__ row, int Kin, int k, int N2,
                                            const cplx<R>* __restrict__ twr, R scale) {
   using Cx = cplx<R>;
-  Cx a = k < Kin ? row[k] : cmk<Cx>((R)0, (R)0);
+  Cx a = ld_or_zero(row, k, k < Kin);
   const int kk = N2 - k;
-  Cx b = kk < Kin ? row[kk] : cmk<Cx>((R)0, (R)0);
+  Cx b = ld_or_zero(row, kk, kk < Kin);
   if (k == 0) { a.y = (R)0; b.y = (R)0; }
   const Cx s = cmk<Cx>(a.x + b.x, a.y - b.y);   // X[k] + conj X[N2-k]
   const Cx d = cmk<Cx>(a.x - b.x, a.y + b.y);   // X[k] - conj X[N2-k]

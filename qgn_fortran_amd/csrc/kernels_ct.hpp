@@ -132,9 +132,9 @@ __global__ void __launch_bounds__(NTHR) k_row_jac_ct(RowJacArgs a, Geom g, const
       const bool ok = (ROWS * N2) % NTHR == 0 || e < ROWS * N2;
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
-        const Cx* p = srcs[s] + (row0 + r) * Kxp;
-        ra[s][b] = (ok && k < Kx) ? p[k] : zero;
-        rb[s][b] = (ok && N2 - k < Kx) ? p[N2 - k] : zero;
+        const Cx* p = srcs[s] + (row0 + (ok ? r : 0)) * Kxp;
+        ra[s][b] = ld_or_zero(p, k, ok && k < Kx);
+        rb[s][b] = ld_or_zero(p, N2 - k, ok && N2 - k < Kx);
       }
     }
   };
@@ -223,7 +223,8 @@ __global__ void __launch_bounds__(ROWS * (N2 / 8)) k_row_jac_fused2(RowJacArgs a
   load_table<R>(tr, twr, N2 + 1);
   const int r = threadIdx.x / NB;
   const int t = threadIdx.x - r * NB;
-  const int row0 = blockIdx.x * ROWS;
+  // last rows first: the bottom layer's heavier groups (drag) start early
+  const int row0 = (gridDim.x - 1 - blockIdx.x) * ROWS;
   const int z = row0 / g.ny;
   const int j0 = row0 - z * g.ny;
   const bool bottom = a.drag && (z % g.nzm == g.nzm - 1);
@@ -238,8 +239,8 @@ __global__ void __launch_bounds__(ROWS * (N2 / 8)) k_row_jac_fused2(RowJacArgs a
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       const int k = t + j * NB;
-      const Cx x = k < Kx ? p[k] : zero;
-      const Cx y = (N2 - k) < Kx ? p[N2 - k] : zero;
+      const Cx x = ld_or_zero(p, k, k < Kx);
+      const Cx y = ld_or_zero(p, N2 - k, N2 - k < Kx);
       v[j] = c2r_pre_ab<R>(x, y, k, tr, scale);
     }
     Dft<Cx, 8>::run(v, +1);
