@@ -30,6 +30,28 @@ template <typename C> __device__ __forceinline__ C cmul(C a, C b) {
   return cmk<C>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 template <typename C> __device__ __forceinline__ C cconj(C a) { return cmk<C>(a.x, -a.y); }
+// Raw buffer resource over the nbytes at base (base wave-uniform; dword 3
+// is the gfx9-family default format word).  A load whose byte offset is at
+// or past nbytes returns zero, so masked loads need no branch: pass
+// kRowOob as the offset of a lane that has nothing to load.
+constexpr int kRowOob = 0x7ffffff0;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(const void* base, int nbytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, nbytes, 0x00020000);
+}
+template <typename C> __device__ __forceinline__ C rsrc_ld(__amdgpu_buffer_rsrc_t rs, int off);
+template <> __device__ __forceinline__ float2 rsrc_ld<float2>(__amdgpu_buffer_rsrc_t rs, int off) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, 0);
+  float2 c;
+  __builtin_memcpy(&c, &v, sizeof(c));
+  return c;
+}
+template <> __device__ __forceinline__ double2 rsrc_ld<double2>(__amdgpu_buffer_rsrc_t rs, int off) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
+  double2 c;
+  __builtin_memcpy(&c, &v, sizeof(c));
+  return c;
+}
+
 // p[k] if ok, else zero.  The load is unconditional, from index 0 when !ok
 // (p must point at one valid element), so it stays one vector load with a
 // select after it instead of a branch around each component.

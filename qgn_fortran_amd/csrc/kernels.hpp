@@ -215,13 +215,20 @@ __device__ __forceinline__ cplx<R> c2r_pre(const cplx<R>* __restrict__ row, int 
   Cx a = ld_or_zero(row, k, k < Kin);
   const int kk = N2 - k;
   Cx b = ld_or_zero(row, kk, kk < Kin);
-  if (k == 0) { a.y = (R)0; b.y = (R)0; }
-  const Cx s = cmk<Cx>(a.x + b.x, a.y - b.y);   // X[k] + conj X[N2-k]
-  const Cx d = cmk<Cx>(a.x - b.x, a.y + b.y);   // X[k] - conj X[N2-k]
-  Cx w = twr[k];
-  w.y = -w.y;                                   // e^{+2 pi i k / nx}
-  const Cx wd = cmul(w, d);
-  return cmk<Cx>((s.x - wd.y) * scale, (s.y + wd.x) * scale);  // s + i*wd
+  return c2r_pre_ab<R>(a, b, k, twr, scale);
+}
+
+// The same, the row at element offset rel (a lane-dependent row of a
+// workgroup's block) of the buffer rs: out-of-range bins load as zero.
+template <typename R>
+__device__ __forceinline__ cplx<R> c2r_pre_rs(__amdgpu_buffer_rsrc_t rs, long rel, int Kin, int k, int N2,
+                                              const cplx<R>* __restrict__ twr, R scale) {
+  using Cx = cplx<R>;
+  constexpr int SZ = (int)sizeof(Cx);
+  const int kk = N2 - k;
+  const Cx a = rsrc_ld<Cx>(rs, k < Kin ? (int)((rel + k) * SZ) : kRowOob);
+  const Cx b = rsrc_ld<Cx>(rs, kk < Kin ? (int)((rel + kk) * SZ) : kRowOob);
+  return c2r_pre_ab<R>(a, b, k, twr, scale);
 }
 
 // x-r2c post-processing: X[k] = (Z[k] + conj Z[N2-k])/2

@@ -130,11 +130,14 @@ __global__ void __launch_bounds__(NTHR) k_row_jac_ct(RowJacArgs a, Geom g, const
       const int e = threadIdx.x + b * NTHR;
       const int r = e / N2, k = e - r * N2;
       const bool ok = (ROWS * N2) % NTHR == 0 || e < ROWS * N2;
+      const int rel = r * (int)Kxp;
+      const int oa = (ok && k < Kx) ? (rel + k) * (int)sizeof(Cx) : kRowOob;
+      const int ob = (ok && N2 - k < Kx) ? (rel + N2 - k) * (int)sizeof(Cx) : kRowOob;
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
-        const Cx* p = srcs[s] + (row0 + (ok ? r : 0)) * Kxp;
-        ra[s][b] = ld_or_zero(p, k, ok && k < Kx);
-        rb[s][b] = ld_or_zero(p, N2 - k, ok && N2 - k < Kx);
+        const __amdgpu_buffer_rsrc_t rs = row_rsrc(srcs[s] + row0 * Kxp, ROWS * (int)Kxp * (int)sizeof(Cx));
+        ra[s][b] = rsrc_ld<Cx>(rs, oa);
+        rb[s][b] = rsrc_ld<Cx>(rs, ob);
       }
     }
   };
@@ -224,24 +227,23 @@ __global__ void __launch_bounds__(ROWS * (N2 / 8)) k_row_jac_fused2(RowJacArgs a
   const int r = threadIdx.x / NB;
   const int t = threadIdx.x - r * NB;
   // last rows first: the bottom layer's heavier groups (drag) start early
-  const int row0 = (gridDim.x - 1 - blockIdx.x) * ROWS;
+  // (the group count from the kernel arguments keeps row0 in scalar registers)
+  const int row0 = (g.nz * g.ny / ROWS - 1 - (int)blockIdx.x) * ROWS;
   const int z = row0 / g.ny;
   const int j0 = row0 - z * g.ny;
   const bool bottom = a.drag && (z % g.nzm == g.nzm - 1);
   const int Kx = g.Kx;
   const long Kxp = g.Kxp;
-  const long roff = (long)(row0 + r) * Kxp;
-  const Cx zero = cmk<Cx>((R)0, (R)0);
+  const long rel = (long)r * Kxp;   // this thread's row within the group
   // x-c2r of one mixed row into registers: o[j] holds x[2m], x[2m+1] at m = t + j NB
   auto inverse = [&](const void* src, Cx(&o)[8]) {
-    const Cx* p = reinterpret_cast<const Cx*>(src) + roff;
+    const __amdgpu_buffer_rsrc_t rs =
+        row_rsrc(reinterpret_cast<const Cx*>(src) + (long)row0 * Kxp, ROWS * (int)Kxp * (int)sizeof(Cx));
     Cx v[8];
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       const int k = t + j * NB;
-      const Cx x = ld_or_zero(p, k, k < Kx);
-      const Cx y = ld_or_zero(p, N2 - k, N2 - k < Kx);
-      v[j] = c2r_pre_ab<R>(x, y, k, tr, scale);
+      v[j] = c2r_pre_rs<R>(rs, rel, Kx, k, N2, tr, scale);
     }
     Dft<Cx, 8>::run(v, +1);
     __syncthreads();   // earlier readers of buf are done
@@ -476,9 +478,10 @@ __global__ void __launch_bounds__(NTHR) k_row_maxabs_ct(const cplx<R>* __restric
   load_table<R>(tr, twr, N2 + 1);
   __syncthreads();
   const long row0 = (long)blockIdx.x * ROWS;
+  const __amdgpu_buffer_rsrc_t rs = row_rsrc(mixed + row0 * stride, ROWS * stride * (int)sizeof(Cx));
   for (int e = threadIdx.x; e < ROWS * N2; e += NTHR) {
     const int r = e / N2, k = e - r * N2;
-    buf[padded<PS>(e)] = c2r_pre<R>(mixed + (row0 + r) * stride, Kin, k, N2, tr, (R)1);
+    buf[padded<PS>(e)] = c2r_pre_rs<R>(rs, (long)r * stride, Kin, k, N2, tr, (R)1);
   }
   __syncthreads();
   ct_fft<R, N2, CtLayout<ROWS, N2, 1, false, PS>, NTHR>(buf, ts, +1);
@@ -519,9 +522,10 @@ __global__ void __launch_bounds__(NTHR) k_row_c2r_ct(const cplx<R>* __restrict__
   load_table<R>(tr, twr, N2 + 1);
   __syncthreads();
   const long row0 = (long)blockIdx.x * ROWS;
+  const __amdgpu_buffer_rsrc_t rs = row_rsrc(mixed + row0 * stride, ROWS * stride * (int)sizeof(Cx));
   for (int e = threadIdx.x; e < ROWS * N2; e += NTHR) {
     const int r = e / N2, k = e - r * N2;
-    buf[padded<PS>(e)] = c2r_pre<R>(mixed + (row0 + r) * stride, Kin, k, N2, tr, scale);
+    buf[padded<PS>(e)] = c2r_pre_rs<R>(rs, (long)r * stride, Kin, k, N2, tr, scale);
   }
   __syncthreads();
   ct_fft<R, N2, CtLayout<ROWS, N2, 1, false, PS>, NTHR>(buf, ts, +1);
