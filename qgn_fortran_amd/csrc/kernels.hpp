@@ -265,38 +265,59 @@ struct StageArgs {
   double2* out;
 };
 
-template <typename S>
+// ARK stage value of element e (layer kl of the batch, point pt of its
+// plane), the stage blocks of ARK43*.f90:
+//   q_tmp = Mq (q + dt sum_j (ae_j N_j + ai_j L_j)),  L_j = -lam Y_j (Y_0 = q),
+//   Mq = 1 / (1 + dt lam / 4),  lam = A2 k2 + A8 k2^4,
+// with Mq = 1 and no L_j without implicit viscosity.  NS, the number of
+// previous stages, is a compile-time count, so all of the element's loads
+// are issued before the first sum.
+template <typename S, int NS>
+__device__ __forceinline__ double2 stage_value(const StageArgs& a, const Geom& g, long e, long pt, int kl,
+                                               double dt) {
+  const double2 q = a.q[e];
+  cplx<S> nv[NS > 0 ? NS : 1];
+#pragma unroll
+  for (int jj = 0; jj < NS; ++jj) nv[jj] = reinterpret_cast<const cplx<S>*>(a.N[jj])[e];
+  double2 yv[NS > 1 ? NS : 1];
+  if (a.implicit_visc) {
+#pragma unroll
+    for (int jj = 1; jj < NS; ++jj) yv[jj] = a.Y[jj][e];
+  }
+  double sx = 0.0, sy = 0.0;
+#pragma unroll
+  for (int jj = 0; jj < NS; ++jj) {
+    sx += a.ae[jj] * (double)nv[jj].x;
+    sy += a.ae[jj] * (double)nv[jj].y;
+  }
+  double mq = 1.0;
+  if (a.implicit_visc) {
+    const int i = (int)(pt % g.Kxp);
+    const int j = compact_row_to_full((int)(pt / g.Kxp), g);
+    const double kx = kx_of(i, g), ky = ky_of(j, g);
+    const double k2 = kx * kx + ky * ky;
+    const double k4 = k2 * k2;
+    const double a8 = a.a8l ? a.a8l[kl] : a.A8;
+    const double lam = a.A2 * k2 + a8 * (k4 * k4);
+#pragma unroll
+    for (int jj = 0; jj < NS; ++jj) {
+      const double2 y = jj == 0 ? q : yv[jj];
+      sx += a.ai[jj] * (-lam * y.x);
+      sy += a.ai[jj] * (-lam * y.y);
+    }
+    mq = 1.0 / (1.0 + 0.25 * dt * lam);
+  }
+  return make_double2(mq * (q.x + dt * sx), mq * (q.y + dt * sy));
+}
+
+template <typename S, int NS>
 __global__ void k_stage(StageArgs a, Geom g) {
   const long plane = (long)g.Ky * g.Kxp;
   const long n = plane * g.nz;
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
-    const double dt = a.dtv ? a.dtv[(int)(e / plane) / g.nzm] : a.dt;
-    const double2 q = a.q[e];
-    double sx = 0.0, sy = 0.0;
-    for (int jj = 0; jj < a.ns; ++jj) {
-      const cplx<S> v = reinterpret_cast<const cplx<S>*>(a.N[jj])[e];
-      sx += a.ae[jj] * (double)v.x;
-      sy += a.ae[jj] * (double)v.y;
-    }
-    double mq = 1.0;
-    if (a.implicit_visc) {
-      const int z = (int)(e / plane);
-      const long pt = e % plane;
-      const int i = (int)(pt % g.Kxp);
-      const int j = compact_row_to_full((int)(pt / g.Kxp), g);
-      const double kx = kx_of(i, g), ky = ky_of(j, g);
-      const double k2 = kx * kx + ky * ky;
-      const double k4 = k2 * k2;
-      const double a8 = a.a8l ? a.a8l[z] : a.A8;
-      const double lam = a.A2 * k2 + a8 * (k4 * k4);
-      for (int jj = 0; jj < a.ns; ++jj) {
-        const double2 y = jj == 0 ? q : a.Y[jj][e];
-        sx += a.ai[jj] * (-lam * y.x);
-        sy += a.ai[jj] * (-lam * y.y);
-      }
-      mq = 1.0 / (1.0 + 0.25 * dt * lam);
-    }
-    a.out[e] = make_double2(mq * (q.x + dt * sx), mq * (q.y + dt * sy));
+    const int z = (int)(e / plane);
+    const double dt = a.dtv ? a.dtv[z / g.nzm] : a.dt;
+    a.out[e] = stage_value<S, NS>(a, g, e, e % plane, z, dt);
   }
 }
 
@@ -304,7 +325,7 @@ __global__ void k_stage(StageArgs a, Geom g) {
 // x nz layers, one thread per (point, layer).  Each thread forms q_tmp of its
 // layer (as k_stage) and writes it; the workgroup then applies the
 // layer -> mode -> layer projection of k_project through LDS and writes psi.
-template <typename S, typename Tp>
+template <typename S, typename Tp, int NS>
 __global__ void __launch_bounds__(256) k_stage_proj(StageArgs a, Geom g, const double* __restrict__ HM,
                                                     const double* __restrict__ MD, const double* __restrict__ evals,
                                                     double inv_htot, int pp_per_block, cplx<Tp>* __restrict__ psi) {
@@ -335,26 +356,7 @@ __global__ void __launch_bounds__(256) k_stage_proj(StageArgs a, Geom g, const d
     const double kx = kx_of(i, g), ky = ky_of(j, g);
     k2 = kx * kx + ky * ky;
     const long e = kl * plane + pt;
-    const double2 q = a.q[e];
-    double sx = 0.0, sy = 0.0;
-    for (int jj = 0; jj < a.ns; ++jj) {
-      const cplx<S> v = reinterpret_cast<const cplx<S>*>(a.N[jj])[e];
-      sx += a.ae[jj] * (double)v.x;
-      sy += a.ae[jj] * (double)v.y;
-    }
-    double mq = 1.0;
-    if (a.implicit_visc) {
-      const double k4 = k2 * k2;
-      const double a8 = a.a8l ? a.a8l[kl] : a.A8;
-      const double lam = a.A2 * k2 + a8 * (k4 * k4);
-      for (int jj = 0; jj < a.ns; ++jj) {
-        const double2 y = jj == 0 ? q : a.Y[jj][e];
-        sx += a.ai[jj] * (-lam * y.x);
-        sy += a.ai[jj] * (-lam * y.y);
-      }
-      mq = 1.0 / (1.0 + 0.25 * dt * lam);
-    }
-    const double2 qt = make_double2(mq * (q.x + dt * sx), mq * (q.y + dt * sy));
+    const double2 qt = stage_value<S, NS>(a, g, e, pt, kl, dt);
     a.out[e] = qt;
     sq[k * pp_per_block + pp] = qt;
   }
@@ -420,7 +422,7 @@ __device__ __forceinline__ qgn_float4 mfma3(const qgn_short4& ah, const qgn_shor
 // and 1 then project 8 points each (16 columns: re, im), building the split
 // matrix operands as they go (measured faster at 16 layers than holding
 // them across tiles, which needs twice the registers).
-template <typename S>
+template <typename S, int NS>
 __global__ void __launch_bounds__(256) k_stage_proj_mfma16(StageArgs a, Geom g, const double* __restrict__ HM,
                                                          const double* __restrict__ MD,
                                                          const double* __restrict__ evals, double inv_htot,
@@ -441,30 +443,7 @@ __global__ void __launch_bounds__(256) k_stage_proj_mfma16(StageArgs a, Geom g, 
     double2 qt = make_double2(0.0, 0.0);
     if (k < nz && pt < plane) {
       const long e = moff + k * plane + pt;
-      const double2 q = a.q[e];
-      double sx = 0.0, sy = 0.0;
-      for (int jj = 0; jj < a.ns; ++jj) {
-        const cplx<S> v = reinterpret_cast<const cplx<S>*>(a.N[jj])[e];
-        sx += a.ae[jj] * (double)v.x;
-        sy += a.ae[jj] * (double)v.y;
-      }
-      double mq = 1.0;
-      if (a.implicit_visc) {
-        const int i = (int)(pt % g.Kxp);
-        const int j = compact_row_to_full((int)(pt / g.Kxp), g);
-        const double kx = kx_of(i, g), ky = ky_of(j, g);
-        const double k2 = kx * kx + ky * ky;
-        const double k4 = k2 * k2;
-        const double a8 = a.a8l ? a.a8l[mb * nz + k] : a.A8;
-        const double lam = a.A2 * k2 + a8 * (k4 * k4);
-        for (int jj = 0; jj < a.ns; ++jj) {
-          const double2 y = jj == 0 ? q : a.Y[jj][e];
-          sx += a.ai[jj] * (-lam * y.x);
-          sy += a.ai[jj] * (-lam * y.y);
-        }
-        mq = 1.0 / (1.0 + 0.25 * dt * lam);
-      }
-      qt = make_double2(mq * (q.x + dt * sx), mq * (q.y + dt * sy));
+      qt = stage_value<S, NS>(a, g, e, pt, mb * nz + k, dt);
       if (a.out) a.out[e] = qt;
     }
     sq[k][pp] = qt;
@@ -531,7 +510,7 @@ __global__ void __launch_bounds__(256) k_stage_proj_mfma16(StageArgs a, Geom g, 
 // The tile buffer alternates between two halves, so one barrier per tile
 // separates its writers from its readers (each half is rewritten only after
 // the next tile's barrier, by which time its readers are done).
-template <typename S, int NZT>
+template <typename S, int NZT, int NS>
 __global__ void __launch_bounds__(256) k_stage_proj_mfma(StageArgs a, Geom g, const double* __restrict__ HM,
                                                          const double* __restrict__ MD,
                                                          const double* __restrict__ evals, double inv_htot,
@@ -584,30 +563,7 @@ __global__ void __launch_bounds__(256) k_stage_proj_mfma(StageArgs a, Geom g, co
       double2 qt = make_double2(0.0, 0.0);
       if (k < nz && pt < plane) {
         const long e = moff + k * plane + pt;
-        const double2 q = a.q[e];
-        double sx = 0.0, sy = 0.0;
-        for (int jj = 0; jj < a.ns; ++jj) {
-          const cplx<S> v = reinterpret_cast<const cplx<S>*>(a.N[jj])[e];
-          sx += a.ae[jj] * (double)v.x;
-          sy += a.ae[jj] * (double)v.y;
-        }
-        double mq = 1.0;
-        if (a.implicit_visc) {
-          const int i = (int)(pt % g.Kxp);
-          const int j = compact_row_to_full((int)(pt / g.Kxp), g);
-          const double kx = kx_of(i, g), ky = ky_of(j, g);
-          const double k2 = kx * kx + ky * ky;
-          const double k4 = k2 * k2;
-          const double a8 = a.a8l ? a.a8l[mb * nz + k] : a.A8;
-          const double lam = a.A2 * k2 + a8 * (k4 * k4);
-          for (int jj = 0; jj < a.ns; ++jj) {
-            const double2 y = jj == 0 ? q : a.Y[jj][e];
-            sx += a.ai[jj] * (-lam * y.x);
-            sy += a.ai[jj] * (-lam * y.y);
-          }
-          mq = 1.0 / (1.0 + 0.25 * dt * lam);
-        }
-        qt = make_double2(mq * (q.x + dt * sx), mq * (q.y + dt * sy));
+        qt = stage_value<S, NS>(a, g, e, pt, mb * nz + k, dt);
         if (a.out) a.out[e] = qt;
       }
       sq[half][k][pp] = qt;

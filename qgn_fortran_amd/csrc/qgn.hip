@@ -494,6 +494,22 @@ int launch_col_inv(qgn_ctx* c, const ColInvArgs& a) {
   return rc;
 }
 
+// f(std::integral_constant<int, NS>) for the stage count a.ns = NS (0..6):
+// the stage kernels take it as a template argument.
+template <typename F>
+int with_ns(int ns, F&& f) {
+  switch (ns) {
+    case 0: return f(std::integral_constant<int, 0>{});
+    case 1: return f(std::integral_constant<int, 1>{});
+    case 2: return f(std::integral_constant<int, 2>{});
+    case 3: return f(std::integral_constant<int, 3>{});
+    case 4: return f(std::integral_constant<int, 4>{});
+    case 5: return f(std::integral_constant<int, 5>{});
+    case 6: return f(std::integral_constant<int, 6>{});
+    default: return fail(QGN_EINVAL, "stage count out of range");
+  }
+}
+
 // Mixed precision with nz <= 32: the stage + split-BF16 MFMA projection.
 int launch_stage_proj_mfma(qgn_ctx* c, const StageArgs& a, int kernel_id) {
   const long plane = (long)c->g.Ky * c->g.Kxp;
@@ -503,13 +519,16 @@ int launch_stage_proj_mfma(qgn_ctx* c, const StageArgs& a, int kernel_id) {
   // workgroup
   const long want = c->g.nzm <= 16 ? tiles : std::max(1L, (8L * c->ncu + c->nm - 1) / c->nm);
   const int blocks = (int)std::min(tiles, want);
-  return timed(c, kernel_id, [&] {
-    if (c->g.nzm <= 16)
-      hipLaunchKernelGGL((k_stage_proj_mfma16<float>), dim3(blocks, c->nm), dim3(256), 0, c->stream, a, c->g, c->HM,
-                         c->MD, c->evals, 1.0 / c->p.Htot, reinterpret_cast<float2*>(c->psi));
-    else
-      hipLaunchKernelGGL((k_stage_proj_mfma<float, 32>), dim3(blocks, c->nm), dim3(256), 0, c->stream, a, c->g, c->HM,
-                         c->MD, c->evals, 1.0 / c->p.Htot, reinterpret_cast<float2*>(c->psi));
+  return with_ns(a.ns, [&](auto nsc) {
+    constexpr int NS = decltype(nsc)::value;
+    return timed(c, kernel_id, [&] {
+      if (c->g.nzm <= 16)
+        hipLaunchKernelGGL((k_stage_proj_mfma16<float, NS>), dim3(blocks, c->nm), dim3(256), 0, c->stream, a, c->g,
+                           c->HM, c->MD, c->evals, 1.0 / c->p.Htot, reinterpret_cast<float2*>(c->psi));
+      else
+        hipLaunchKernelGGL((k_stage_proj_mfma<float, 32, NS>), dim3(blocks, c->nm), dim3(256), 0, c->stream, a, c->g,
+                           c->HM, c->MD, c->evals, 1.0 / c->p.Htot, reinterpret_cast<float2*>(c->psi));
+    });
   });
 }
 bool use_mfma_projection(const qgn_ctx* c) { return c->p.precision == QGN_MIXED && c->g.nzm <= 32; }
@@ -804,8 +823,11 @@ int leith(qgn_ctx* c, const double2* q) {
 template <typename S>
 int launch_stage(qgn_ctx* c, const StageArgs& a) {
   const long n = c->ncmp;
-  return timed(c, QGN_K_STAGE, [&] {
-    hipLaunchKernelGGL((k_stage<S>), dim3(grid1d(n)), dim3(256), 0, c->stream, a, c->g);
+  return with_ns(a.ns, [&](auto nsc) {
+    constexpr int NS = decltype(nsc)::value;
+    return timed(c, QGN_K_STAGE, [&] {
+      hipLaunchKernelGGL((k_stage<S, NS>), dim3(grid1d(n)), dim3(256), 0, c->stream, a, c->g);
+    });
   });
 }
 int stage(qgn_ctx* c, const StageArgs& a) {
@@ -824,9 +846,12 @@ int launch_stage_proj(qgn_ctx* c, const StageArgs& a, bool* done) {
   const int blocks = (int)((plane + pp - 1) / pp);
   const size_t lds = 2 * (size_t)nz * nz * sizeof(double) + 2 * (size_t)nz * pp * sizeof(double2);
   void* psi = std::is_same<S, double>::value ? (void*)c->psi64 : c->psi;
-  return timed(c, QGN_K_STAGE, [&] {
-    hipLaunchKernelGGL((k_stage_proj<S, S>), dim3(blocks, c->nm), dim3(threads), lds, c->stream, a, c->g, c->HM, c->MD,
-                       c->evals, 1.0 / c->p.Htot, pp, reinterpret_cast<cplx<S>*>(psi));
+  return with_ns(a.ns, [&](auto nsc) {
+    constexpr int NS = decltype(nsc)::value;
+    return timed(c, QGN_K_STAGE, [&] {
+      hipLaunchKernelGGL((k_stage_proj<S, S, NS>), dim3(blocks, c->nm), dim3(threads), lds, c->stream, a, c->g, c->HM,
+                         c->MD, c->evals, 1.0 / c->p.Htot, pp, reinterpret_cast<cplx<S>*>(psi));
+    });
   });
 }
 int stage_proj(qgn_ctx* c, const StageArgs& a, bool* done) {
@@ -1253,14 +1278,15 @@ int autotune(qgn_ctx* c) {
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
-  auto time3 = [&](auto&& launch, float* ms) -> int {
+  // mean of 5 launches after one untimed launch
+  auto time5 = [&](auto&& launch, float* ms) -> int {
     CHK(launch());
     HIPCHK(hipEventRecord(e0, s));
-    for (int i = 0; i < 3; ++i) CHK(launch());
+    for (int i = 0; i < 5; ++i) CHK(launch());
     HIPCHK(hipEventRecord(e1, s));
     HIPCHK(hipEventSynchronize(e1));
     HIPCHK(hipEventElapsedTime(ms, e0, e1));
-    *ms /= 3.f;
+    *ms /= 5.f;
     return 0;
   };
   ColInvArgs ca{};
@@ -1278,29 +1304,40 @@ int autotune(qgn_ctx* c) {
   ka.J = c->Jm; ka.q = c->q; ka.psi = c->psi; ka.out = c->N[5];
   ka.ubar = c->ubar; ka.qybar = c->qybar; ka.ekman = c->ekm;
   const int tiles[4] = {2, 4, 8, 16}, rows[3] = {2, 4, 8};
-  int rc = 0;
-  float best[3] = {1e30f, 1e30f, 1e30f};
-  int pick[3] = {0, 0, 0};
-  for (int k = 0; k < 4 && !rc; ++k) {
-    if (!col_tile_ok(tiles[k], g.ny, sizeof(float2))) continue;
-    float ms = 0.f;
-    c->tune_col_inv = tiles[k];
-    rc = time3([&] { return launch_col_inv<float>(c, ca); }, &ms);
-    c->tune_ms[0][k] = ms * 1e3f;
-    if (!rc && ms < best[0]) { best[0] = ms; pick[0] = tiles[k]; }
-    c->tune_col_rhs = tiles[k];
-    rc = rc ? rc : time3([&] { return launch_col_rhs<float>(c, ka, ra.drag); }, &ms);
-    c->tune_ms[1][k] = ms * 1e3f;
-    if (!rc && ms < best[1]) { best[1] = ms; pick[1] = tiles[k]; }
-  }
   const bool fused_row = g.N2 % 64 == 0 && g.N2 <= 1024;
-  for (int k = 0; k < 3 && !rc && fused_row; ++k) {
-    if (rows[k] * g.N2 / 8 > 1024) continue;
-    float ms = 0.f;
-    c->tune_row = rows[k];
-    rc = time3([&] { return launch_row_jac<float>(c, ra); }, &ms);
-    c->tune_ms[2][k] = ms * 1e3f;
-    if (!rc && ms < best[2]) { best[2] = ms; pick[2] = rows[k]; }
+  int rc = 0;
+  float best[3][4];
+  for (auto& b : best)
+    for (float& v : b) v = 1e30f;
+  // an idle GPU runs its first launches at lower clocks: warm up, then time
+  // every candidate in two interleaved rounds and keep each one's faster
+  for (int i = 0; i < 10 && !rc; ++i) rc = launch_col_inv<float>(c, ca);
+  for (int round = 0; round < 2 && !rc; ++round) {
+    for (int k = 0; k < 4 && !rc; ++k) {
+      if (!col_tile_ok(tiles[k], g.ny, sizeof(float2))) continue;
+      float ms = 0.f;
+      c->tune_col_inv = tiles[k];
+      rc = time5([&] { return launch_col_inv<float>(c, ca); }, &ms);
+      if (!rc) best[0][k] = std::min(best[0][k], ms);
+      c->tune_col_rhs = tiles[k];
+      rc = rc ? rc : time5([&] { return launch_col_rhs<float>(c, ka, ra.drag); }, &ms);
+      if (!rc) best[1][k] = std::min(best[1][k], ms);
+    }
+    for (int k = 0; k < 3 && !rc && fused_row; ++k) {
+      if (rows[k] * g.N2 / 8 > 1024) continue;
+      float ms = 0.f;
+      c->tune_row = rows[k];
+      rc = time5([&] { return launch_row_jac<float>(c, ra); }, &ms);
+      if (!rc) best[2][k] = std::min(best[2][k], ms);
+    }
+  }
+  int pick[3] = {0, 0, 0};
+  for (int w = 0; w < 3; ++w) {
+    float bm = 1e30f;
+    for (int k = 0; k < (w == 2 ? 3 : 4); ++k) {
+      if (best[w][k] < 1e29f) c->tune_ms[w][k] = best[w][k] * 1e3f;
+      if (best[w][k] < bm) { bm = best[w][k]; pick[w] = w == 2 ? rows[k] : tiles[k]; }
+    }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
