@@ -164,14 +164,16 @@ int qgn_get_stats(qgn_ctx* ctx, qgn_stats* s);
 int qgn_get_stream(qgn_ctx* ctx, void** stream);
 
 /* Launch choices of the mixed path's FFT passes.  At creation a mixed
- * context times every compiled tile width of the column passes (2, 4, 8, 16
- * columns; inverse and forward separately) and row group of the fused row
- * pass (2, 4, 8 rows) on its own buffers and keeps the fastest
- * (QGN_AUTOTUNE=0 in the environment keeps the fixed rules: 8 columns, 4
- * rows).  qgn_get_tuning returns the choices (0 = fixed rule) and, in
- * times_us[3][4], the measured times (column inverse and forward by tile
- * width index, row pass by group index); qgn_set_tuning forces choices.
- * No counterpart in the reference. */
+ * context picks the tile width of each column pass (2, 4, 8, 16 columns;
+ * inverse and forward separately) and the row group of the fused row pass
+ * (2, 4, 8 rows) by timing whole GetRHS evaluations on its own buffers, one
+ * pass's candidates at a time with the others at their current picks, two
+ * sweeps, a candidate displacing the current pick only when 2% faster
+ * (QGN_AUTOTUNE=0 in the environment keeps the fixed rules: 8 columns for
+ * the inverse and 4 for the forward column pass up to ny = 1024, 4 rows).  qgn_get_tuning returns the choices (0 = fixed rule)
+ * and, in times_us[3][4], the GetRHS time of each candidate in the last
+ * sweep (column inverse and forward by tile width index, row pass by group
+ * index); qgn_set_tuning forces choices.  No counterpart in the reference. */
 int qgn_get_tuning(qgn_ctx* ctx, int* col_inv, int* col_rhs, int* row, float* times_us);
 int qgn_set_tuning(qgn_ctx* ctx, int col_inv, int col_rhs, int row);
 

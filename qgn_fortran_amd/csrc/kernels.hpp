@@ -417,17 +417,19 @@ __device__ __forceinline__ qgn_float4 mfma3(const qgn_short4& ah, const qgn_shor
 }
 
 // Stage combination (or, with a.ns = 0 and a.out = nullptr, a plain copy of
-// q) followed by the MFMA projection, nz <= 16.  Workgroup: 16 points x 16
-// layers, one thread per (point, layer) for the FP64 stage sums; waves 0
-// and 1 then project 8 points each (16 columns: re, im), building the split
-// matrix operands as they go (measured faster at 16 layers than holding
-// them across tiles, which needs twice the registers).
+// q) followed by the MFMA projection, nz <= 16.  Workgroup: 32 points x 16
+// layers, two (point, layer) elements per thread for the FP64 stage sums
+// (consecutive threads on consecutive points: 512 contiguous bytes of q per
+// layer); then each of the 4 waves projects 8 points (16 columns: re, im).
+// Every wave builds the split matrix operands before the stage loads, so
+// those small loads overlap them (measured faster at 16 layers than holding
+// the operands across tiles, which needs twice the registers).
 template <typename S, int NS>
 __global__ void __launch_bounds__(256) k_stage_proj_mfma16(StageArgs a, Geom g, const double* __restrict__ HM,
                                                          const double* __restrict__ MD,
                                                          const double* __restrict__ evals, double inv_htot,
                                                          float2* __restrict__ psi) {
-  __shared__ double2 sq[16][16];    // [layer][point]
+  __shared__ double2 sq[16][32];    // [layer][point]
   const int nz = g.nzm;
   const int mb = blockIdx.y;         // member
   HM += mb * nz * nz;
@@ -437,23 +439,11 @@ __global__ void __launch_bounds__(256) k_stage_proj_mfma16(StageArgs a, Geom g, 
   const double dt = a.dtv ? a.dtv[mb] : a.dt;
   const int tid = threadIdx.x;
   const long plane = (long)g.Ky * g.Kxp;
-  {
-    const int pp = tid & 15, k = tid >> 4;
-    const long pt = (long)blockIdx.x * 16 + pp;
-    double2 qt = make_double2(0.0, 0.0);
-    if (k < nz && pt < plane) {
-      const long e = moff + k * plane + pt;
-      qt = stage_value<S, NS>(a, g, e, pt, mb * nz + k, dt);
-      if (a.out) a.out[e] = qt;
-    }
-    sq[k][pp] = qt;
-  }
-  __syncthreads();
-  if (tid >= 128) return;
   const int l = tid & 63;
-  const int tile = tid >> 6;                 // points 8*tile .. 8*tile+7
+  const int tile = tid >> 6;                  // wave: points 8*tile .. 8*tile+7
   const int row = l & 15, kb = 4 * (l >> 4);  // A: i = row; B, D: j = row
   qgn_short4 a1h, a1l, a2h, a2l, bh, bl;
+  double ev[4];
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const int kk = kb + r;
@@ -463,14 +453,36 @@ __global__ void __launch_bounds__(256) k_stage_proj_mfma16(StageArgs a, Geom g, 
     a1l[r] = bf16_lo(m1, a1h[r]);
     a2h[r] = bf16_rne(m2);
     a2l[r] = bf16_lo(m2, a2h[r]);
-    const double2 v = sq[kk][8 * tile + (row >> 1)];
+    ev[r] = kk < nz ? evals[kk] : 1.0;
+  }
+  {
+    const int pp = tid & 31, k0 = tid >> 5;   // layers k0 and k0 + 8
+    const long pt = (long)blockIdx.x * 32 + pp;
+    double2 qt[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int k = k0 + 8 * h;
+      qt[h] = make_double2(0.0, 0.0);
+      if (k < nz && pt < plane) qt[h] = stage_value<S, NS>(a, g, moff + k * plane + pt, pt, mb * nz + k, dt);
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int k = k0 + 8 * h;
+      if (a.out && k < nz && pt < plane) a.out[moff + k * plane + pt] = qt[h];
+      sq[k][pp] = qt[h];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const double2 v = sq[kb + r][8 * tile + (row >> 1)];
     const float x = (float)((row & 1) ? v.y : v.x);
     bh[r] = bf16_rne(x);
     bl[r] = bf16_lo(x, bh[r]);
   }
   qgn_float4 d = {0.f, 0.f, 0.f, 0.f};
   d = mfma3(a1h, a1l, bh, bl, d);            // modal amplitudes, D[mode][column]
-  const long pt = (long)blockIdx.x * 16 + 8 * tile + (row >> 1);
+  const long pt = (long)blockIdx.x * 32 + 8 * tile + (row >> 1);
   bool zero_mode = pt >= plane;
   double k2 = 0.0;
   if (!zero_mode) {
@@ -483,7 +495,7 @@ __global__ void __launch_bounds__(256) k_stage_proj_mfma16(StageArgs a, Geom g, 
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const int mode = kb + r;
-    const float pm = (zero_mode || mode >= nz) ? 0.f : (float)((double)d[r] / (-k2 + evals[mode]));
+    const float pm = (zero_mode || mode >= nz) ? 0.f : (float)((double)d[r] / (-k2 + ev[r]));
     bh[r] = bf16_rne(pm);
     bl[r] = bf16_lo(pm, bh[r]);
   }

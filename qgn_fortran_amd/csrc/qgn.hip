@@ -405,6 +405,14 @@ template <typename R, int NY> struct ColCfg {
   static constexpr int NTHR = ColTile<R, NY, C>::NTHR;
   static constexpr size_t LDS = ColTile<R, NY, C>::LDS;
 };
+// the fixed rule's FP32 column tile at run time
+int ColCfgF(int ny) { return ny <= 1024 ? 8 : pow2_floor(8192 / ny); }
+// P3 in FP32: half that (more tiles share out the bottom layers' drag work;
+// 0.8% faster steps at 1024^2 x 16)
+template <int NY> struct ColRhsCfg {
+  static constexpr int C = ColCfg<float, NY>::C / 2 > 2 ? ColCfg<float, NY>::C / 2 : 2;
+};
+int ColRhsF(int ny) { return std::max(2, ColCfgF(ny) / 2); }
 // runtime tile width -> compile-time constant
 template <typename F>
 bool with_tile(int C, F&& f) {
@@ -513,10 +521,10 @@ int with_ns(int ns, F&& f) {
 // Mixed precision with nz <= 32: the stage + split-BF16 MFMA projection.
 int launch_stage_proj_mfma(qgn_ctx* c, const StageArgs& a, int kernel_id) {
   const long plane = (long)c->g.Ky * c->g.Kxp;
-  const long tiles = (plane + 15) / 16;
-  // nz <= 16: one tile per workgroup; nz <= 32: workgroups walk tiles, 8 per
-  // CU over all members, so the 4x larger split matrices are built once per
-  // workgroup
+  // nz <= 16: one tile of 32 points per workgroup; nz <= 32: workgroups walk
+  // tiles of 16 points, 8 per CU over all members, so the 4x larger split
+  // matrices are built once per workgroup
+  const long tiles = c->g.nzm <= 16 ? (plane + 31) / 32 : (plane + 15) / 16;
   const long want = c->g.nzm <= 16 ? tiles : std::max(1L, (8L * c->ncu + c->nm - 1) / c->nm);
   const int blocks = (int)std::min(tiles, want);
   return with_ns(a.ns, [&](auto nsc) {
@@ -617,7 +625,7 @@ int launch_col_rhs(qgn_ctx* c, const ColRhsArgs& ka0, bool drag) {
       }
     };
     if constexpr (std::is_same<R, float>::value) {
-      if (!with_tile(c->tune_col_rhs ? c->tune_col_rhs : Cfg::C, go)) rc = fail(QGN_EINVAL, "bad column tile");
+      if (!with_tile(c->tune_col_rhs ? c->tune_col_rhs : ColRhsCfg<NY>::C, go)) rc = fail(QGN_EINVAL, "bad column tile");
     } else {
       go(std::integral_constant<int, Cfg::C>{});
     }
@@ -1278,67 +1286,68 @@ int autotune(qgn_ctx* c) {
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
-  // mean of 5 launches after one untimed launch
-  auto time5 = [&](auto&& launch, float* ms) -> int {
-    CHK(launch());
+  // One GetRHS on the context's own buffers (q -> N_6), timed as the mean
+  // of 10 after one untimed: each pass then meets the cache state it has in
+  // a step (back-to-back launches of one kernel alone find their inputs in
+  // the cache and favour other tiles).
+  auto time_rhs = [&](float* ms) -> int {
+    CHK(rhs_impl<float>(c, c->q, c->N[5], false));
     HIPCHK(hipEventRecord(e0, s));
-    for (int i = 0; i < 5; ++i) CHK(launch());
+    for (int i = 0; i < 10; ++i) CHK(rhs_impl<float>(c, c->q, c->N[5], false));
     HIPCHK(hipEventRecord(e1, s));
     HIPCHK(hipEventSynchronize(e1));
     HIPCHK(hipEventElapsedTime(ms, e0, e1));
-    *ms /= 5.f;
+    *ms /= 10.f;
     return 0;
   };
-  ColInvArgs ca{};
-  ca.nf = 4;
-  for (int f = 0; f < 4; ++f) {   // as the RHS: u, v from psi (FP32); q_x, q_y from q (FP64)
-    ca.src[f] = f < 2 ? (const void*)c->psi : (const void*)c->q;
-    ca.src_is_double[f] = f < 2 ? 0 : 1;
-  }
-  ca.mult[0] = 1; ca.mult[1] = 2; ca.mult[2] = 2; ca.mult[3] = 3;
-  ca.dst[0] = c->Um; ca.dst[1] = c->Vm; ca.dst[2] = c->QXm; ca.dst[3] = c->QYm;
-  RowJacArgs ra{};
-  ra.U = c->Um; ra.V = c->Vm; ra.QX = c->QXm; ra.QY = c->QYm; ra.J = c->Jm; ra.DU = c->DUm; ra.DV = c->DVm;
-  ra.drag = c->p.C_d != 0.0;   // the bottom-layer drag rides on the row and P3 passes
-  ColRhsArgs ka{};
-  ka.J = c->Jm; ka.q = c->q; ka.psi = c->psi; ka.out = c->N[5];
-  ka.ubar = c->ubar; ka.qybar = c->qybar; ka.ekman = c->ekm;
   const int tiles[4] = {2, 4, 8, 16}, rows[3] = {2, 4, 8};
   const bool fused_row = g.N2 % 64 == 0 && g.N2 <= 1024;
+  const qgn_stats st0 = c->st;
   int rc = 0;
-  float best[3][4];
-  for (auto& b : best)
+  // an idle GPU runs its first launches at lower clocks: warm up first
+  for (int i = 0; i < 4 && !rc; ++i) rc = rhs_impl<float>(c, c->q, c->N[5], false);
+  // coordinate descent from the fixed rules, three sweeps: each pass's
+  // candidates with the other passes at their current picks, each
+  // candidate's best time over the sweeps.  GetRHS alone misjudges the
+  // tiles by about 1% against whole steps (the stage kernels change what
+  // the cache holds), so a candidate displaces the current pick only when
+  // faster by more than 2%
+  int* knob[3] = {&c->tune_col_inv, &c->tune_col_rhs, &c->tune_row};
+  float tbest[3][4];
+  for (auto& b : tbest)
     for (float& v : b) v = 1e30f;
-  // an idle GPU runs its first launches at lower clocks: warm up, then time
-  // every candidate in two interleaved rounds and keep each one's faster
-  for (int i = 0; i < 10 && !rc; ++i) rc = launch_col_inv<float>(c, ca);
-  for (int round = 0; round < 2 && !rc; ++round) {
-    for (int k = 0; k < 4 && !rc; ++k) {
-      if (!col_tile_ok(tiles[k], g.ny, sizeof(float2))) continue;
-      float ms = 0.f;
-      c->tune_col_inv = tiles[k];
-      rc = time5([&] { return launch_col_inv<float>(c, ca); }, &ms);
-      if (!rc) best[0][k] = std::min(best[0][k], ms);
-      c->tune_col_rhs = tiles[k];
-      rc = rc ? rc : time5([&] { return launch_col_rhs<float>(c, ka, ra.drag); }, &ms);
-      if (!rc) best[1][k] = std::min(best[1][k], ms);
-    }
-    for (int k = 0; k < 3 && !rc && fused_row; ++k) {
-      if (rows[k] * g.N2 / 8 > 1024) continue;
-      float ms = 0.f;
-      c->tune_row = rows[k];
-      rc = time5([&] { return launch_row_jac<float>(c, ra); }, &ms);
-      if (!rc) best[2][k] = std::min(best[2][k], ms);
-    }
-  }
-  int pick[3] = {0, 0, 0};
-  for (int w = 0; w < 3; ++w) {
-    float bm = 1e30f;
-    for (int k = 0; k < (w == 2 ? 3 : 4); ++k) {
-      if (best[w][k] < 1e29f) c->tune_ms[w][k] = best[w][k] * 1e3f;
-      if (best[w][k] < bm) { bm = best[w][k]; pick[w] = w == 2 ? rows[k] : tiles[k]; }
+  for (int sweep = 0; sweep < 3 && !rc; ++sweep) {
+    for (int w = 0; w < 3 && !rc; ++w) {
+      if (w == 2 && !fused_row) continue;
+      const int n = w == 2 ? 3 : 4;
+      const int cur = *knob[w];
+      for (int k = 0; k < n && !rc; ++k) {
+        const int v = w == 2 ? rows[k] : tiles[k];
+        if (w < 2 && !col_tile_ok(v, g.ny, sizeof(float2))) continue;
+        if (w == 2 && v * g.N2 / 8 > 1024) continue;
+        *knob[w] = v;
+        float ms = 0.f;
+        rc = time_rhs(&ms);
+        if (!rc) tbest[w][k] = std::min(tbest[w][k], ms);
+      }
+      // the current pick (0: the fixed rule's value) against the others
+      const int dflt = w == 2 ? 4 : w == 1 ? ColRhsF(g.ny) : ColCfgF(g.ny);
+      int ci = -1;
+      for (int k = 0; k < n; ++k)
+        if ((w == 2 ? rows[k] : tiles[k]) == (cur ? cur : dflt)) ci = k;
+      int bk = ci;
+      for (int k = 0; k < n; ++k) {
+        if (tbest[w][k] >= 1e29f) continue;
+        if (bk < 0 || tbest[w][k] < 0.98f * tbest[w][bk]) bk = k;
+      }
+      *knob[w] = bk < 0 ? cur : (w == 2 ? rows[bk] : tiles[bk]);
     }
   }
+  for (int w = 0; w < 3; ++w)
+    for (int k = 0; k < 4; ++k)
+      if (tbest[w][k] < 1e29f) c->tune_ms[w][k] = tbest[w][k] * 1e3f;
+  c->st = st0;
+  int pick[3] = {c->tune_col_inv, c->tune_col_rhs, c->tune_row};
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   c->tune_col_inv = pick[0];
