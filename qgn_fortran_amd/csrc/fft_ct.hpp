@@ -160,7 +160,17 @@ __device__ __forceinline__ void ct_fft(cplx<Rt>* buf, const cplx<Rt>* __restrict
 // copy a table of `count` complex values into LDS (caller synchronises)
 template <typename Rt>
 __device__ __forceinline__ void load_table(cplx<Rt>* dst, const cplx<Rt>* __restrict__ src, int count) {
-  for (int i = threadIdx.x; i < count; i += blockDim.x) dst[i] = src[i];
+  // four loads in flight per thread before their LDS stores
+  const int step = blockDim.x;
+  int i = threadIdx.x;
+  for (; i + 3 * step < count; i += 4 * step) {
+    const cplx<Rt> a = src[i], b = src[i + step], c = src[i + 2 * step], d = src[i + 3 * step];
+    dst[i] = a;
+    dst[i + step] = b;
+    dst[i + 2 * step] = c;
+    dst[i + 3 * step] = d;
+  }
+  for (; i < count; i += step) dst[i] = src[i];
 }
 
 }  // namespace qgn

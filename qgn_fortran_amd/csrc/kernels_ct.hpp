@@ -49,27 +49,47 @@ __global__ void __launch_bounds__(NTHR) k_col_inv_ct(ColInvArgs a, Geom g, const
   {
     const int mult = a.mult[f];
     const bool isd = a.src_is_double[f];
-    const void* src = a.src[f];
-    for (int e = threadIdx.x; e < TOT; e += NTHR) {
+    // the layer's compact plane as a raw buffer: the dropped ky rows (and
+    // lanes past the tile) load as zero, and all EPT loads of a thread are
+    // in flight together
+    constexpr int EPT = (TOT + NTHR - 1) / NTHR;
+    const int lbytes = g.Ky * g.Kxp * (isd ? (int)sizeof(double2) : (int)sizeof(Cx));
+    const __amdgpu_buffer_rsrc_t rs =
+        row_rsrc(static_cast<const unsigned char*>(a.src[f]) + (long)z * lbytes, lbytes);
+    int off[EPT];
+#pragma unroll
+    for (int u = 0; u < EPT; ++u) {
+      const int e = threadIdx.x + u * NTHR;
+      const int c = e % C, j = e / C;
+      const int jc = e < TOT ? full_row_to_compact(j, g) : -1;
+      off[u] = jc >= 0 ? jc * g.Kxp + i0 + c : -1;
+    }
+    Cx v[EPT];
+    if (isd) {
+#pragma unroll
+      for (int u = 0; u < EPT; ++u)
+        v[u] = to_c<R>(rsrc_ld<double2>(rs, off[u] >= 0 ? off[u] * (int)sizeof(double2) : kRowOob));
+    } else {
+#pragma unroll
+      for (int u = 0; u < EPT; ++u) v[u] = rsrc_ld<Cx>(rs, off[u] >= 0 ? off[u] * (int)sizeof(Cx) : kRowOob);
+    }
+#pragma unroll
+    for (int u = 0; u < EPT; ++u) {
+      const int e = threadIdx.x + u * NTHR;
+      if (EPT * NTHR != TOT && e >= TOT) continue;
       const int c = e % C, j = e / C, i = i0 + c;
-      const int jc = full_row_to_compact(j, g);
-      Cx v = cmk<Cx>((R)0, (R)0);
-      if (jc >= 0) {
-        const long idx = ((long)z * g.Ky + jc) * g.Kxp + i;
-        if (isd) v = to_c<R>(reinterpret_cast<const double2*>(src)[idx]);
-        else v = reinterpret_cast<const Cx*>(src)[idx];
-        if (mult == 1) {          // -i ky
-          const R ky = (R)ky_of(j, g);
-          v = cmk<Cx>(ky * v.y, -ky * v.x);
-        } else if (mult == 2) {   // +i kx
-          const R kx = (R)kx_of(i, g);
-          v = cmk<Cx>(-kx * v.y, kx * v.x);
-        } else if (mult == 3) {   // +i ky
-          const R ky = (R)ky_of(j, g);
-          v = cmk<Cx>(-ky * v.y, ky * v.x);
-        }
+      Cx x = v[u];
+      if (mult == 1) {          // -i ky
+        const R ky = (R)ky_of(j, g);
+        x = cmk<Cx>(ky * x.y, -ky * x.x);
+      } else if (mult == 2) {   // +i kx
+        const R kx = (R)kx_of(i, g);
+        x = cmk<Cx>(-kx * x.y, kx * x.x);
+      } else if (mult == 3) {   // +i ky
+        const R ky = (R)ky_of(j, g);
+        x = cmk<Cx>(-ky * x.y, ky * x.x);
       }
-      buf[e] = v;
+      buf[e] = x;
     }
     __syncthreads();
     ct_fft<R, NY, L, NTHR>(buf, tw, +1);
@@ -343,13 +363,34 @@ __device__ __forceinline__ void rhs_tile(int b, int total, int ntile, int nzm, i
 //         [- (A2 k2 + A8 k8) q  if is_explicit]  [+ r0 Htot/H_nz k2 psi on layer nz]
 //         [+ C_d Htot/H_nz (i ky du - i kx dv) on layer nz, QGN_Module.f90:284-292]
 // on the kept modes (DeAlias, QGN_Module.f90:294), (0,0) and padding zero.
-// A bottom-layer tile with drag keeps its values in registers (each thread
-// owns at most C Ky / NTHR <= 8 Ky / NY < 6 of them) while |u| u and |u| v
-// go through the same LDS buffer.
+// Each thread keeps its modes' values in registers (at most C Ky / NTHR <=
+// 8 Ky / NY < 6 of them); on a bottom-layer tile with drag, |u| u and |u| v
+// then go through the same LDS buffer.
+// A column tile of a mixed slab into LDS: C columns from i0, NY = TOT / C
+// rows of `stride` complex values; each thread's loads are all issued before
+// its LDS stores.
+template <typename R, int C, int NTHR, int TOT>
+__device__ __forceinline__ void load_cols(cplx<R>* buf, const cplx<R>* __restrict__ src, long stride, int i0) {
+  constexpr int EPT = (TOT + NTHR - 1) / NTHR;
+  constexpr bool EXACT = EPT * NTHR == TOT;
+  cplx<R> v[EPT];
+#pragma unroll
+  for (int u = 0; u < EPT; ++u) {
+    const int e = threadIdx.x + u * NTHR;
+    const int ee = (EXACT || e < TOT) ? e : TOT - 1;
+    v[u] = src[(long)(ee / C) * stride + i0 + ee % C];
+  }
+#pragma unroll
+  for (int u = 0; u < EPT; ++u) {
+    const int e = threadIdx.x + u * NTHR;
+    if (EXACT || e < TOT) buf[e] = v[u];
+  }
+}
+
 template <typename R, typename S, int NY, int C, int NTHR>
 __global__ void __launch_bounds__(NTHR) k_col_rhs_ct(ColRhsArgs a, Geom g, const cplx<R>* __restrict__ twy) {
   using Cx = cplx<R>;
-  constexpr int MAXO = 6;
+  constexpr int MAXO = 6;   // kept modes per thread: C Ky <= 6 NTHR since Ky < 3 ny / 4
   extern __shared__ __align__(16) unsigned char lds_raw[];
   Cx* buf = reinterpret_cast<Cx*>(lds_raw);
   Cx* tw = buf + C * NY;
@@ -360,27 +401,41 @@ __global__ void __launch_bounds__(NTHR) k_col_rhs_ct(ColRhsArgs a, Geom g, const
   rhs_tile(blockIdx.x, gridDim.x, ntile, g.nzm, a.drag, &z, &i0);   // grid = ntile * nz
   i0 *= C;
   load_table<R>(tw, twy, ct_tw_size(NY));
-  const Cx* src = reinterpret_cast<const Cx*>(a.J);
-  for (int e = threadIdx.x; e < TOT; e += NTHR) {
-    const int c = e % C, j = e / C;
-    buf[e] = src[((long)z * NY + j) * g.Kxp + i0 + c];
-  }
+  load_cols<R, C, NTHR, TOT>(buf, reinterpret_cast<const Cx*>(a.J) + (long)z * NY * g.Kxp, g.Kxp, i0);
   __syncthreads();
   ct_fft<R, NY, L, NTHR>(buf, tw, -1);
-  const Cx* psi = reinterpret_cast<const Cx*>(a.psi);
   cplx<S>* out = reinterpret_cast<cplx<S>*>(a.out);
   const R ub = (R)a.ubar[z];
   const R bq = (R)(a.beta + a.qybar[z]);
   const bool bottom = (z % g.nzm == g.nzm - 1);
   const R ek = bottom ? (R)a.ekman[z / g.nzm] : (R)0;
   const int nown = C * g.Ky;
-  auto rhs_at = [&](int o) {
+  // q and psi of the owned modes through raw buffers over the layer's
+  // compact plane (all loads in flight together; lanes past the tile read
+  // zero)
+  const long lay = (long)z * g.Ky * g.Kxp;
+  const int plane = g.Ky * g.Kxp;
+  const __amdgpu_buffer_rsrc_t rq = row_rsrc(a.q + lay, plane * (int)sizeof(double2));
+  const __amdgpu_buffer_rsrc_t rp = row_rsrc(reinterpret_cast<const Cx*>(a.psi) + lay, plane * (int)sizeof(Cx));
+  double2 qv[MAXO];
+  Cx pv[MAXO];
+#pragma unroll
+  for (int u = 0; u < MAXO; ++u) {
+    const int o = threadIdx.x + u * NTHR;
+    const int off = o < nown ? (o / C) * g.Kxp + i0 + o % C : -1;
+    qv[u] = rsrc_ld<double2>(rq, off >= 0 ? off * (int)sizeof(double2) : kRowOob);
+    pv[u] = rsrc_ld<Cx>(rp, off >= 0 ? off * (int)sizeof(Cx) : kRowOob);
+  }
+  Cx keep[MAXO];
+#pragma unroll
+  for (int u = 0; u < MAXO; ++u) {
+    const int o = threadIdx.x + u * NTHR;
+    if (o >= nown) continue;
     const int c = o % C, jc = o / C, i = i0 + c;
     const int j = compact_row_to_full(jc, g);
-    const long idx = ((long)z * g.Ky + jc) * g.Kxp + i;
     const Cx v = buf[j * C + c];
-    const Cx q = to_c<R>(a.q[idx]);
-    const Cx ps = psi[idx];
+    const Cx q = to_c<R>(qv[u]);
+    const Cx ps = pv[u];
     const R kx = (R)kx_of(i, g), ky = (R)ky_of(j, g);
     Cx r = cmk<Cx>(-v.x, -v.y);
     r.x += ub * kx * q.y + bq * kx * ps.y;
@@ -397,50 +452,33 @@ __global__ void __launch_bounds__(NTHR) k_col_rhs_ct(ColRhsArgs a, Geom g, const
       r.x += ek * k2 * ps.x;
       r.y += ek * k2 * ps.y;
     }
-    return r;
-  };
-  if (!(bottom && a.drag)) {
-    for (int o = threadIdx.x; o < nown; o += NTHR) {
-      const int c = o % C, jc = o / C, i = i0 + c;
-      const int j = compact_row_to_full(jc, g);
-      Cx r = rhs_at(o);
-      if ((i == 0 && j == 0) || i >= g.Kx) r = cmk<Cx>((R)0, (R)0);
-      out[((long)z * g.Ky + jc) * g.Kxp + i] = cmk<cplx<S>>((S)r.x, (S)r.y);
-    }
-    return;
+    keep[u] = r;
   }
-  Cx keep[MAXO];
+  if (bottom && a.drag) {
+    const int mb = z / g.nzm;
+    const R cd = (R)a.cdrag[mb];
 #pragma unroll
-  for (int u = 0; u < MAXO; ++u) {
-    const int o = threadIdx.x + u * NTHR;
-    if (o < nown) keep[u] = rhs_at(o);
-  }
-  const int mb = z / g.nzm;
-  const R cd = (R)a.cdrag[mb];
+    for (int pass = 0; pass < 2; ++pass) {   // |u| u, then |u| v
+      const Cx* d = reinterpret_cast<const Cx*>(pass ? a.DV : a.DU) + (long)mb * NY * g.Kxp;
+      __syncthreads();   // the previous contents of buf are read
+      load_cols<R, C, NTHR, TOT>(buf, d, g.Kxp, i0);
+      __syncthreads();
+      ct_fft<R, NY, L, NTHR>(buf, tw, -1);
 #pragma unroll
-  for (int pass = 0; pass < 2; ++pass) {   // |u| u, then |u| v
-    const Cx* d = reinterpret_cast<const Cx*>(pass ? a.DV : a.DU) + (long)mb * NY * g.Kxp;
-    __syncthreads();   // the previous contents of buf are read
-    for (int e = threadIdx.x; e < TOT; e += NTHR) {
-      const int c = e % C, j = e / C;
-      buf[e] = d[(long)j * g.Kxp + i0 + c];
-    }
-    __syncthreads();
-    ct_fft<R, NY, L, NTHR>(buf, tw, -1);
-#pragma unroll
-    for (int u = 0; u < MAXO; ++u) {
-      const int o = threadIdx.x + u * NTHR;
-      if (o >= nown) continue;
-      const int c = o % C, jc = o / C, i = i0 + c;
-      const int j = compact_row_to_full(jc, g);
-      const Cx x = buf[j * C + c];
-      const R kx = (R)kx_of(i, g), ky = (R)ky_of(j, g);
-      if (pass == 0) {   // + cd i ky du
-        keep[u].x -= cd * ky * x.y;
-        keep[u].y += cd * ky * x.x;
-      } else {           // - cd i kx dv
-        keep[u].x += cd * kx * x.y;
-        keep[u].y -= cd * kx * x.x;
+      for (int u = 0; u < MAXO; ++u) {
+        const int o = threadIdx.x + u * NTHR;
+        if (o >= nown) continue;
+        const int c = o % C, jc = o / C, i = i0 + c;
+        const int j = compact_row_to_full(jc, g);
+        const Cx x = buf[j * C + c];
+        const R kx = (R)kx_of(i, g), ky = (R)ky_of(j, g);
+        if (pass == 0) {   // + cd i ky du
+          keep[u].x -= cd * ky * x.y;
+          keep[u].y += cd * ky * x.x;
+        } else {           // - cd i kx dv
+          keep[u].x += cd * kx * x.y;
+          keep[u].y -= cd * kx * x.x;
+        }
       }
     }
   }
@@ -452,7 +490,33 @@ __global__ void __launch_bounds__(NTHR) k_col_rhs_ct(ColRhsArgs a, Geom g, const
     const int j = compact_row_to_full(jc, g);
     Cx r = keep[u];
     if ((i == 0 && j == 0) || i >= g.Kx) r = cmk<Cx>((R)0, (R)0);
-    out[((long)z * g.Ky + jc) * g.Kxp + i] = cmk<cplx<S>>((S)r.x, (S)r.y);
+    out[lay + (long)jc * g.Kxp + i] = cmk<cplx<S>>((S)r.x, (S)r.y);
+  }
+}
+
+// x-c2r pre-processing of ROWS mixed rows (stride complex values apart,
+// Kin kept bins) read from the raw buffer rs into padded LDS rows; all loads
+// of a thread are issued before its LDS stores.
+template <typename R, int N2, int ROWS, int NTHR, int PS>
+__device__ __forceinline__ void c2r_pre_rows(cplx<R>* buf, __amdgpu_buffer_rsrc_t rs, int stride, int Kin,
+                                             const cplx<R>* __restrict__ tr, R scale) {
+  using Cx = cplx<R>;
+  constexpr int TOT = ROWS * N2, EPT = (TOT + NTHR - 1) / NTHR;
+  constexpr bool EXACT = EPT * NTHR == TOT;
+  constexpr int SZ = (int)sizeof(Cx);
+  Cx va[EPT], vb[EPT];
+#pragma unroll
+  for (int u = 0; u < EPT; ++u) {
+    const int e = threadIdx.x + u * NTHR;
+    const bool ok = EXACT || e < TOT;
+    const int r = e / N2, k = e - r * N2, kk = N2 - k;
+    va[u] = rsrc_ld<Cx>(rs, ok && k < Kin ? (r * stride + k) * SZ : kRowOob);
+    vb[u] = rsrc_ld<Cx>(rs, ok && kk < Kin ? (r * stride + kk) * SZ : kRowOob);
+  }
+#pragma unroll
+  for (int u = 0; u < EPT; ++u) {
+    const int e = threadIdx.x + u * NTHR;
+    if (EXACT || e < TOT) buf[padded<PS>(e)] = c2r_pre_ab<R>(va[u], vb[u], e % N2, tr, scale);
   }
 }
 
@@ -479,10 +543,7 @@ __global__ void __launch_bounds__(NTHR) k_row_maxabs_ct(const cplx<R>* __restric
   __syncthreads();
   const long row0 = (long)blockIdx.x * ROWS;
   const __amdgpu_buffer_rsrc_t rs = row_rsrc(mixed + row0 * stride, ROWS * stride * (int)sizeof(Cx));
-  for (int e = threadIdx.x; e < ROWS * N2; e += NTHR) {
-    const int r = e / N2, k = e - r * N2;
-    buf[padded<PS>(e)] = c2r_pre_rs<R>(rs, (long)r * stride, Kin, k, N2, tr, (R)1);
-  }
+  c2r_pre_rows<R, N2, ROWS, NTHR, PS>(buf, rs, stride, Kin, tr, (R)1);
   __syncthreads();
   ct_fft<R, N2, CtLayout<ROWS, N2, 1, false, PS>, NTHR>(buf, ts, +1);
   double m = 0.0;
@@ -523,10 +584,7 @@ __global__ void __launch_bounds__(NTHR) k_row_c2r_ct(const cplx<R>* __restrict__
   __syncthreads();
   const long row0 = (long)blockIdx.x * ROWS;
   const __amdgpu_buffer_rsrc_t rs = row_rsrc(mixed + row0 * stride, ROWS * stride * (int)sizeof(Cx));
-  for (int e = threadIdx.x; e < ROWS * N2; e += NTHR) {
-    const int r = e / N2, k = e - r * N2;
-    buf[padded<PS>(e)] = c2r_pre_rs<R>(rs, (long)r * stride, Kin, k, N2, tr, scale);
-  }
+  c2r_pre_rows<R, N2, ROWS, NTHR, PS>(buf, rs, stride, Kin, tr, scale);
   __syncthreads();
   ct_fft<R, N2, CtLayout<ROWS, N2, 1, false, PS>, NTHR>(buf, ts, +1);
   Cx* o = reinterpret_cast<Cx*>(out) + row0 * N2;

@@ -604,8 +604,8 @@ int launch_col_rhs(qgn_ctx* c, const ColRhsArgs& ka0, bool drag) {
   ka.DU = c->DUm; ka.DV = c->DVm;
   ka.cdrag = c->cdm;
   ka.drag = drag;
-  // each thread of a drag tile holds at most C Ky / NTHR values, NTHR >= C ny / 8
-  if (drag && 8L * g.Ky > 6L * g.ny) return fail(QGN_EINVAL, "drag tile: too many kept rows");
+  // each thread holds at most C Ky / NTHR values, NTHR >= C ny / 8
+  if (8L * g.Ky > 6L * g.ny) return fail(QGN_EINVAL, "column tile: too many kept rows");
   int rc = 0;
   with_size(g.ny, [&](auto nyc) {
     constexpr int NY = decltype(nyc)::value;
