@@ -125,8 +125,8 @@ __global__ void k_project(const double2* __restrict__ q, cplx<Tout>* __restrict_
       acc.x += w * v.x;
       acc.y += w * v.y;
     }
-    const int i = (int)(pt % g.Kxp);
-    const int j = compact_row_to_full((int)(pt / g.Kxp), g);
+    const int i = (int)pt % g.Kxp;
+    const int j = compact_row_to_full((int)pt / g.Kxp, g);
     const double kx = kx_of(i, g), ky = ky_of(j, g);
     const double den = -(kx * kx + ky * ky) + evals[k];
     double2 pm = make_double2(acc.x * inv_htot / den, acc.y * inv_htot / den);
@@ -292,8 +292,8 @@ __device__ __forceinline__ double2 stage_value(const StageArgs& a, const Geom& g
   }
   double mq = 1.0;
   if (a.implicit_visc) {
-    const int i = (int)(pt % g.Kxp);
-    const int j = compact_row_to_full((int)(pt / g.Kxp), g);
+    const int i = (int)pt % g.Kxp;
+    const int j = compact_row_to_full((int)pt / g.Kxp, g);
     const double kx = kx_of(i, g), ky = ky_of(j, g);
     const double k2 = kx * kx + ky * ky;
     const double k4 = k2 * k2;
@@ -351,8 +351,8 @@ __global__ void __launch_bounds__(256) k_stage_proj(StageArgs a, Geom g, const d
   int i = 0, j = 0;
   double k2 = 0.0;
   if (active) {
-    i = (int)(pt % g.Kxp);
-    j = compact_row_to_full((int)(pt / g.Kxp), g);
+    i = (int)pt % g.Kxp;
+    j = compact_row_to_full((int)pt / g.Kxp, g);
     const double kx = kx_of(i, g), ky = ky_of(j, g);
     k2 = kx * kx + ky * ky;
     const long e = kl * plane + pt;
@@ -486,8 +486,8 @@ __global__ void __launch_bounds__(256) k_stage_proj_mfma16(StageArgs a, Geom g, 
   bool zero_mode = pt >= plane;
   double k2 = 0.0;
   if (!zero_mode) {
-    const int i = (int)(pt % g.Kxp);
-    const int j = compact_row_to_full((int)(pt / g.Kxp), g);
+    const int i = (int)pt % g.Kxp;
+    const int j = compact_row_to_full((int)pt / g.Kxp, g);
     const double kx = kx_of(i, g), ky = ky_of(j, g);
     k2 = kx * kx + ky * ky;
     zero_mode = (i == 0 && j == 0) || i >= g.Kx;
@@ -495,7 +495,7 @@ __global__ void __launch_bounds__(256) k_stage_proj_mfma16(StageArgs a, Geom g, 
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const int mode = kb + r;
-    const float pm = (zero_mode || mode >= nz) ? 0.f : (float)((double)d[r] / (-k2 + ev[r]));
+    const float pm = (zero_mode || mode >= nz) ? 0.f : d[r] / (float)(-k2 + ev[r]);
     bh[r] = bf16_rne(pm);
     bl[r] = bf16_lo(pm, bh[r]);
   }
@@ -603,8 +603,8 @@ __global__ void __launch_bounds__(256) k_stage_proj_mfma(StageArgs a, Geom g, co
     bool zero_mode = pt >= plane;
     double k2 = 0.0;
     if (!zero_mode) {
-      const int i = (int)(pt % g.Kxp);
-      const int j = compact_row_to_full((int)(pt / g.Kxp), g);
+      const int i = (int)pt % g.Kxp;
+      const int j = compact_row_to_full((int)pt / g.Kxp, g);
       const double kx = kx_of(i, g), ky = ky_of(j, g);
       k2 = kx * kx + ky * ky;
       zero_mode = (i == 0 && j == 0) || i >= g.Kx;
@@ -616,7 +616,7 @@ __global__ void __launch_bounds__(256) k_stage_proj_mfma(StageArgs a, Geom g, co
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int mode = 16 * mi + kb + r;
-        const float pm = (zero_mode || mode >= nz) ? 0.f : (float)((double)d[mi][r] / (-k2 + ev[mi][r]));
+        const float pm = (zero_mode || mode >= nz) ? 0.f : d[mi][r] / (float)(-k2 + ev[mi][r]);
         ph[mi][r] = bf16_rne(pm);
         pl[mi][r] = bf16_lo(pm, ph[mi][r]);
       }
@@ -667,8 +667,8 @@ __global__ void k_update(UpdateArgs a, Geom g) {
     const double dt = a.dtv ? a.dtv[z / g.nzm] : a.dt;
     double lam = 0.0;
     if (a.implicit_visc) {
-      const int i = (int)(pt % g.Kxp);
-      const int j = compact_row_to_full((int)(pt / g.Kxp), g);
+      const int i = (int)pt % g.Kxp;
+      const int j = compact_row_to_full((int)pt / g.Kxp, g);
       const double kx = kx_of(i, g), ky = ky_of(j, g);
       const double k2 = kx * kx + ky * ky;
       const double k4 = k2 * k2;
@@ -755,9 +755,9 @@ __global__ void k_leith_partial(const double2* __restrict__ q, Geom g, double* _
   const long plane = (long)g.Ky * g.Kxp;
   double s = 0.0;
   for (long pt = blockIdx.x * (long)blockDim.x + threadIdx.x; pt < plane; pt += (long)nblk * blockDim.x) {
-    const int i = (int)(pt % g.Kxp);
+    const int i = (int)pt % g.Kxp;
     if (i == 0 || i >= g.Kx) continue;
-    const int j = compact_row_to_full((int)(pt / g.Kxp), g);
+    const int j = compact_row_to_full((int)pt / g.Kxp, g);
     const double kx = kx_of(i, g), ky = ky_of(j, g);
     const double k2 = kx * kx + ky * ky;
     const double k4 = k2 * k2;
@@ -1041,9 +1041,9 @@ __global__ void k_spectra(const double2* __restrict__ q, const double2* __restri
   const double2* qz = q + z * plane;
   const double2* pz = psi + z * plane;
   for (long pt = blockIdx.x * (long)blockDim.x + threadIdx.x; pt < plane; pt += (long)gridDim.x * blockDim.x) {
-    const int i = (int)(pt % g.Kxp);
+    const int i = (int)pt % g.Kxp;
     if (i == 0 || i >= g.Kx) continue;
-    const int j = compact_row_to_full((int)(pt / g.Kxp), g);
+    const int j = compact_row_to_full((int)pt / g.Kxp, g);
     const double kx = kx_of(i, g), ky = ky_of(j, g);
     const double k2 = kx * kx + ky * ky;
     const int b = (int)floor(sqrt(k2) / dk + 0.5);
