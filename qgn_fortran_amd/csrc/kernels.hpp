@@ -42,6 +42,13 @@ __device__ __forceinline__ double ky_of(int j, const Geom& g) {
   return g.dky * (double)(j <= g.ny / 2 ? j : j - g.ny);
 }
 
+// The same in the pass's arithmetic type (a single rounding of dkx, dky to
+// FP32 on the mixed path instead of FP64 products converted per element).
+template <typename R> __device__ __forceinline__ R kx_r(int i, const Geom& g) { return (R)g.dkx * (R)i; }
+template <typename R> __device__ __forceinline__ R ky_r(int j, const Geom& g) {
+  return (R)g.dky * (R)(j <= g.ny / 2 ? j : j - g.ny);
+}
+
 template <typename T> __device__ __forceinline__ cplx<T> to_c(double2 a) { return cmk<cplx<T>>((T)a.x, (T)a.y); }
 template <typename T> __device__ __forceinline__ cplx<T> to_c(float2 a) { return cmk<cplx<T>>((T)a.x, (T)a.y); }
 

@@ -80,13 +80,13 @@ __global__ void __launch_bounds__(NTHR) k_col_inv_ct(ColInvArgs a, Geom g, const
       const int c = e % C, j = e / C, i = i0 + c;
       Cx x = v[u];
       if (mult == 1) {          // -i ky
-        const R ky = (R)ky_of(j, g);
+        const R ky = ky_r<R>(j, g);
         x = cmk<Cx>(ky * x.y, -ky * x.x);
       } else if (mult == 2) {   // +i kx
-        const R kx = (R)kx_of(i, g);
+        const R kx = kx_r<R>(i, g);
         x = cmk<Cx>(-kx * x.y, kx * x.x);
       } else if (mult == 3) {   // +i ky
-        const R ky = (R)ky_of(j, g);
+        const R ky = ky_r<R>(j, g);
         x = cmk<Cx>(-ky * x.y, ky * x.x);
       }
       buf[e] = x;
@@ -436,7 +436,7 @@ __global__ void __launch_bounds__(NTHR) k_col_rhs_ct(ColRhsArgs a, Geom g, const
     const Cx v = buf[j * C + c];
     const Cx q = to_c<R>(qv[u]);
     const Cx ps = pv[u];
-    const R kx = (R)kx_of(i, g), ky = (R)ky_of(j, g);
+    const R kx = kx_r<R>(i, g), ky = ky_r<R>(j, g);
     Cx r = cmk<Cx>(-v.x, -v.y);
     r.x += ub * kx * q.y + bq * kx * ps.y;
     r.y -= ub * kx * q.x + bq * kx * ps.x;
@@ -471,7 +471,7 @@ __global__ void __launch_bounds__(NTHR) k_col_rhs_ct(ColRhsArgs a, Geom g, const
         const int c = o % C, jc = o / C, i = i0 + c;
         const int j = compact_row_to_full(jc, g);
         const Cx x = buf[j * C + c];
-        const R kx = (R)kx_of(i, g), ky = (R)ky_of(j, g);
+        const R kx = kx_r<R>(i, g), ky = ky_r<R>(j, g);
         if (pass == 0) {   // + cd i ky du
           keep[u].x -= cd * ky * x.y;
           keep[u].y += cd * ky * x.x;
