@@ -34,8 +34,8 @@ def kernel_bytes(kernel, g, rs):
         return 5 * nmix * rs + drag
     if kernel == "col_inv":     # read q, psi; write u, v, q_x, q_y
         return ncmp * (16 + rs) + 4 * nmix * rs
-    if kernel == "col_rhs":     # read J, q, psi; write N
-        return nmix * rs + ncmp * (16 + 2 * rs)
+    if kernel == "col_rhs":     # read J, q, psi; write N (+ read |u|u, |u|v for layer nz)
+        return nmix * rs + ncmp * (16 + 2 * rs) + drag
     raise ValueError(kernel)
 
 
