@@ -141,7 +141,6 @@ __global__ void __launch_bounds__(NTHR) k_row_jac_ct(RowJacArgs a, Geom g, const
   const long Kxp = g.Kxp;
   const Cx* srcs[4] = {reinterpret_cast<const Cx*>(a.U), reinterpret_cast<const Cx*>(a.V),
                        reinterpret_cast<const Cx*>(a.QX), reinterpret_cast<const Cx*>(a.QY)};
-  const Cx zero = cmk<Cx>((R)0, (R)0);
   Cx ra[4][EL], rb[4][EL];
   auto fetch = [&](int grp) {
     const long row0 = (long)grp * ROWS;
