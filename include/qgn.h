@@ -231,7 +231,11 @@ int qgn_cnn_nparams(qgn_cnn* h, int* n);
 /* host copies of the parameters and of the last gradients (either may be NULL) */
 int qgn_cnn_get_params(qgn_cnn* h, float* params, float* grads);
 int qgn_cnn_set_params(qgn_cnn* h, const float* params);
-/* device pointers: x bf16 bit patterns (n, H, W, cin), y fp32 (n, H, W, cout) */
+/* device pointers: x bf16 bit patterns (n, H, W, cin), y fp32 (n, H, W, cout).
+ * The network's kernels run on its own stream, ordered with no other
+ * stream: device inputs (x, target) must be complete when a call is made
+ * (synchronise the stream that wrote them), and each call has finished
+ * its device work when it returns. */
 int qgn_cnn_forward(qgn_cnn* h, const void* x, int n, int H, int W, float* y);
 /* gradients of the mean-squared error of the last forward's y against the
  * device array target (n, H, W, cout); *loss receives the error */

@@ -56,13 +56,23 @@ class CNN:
         n, H, W, _ = x.shape
         return n, H, W
 
+    @staticmethod
+    def _ready(t):
+        """The library runs on its own stream: let the work torch queued on
+        the tensor's current stream (which may still be writing it) finish
+        first.  The library synchronises its own stream before returning."""
+        import torch
+        torch.cuda.current_stream(t.device).synchronize()
+
     def forward(self, x, y):
         """x: int16 CUDA tensor (n, H, W, cin); y: float32 CUDA tensor (n, H, W, cout)."""
         n, H, W = self._shape(x)
+        self._ready(x)
         check_cnn(self.lib.qgn_cnn_forward(self._h, C.c_void_p(x.data_ptr()), n, H, W, C.c_void_p(y.data_ptr())))
 
     def backward(self, target):
         loss = C.c_double()
+        self._ready(target)
         check_cnn(self.lib.qgn_cnn_backward(self._h, C.c_void_p(target.data_ptr()), C.byref(loss)))
         return loss.value
 
@@ -72,6 +82,7 @@ class CNN:
     def train_step(self, x, target, y, lr):
         n, H, W = self._shape(x)
         loss = C.c_double()
+        self._ready(x)
         check_cnn(self.lib.qgn_cnn_train_step(self._h, C.c_void_p(x.data_ptr()), C.c_void_p(target.data_ptr()),
                                               n, H, W, C.c_void_p(y.data_ptr()), lr, C.byref(loss)))
         return loss.value

@@ -154,6 +154,137 @@ __global__ void __launch_bounds__(256) k_conv3x3(const u16* __restrict__ in, con
   }
 }
 
+// The same convolution on 2-D tiles with the input window staged in LDS.
+// A workgroup computes a tile of TH x TW pixels (TH TW = 64 TP) of one
+// image, each wave TP rows of 16 pixels; the (TH + 2) x (TW + 2) input
+// window with its periodic halo is loaded once with 16-byte loads, all of a
+// thread's loads in flight together, so each input pixel is read about
+// (TH + 2)(TW + 2) / (TH TW) times instead of 3.  The MFMA B operands are
+// read from LDS at a per-lane base plus a constant offset per tap.  A staged
+// pixel holds CI + 4 values, so the 16 pixels of an operand fall on
+// distinct banks.  Needs TH | H and TW | W.
+template <int CI, int CO, int MODE, int TP, int TW>
+__global__ void __launch_bounds__(256) k_conv3x3_tile(const u16* __restrict__ in, const u16* __restrict__ wpk,
+                                                      const float* __restrict__ bias, const u16* __restrict__ mask,
+                                                      void* __restrict__ out, int H, int W) {
+  constexpr int COP = pad16(CO);
+  constexpr int NW = COP * 9 * CI;
+  constexpr int TH = 64 * TP / TW;
+  constexpr int WW = TW + 2, NR = TH + 2;
+  constexpr int PS = CI + 4;            // bf16 values per staged pixel
+  constexpr int CH = CI / 8;            // 16-byte chunks per pixel
+  constexpr int NCH = NR * WW * CH;
+  constexpr int MAXC = (NCH + 255) / 256;
+  static_assert(TW % 16 == 0 && (64 * TP) % TW == 0, "tile of whole 16-pixel rows");
+  __shared__ __align__(16) u16 ws[NW];
+  __shared__ __align__(16) u16 xs[NR * WW * PS];
+  const int tx_n = W / TW, ty_n = H / TH;
+  const int img = blockIdx.x / (tx_n * ty_n);
+  const int trem = blockIdx.x - img * tx_n * ty_n;
+  const int y0 = (trem / tx_n) * TH, x0 = (trem % tx_n) * TW;
+  const long imgp = (long)img * H * W;   // first pixel of the image
+  // the image as a raw buffer: lanes past the window load zero, no branch
+  const __amdgpu_buffer_rsrc_t rs =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<u16*>(in + imgp * CI), (short)0, H * W * CI * 2, 0x00020000);
+  typedef int i4v __attribute__((ext_vector_type(4)));
+  i4v v[MAXC];
+#pragma unroll
+  for (int u = 0; u < MAXC; ++u) {
+    const int c = threadIdx.x + u * 256;
+    const int pix = c / CH, sub = c - pix * CH;
+    const int r = pix / WW, cc = pix - r * WW;
+    int yy = y0 - 1 + r;
+    yy += yy < 0 ? H : 0;
+    yy -= yy >= H ? H : 0;
+    int xx = x0 - 1 + cc;
+    xx += xx < 0 ? W : 0;
+    xx -= xx >= W ? W : 0;
+    const int off = c < NCH ? ((yy * W + xx) * CI + sub * 8) * 2 : 0x7ffffff0;
+    v[u] = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
+  }
+  for (int e = threadIdx.x * 8; e < NW; e += 256 * 8)
+    *reinterpret_cast<uint4*>(&ws[e]) = *reinterpret_cast<const uint4*>(&wpk[e]);
+#pragma unroll
+  for (int u = 0; u < MAXC; ++u) {
+    const int c = threadIdx.x + u * 256;
+    if (c >= NCH) continue;
+    const int pix = c / CH, sub = c - pix * CH;
+    uint2* d = reinterpret_cast<uint2*>(xs + pix * PS + sub * 8);   // 8-byte aligned
+    d[0] = make_uint2((unsigned)v[u].x, (unsigned)v[u].y);
+    d[1] = make_uint2((unsigned)v[u].z, (unsigned)v[u].w);
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int j = lane & 15, g = lane >> 4;
+  // tile k = w TP + u: row k / (TW / 16) of the workgroup tile, 16 pixels
+  // from column 16 (k % (TW / 16)); window position (row + 1, column + 1)
+  const u16* base[TP];
+#pragma unroll
+  for (int u = 0; u < TP; ++u) {
+    const int k = w * TP + u;
+    const int tr = k / (TW / 16), tc = (k % (TW / 16)) * 16 + j;
+    base[u] = xs + ((tr + 1) * WW + tc + 1) * PS + 4 * g;
+  }
+  f4v acc[TP][COP / 16];
+#pragma unroll
+  for (int u = 0; u < TP; ++u)
+#pragma unroll
+    for (int mt = 0; mt < COP / 16; ++mt) acc[u][mt] = f4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int tap = 0; tap < 9; ++tap) {
+    const int toff = ((tap / 3 - 1) * WW + (tap % 3 - 1)) * PS;
+#pragma unroll
+    for (int kc = 0; kc < CI / 16; ++kc) {
+      s4v b[TP];
+#pragma unroll
+      for (int u = 0; u < TP; ++u) b[u] = *reinterpret_cast<const s4v*>(base[u] + toff + kc * 16);
+#pragma unroll
+      for (int mt = 0; mt < COP / 16; ++mt) {
+        const s4v a = *reinterpret_cast<const s4v*>(&ws[((mt * 16 + j) * 9 + tap) * CI + kc * 16 + 4 * g]);
+#pragma unroll
+        for (int u = 0; u < TP; ++u) acc[u][mt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[u], acc[u][mt], 0, 0, 0);
+      }
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < TP; ++u) {
+    const int k = w * TP + u;
+    const int tr = k / (TW / 16), tc = (k % (TW / 16)) * 16 + j;
+    const long p = imgp + (long)(y0 + tr) * W + x0 + tc;
+#pragma unroll
+    for (int mt = 0; mt < COP / 16; ++mt) {
+      const int co = mt * 16 + 4 * g;
+      if (co >= CO) continue;               // CO is a multiple of 4
+      if constexpr (MODE == 1) {
+        float4 o;
+        o.x = acc[u][mt][0] + bias[co];
+        o.y = acc[u][mt][1] + bias[co + 1];
+        o.z = acc[u][mt][2] + bias[co + 2];
+        o.w = acc[u][mt][3] + bias[co + 3];
+        *reinterpret_cast<float4*>(reinterpret_cast<float*>(out) + p * CO + co) = o;
+      } else {
+        float f[4];
+        if constexpr (MODE == 0) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) f[r] = fmaxf(acc[u][mt][r] + bias[co + r], 0.f);
+        } else {
+          const ushort4 m = *reinterpret_cast<const ushort4*>(mask + p * CO + co);
+          f[0] = bf2f(m.x) > 0.f ? acc[u][mt][0] : 0.f;
+          f[1] = bf2f(m.y) > 0.f ? acc[u][mt][1] : 0.f;
+          f[2] = bf2f(m.z) > 0.f ? acc[u][mt][2] : 0.f;
+          f[3] = bf2f(m.w) > 0.f ? acc[u][mt][3] : 0.f;
+        }
+        ushort4 o;
+        o.x = f2bf(f[0]);
+        o.y = f2bf(f[1]);
+        o.z = f2bf(f[2]);
+        o.w = f2bf(f[3]);
+        *reinterpret_cast<ushort4*>(reinterpret_cast<u16*>(out) + p * CO + co) = o;
+      }
+    }
+  }
+}
+
 // Weight and bias gradients of one layer:
 //   dW[co][tap][ci] += sum_p dY[p][co] X[p + off(tap)][ci],  db[co] += sum_p dY[p][co]
 // X: (n, H, W, CI) bf16 (layer input), dY: (n, H, W, COP) bf16 (gradient of
@@ -369,6 +500,20 @@ int ensure(qgn_cnn* h, long npix) {
 template <int CI, int CO, int MODE>
 int conv(const u16* in, const u16* w, const float* b, const u16* mask, void* out, int n, int H, int W, hipStream_t s) {
   constexpr int TP = 4;   // pixel tiles per wave
+  // 2-D tiles with the input window in LDS: 16 x 16 pixels where they
+  // divide the image, else 8 x 32
+  if (H % 16 == 0 && W % 16 == 0 && (long)H * W * CI * 2 < 0x7ffffff0L) {
+    hipLaunchKernelGGL((k_conv3x3_tile<CI, CO, MODE, TP, 16>), dim3((unsigned)((long)n * (H / 16) * (W / 16))),
+                       dim3(256), 0, s, in, w, b, mask, out, H, W);
+    CCHK(hipGetLastError());
+    return 0;
+  }
+  if (H % 8 == 0 && W % 32 == 0 && (long)H * W * CI * 2 < 0x7ffffff0L) {
+    hipLaunchKernelGGL((k_conv3x3_tile<CI, CO, MODE, TP, 32>), dim3((unsigned)((long)n * (H / 8) * (W / 32))),
+                       dim3(256), 0, s, in, w, b, mask, out, H, W);
+    CCHK(hipGetLastError());
+    return 0;
+  }
   const long ntiles = (long)n * H * W / 16;
   const long blocks = (ntiles + 4 * TP - 1) / (4 * TP);
   hipLaunchKernelGGL((k_conv3x3<CI, CO, MODE, TP>), dim3((unsigned)blocks), dim3(256), 0, s, in, w, b, mask, out, H,
