@@ -134,3 +134,28 @@ def test_closure_forcing_in_rhs(precision):
     gm.close()
     g2.close()
     net.close()
+
+
+def test_inputs_written_by_torch_just_before_the_call():
+    """The network's kernels run on the library's own stream: inputs that a
+    torch kernel is still writing when the call is made (an index_select of
+    a batch, as the closure's training does) must be complete before the
+    library reads them.  Forward on a freshly gathered batch equals forward
+    on the same batch after a full synchronisation, every time."""
+    import torch
+    from oracle import cnn_oracle as co
+    from qgn_fortran_amd.cnn import CNN, to_bf16_bits
+    cin, cout, n, H, W = 16, 8, 10, 256, 256
+    rng = np.random.default_rng(11)
+    pool = _dev(to_bf16_bits(rng.standard_normal((3 * n, H, W, cin)).astype(np.float32)), torch.int16)
+    net = CNN(cin, cout, co.init_params(cin, cout, 9, last_scale=1.0))
+    y1 = torch.empty((n, H, W, cout), device="cuda", dtype=torch.float32)
+    y2 = torch.empty_like(y1)
+    for trial in range(4):
+        sel = torch.from_numpy(np.sort(rng.permutation(3 * n)[:n])).cuda()
+        xs = pool.index_select(0, sel)        # queued on torch's stream
+        net.forward(xs, y1)
+        torch.cuda.synchronize()
+        net.forward(xs, y2)
+        assert torch.equal(y1, y2), trial
+    net.close()
