@@ -999,31 +999,49 @@ __global__ void k_batch_commit(double2* __restrict__ q, const double2* __restric
 // q_1..q_nzm then psi_1..psi_nzm, each times its scale; and the network
 // output (member, y, x, layer) times its scale -> planar physical forcing.
 // ---------------------------------------------------------------------------
-template <typename R>
+template <typename R, int NZM>
 __global__ void k_closure_in(const R* __restrict__ q, const R* __restrict__ psi,
                              const double* __restrict__ scale, short* __restrict__ x, Geom g) {
-  const int C = 2 * g.nzm;
-  const long plane = (long)g.ny * g.nx;
-  const long n = plane * (g.nz / g.nzm) * C;
-  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
-    const int c = (int)(e % C);
-    const long pm = e / C;               // member * plane + pixel
-    const long m = pm / plane, pix = pm - m * plane;
-    const int layer = (int)m * g.nzm + (c < g.nzm ? c : c - g.nzm);
-    const double v = (double)(c < g.nzm ? q : psi)[layer * plane + pix] * scale[c];
-    x[e] = bf16_rne((float)v);
+  constexpr int C = 2 * NZM;               // channels: q then psi of each layer
+  const int plane = g.ny * g.nx;
+  const int npix = plane * (g.nz / NZM);   // member * plane + pixel
+  for (int pm = blockIdx.x * blockDim.x + threadIdx.x; pm < npix; pm += gridDim.x * blockDim.x) {
+    const int m = pm / plane, pix = pm - m * plane;
+    // each layer read is coalesced across the threads (consecutive pixels)
+    unsigned w[C / 2];
+#pragma unroll
+    for (int c = 0; c < C; c += 2) {
+      const R* f0 = c < NZM ? q : psi;
+      const int l0 = c < NZM ? c : c - NZM;   // c and c + 1 are in the same field (NZM even)
+      const long i0 = (long)(m * NZM + l0) * plane + pix;
+      const short h0 = bf16_rne((float)((double)f0[i0] * scale[c]));
+      const short h1 = bf16_rne((float)((double)f0[i0 + plane] * scale[c + 1]));
+      w[c / 2] = (unsigned)(unsigned short)h0 | ((unsigned)(unsigned short)h1 << 16);
+    }
+    uint4* xo = reinterpret_cast<uint4*>(x + (long)pm * C);   // C / 8 16-byte stores
+#pragma unroll
+    for (int k = 0; k < C / 8; ++k) xo[k] = make_uint4(w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]);
   }
 }
 
-template <typename R>
+template <typename R, int NZM>
 __global__ void k_closure_out(const float* __restrict__ y, const double* __restrict__ scale,
                               R* __restrict__ f, Geom g) {
-  const long plane = (long)g.ny * g.nx;
-  const long n = plane * g.nz;
-  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
-    const long layer = e / plane, pix = e - layer * plane;
-    const int m = (int)(layer / g.nzm), c = (int)(layer - (long)m * g.nzm);
-    f[e] = (R)((double)y[((long)m * plane + pix) * g.nzm + c] * scale[c]);
+  const int plane = g.ny * g.nx;
+  const int npix = plane * (g.nz / NZM);
+  for (int pm = blockIdx.x * blockDim.x + threadIdx.x; pm < npix; pm += gridDim.x * blockDim.x) {
+    const int m = pm / plane, pix = pm - m * plane;
+    const float4* yi = reinterpret_cast<const float4*>(y + (long)pm * NZM);   // NZM / 4 16-byte loads
+#pragma unroll
+    for (int k = 0; k < NZM / 4; ++k) {
+      const float4 v = yi[k];
+      const float vv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int c = 4 * k + r;
+        f[(long)(m * NZM + c) * plane + pix] = (R)((double)vv[r] * scale[c]);
+      }
+    }
   }
 }
 

@@ -750,11 +750,20 @@ int closure_add(qgn_ctx* c, const double2* qin, void* out) {
   R* pp = reinterpret_cast<R*>(c->cl_pp);
   CHK((launch_row_tx<R, true>(c, c->Um, pq, nn)));
   CHK((launch_row_tx<R, true>(c, c->Vm, pp, nn)));
-  hipLaunchKernelGGL((k_closure_in<R>), dim3(grid1d(2 * np)), dim3(256), 0, s, pq, pp, c->cl_in, c->cl_x, g);
+  const long npx = np / g.nzm;   // members x pixels: one thread each
+  if (g.nzm == 8)
+    hipLaunchKernelGGL((k_closure_in<R, 8>), dim3(grid1d(npx)), dim3(256), 0, s, pq, pp, c->cl_in, c->cl_x, g);
+  else if (g.nzm == 16)
+    hipLaunchKernelGGL((k_closure_in<R, 16>), dim3(grid1d(npx)), dim3(256), 0, s, pq, pp, c->cl_in, c->cl_x, g);
+  else
+    return fail(QGN_EINVAL, "closure network: nz must be 8 or 16");
   HIPCHK(hipGetLastError());
   if (qcnn::forward(c->net, reinterpret_cast<const unsigned short*>(c->cl_x), c->nm, g.ny, g.nx, c->cl_y, s))
     return fail(QGN_EINVAL, std::string("closure network: ") + qcnn::last_error());
-  hipLaunchKernelGGL((k_closure_out<R>), dim3(grid1d(np)), dim3(256), 0, s, c->cl_y, c->cl_out, pq, g);
+  if (g.nzm == 8)
+    hipLaunchKernelGGL((k_closure_out<R, 8>), dim3(grid1d(npx)), dim3(256), 0, s, c->cl_y, c->cl_out, pq, g);
+  else
+    hipLaunchKernelGGL((k_closure_out<R, 16>), dim3(grid1d(npx)), dim3(256), 0, s, c->cl_y, c->cl_out, pq, g);
   HIPCHK(hipGetLastError());
   CHK((launch_row_tx<R, false>(c, pq, c->Jm, (R)1)));
   return launch_col_fwd_add<R, S>(c, c->Jm, out);
