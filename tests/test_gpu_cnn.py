@@ -39,7 +39,10 @@ def _blocks(v, cin, cout):
     return out
 
 
-@pytest.mark.parametrize("cin,cout,n,H,W", [(16, 8, 2, 32, 48), (32, 16, 1, 24, 64), (16, 8, 1, 256, 256)])
+# 16 x 16 tiles (32 x 48, 256 x 256), 8 x 32 tiles (24 x 64), and the
+# per-pixel kernel where neither tile divides the image (20 x 48)
+@pytest.mark.parametrize("cin,cout,n,H,W", [(16, 8, 2, 32, 48), (32, 16, 1, 24, 64), (16, 8, 1, 256, 256),
+                                            (16, 8, 1, 20, 48)])
 def test_forward_backward_adam_vs_oracle(cin, cout, n, H, W):
     import torch
     from oracle import cnn_oracle as co
