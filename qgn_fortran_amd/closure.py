@@ -53,9 +53,9 @@ class Setup:
     k_peak: float = 16.0
     rms: float = 1.0e-6
     precision: str = "mixed"
-    epochs: int = 200
+    epochs: int = 2000
     batch: int = 10
-    lr: float = 2.0e-3
+    lr: float = 1.0e-3
     seed: int = 5
     filter_cells: float = 2.0    # Gaussian filter width in coarse cells (0: sharp truncation)
 
@@ -360,7 +360,8 @@ def main(argv=None):
     ap.add_argument("--snapshots", type=int, default=50)
     ap.add_argument("--every", type=int, default=25)
     ap.add_argument("--spinup", type=int, default=3000)
-    ap.add_argument("--epochs", type=int, default=200)
+    ap.add_argument("--epochs", type=int, default=2000)
+    ap.add_argument("--lr", type=float, default=1.0e-3)
     ap.add_argument("--filter-cells", type=float, default=2.0)
     ap.add_argument("--out", default=None)
     a = ap.parse_args(argv)
@@ -368,7 +369,7 @@ def main(argv=None):
     if not torch.cuda.is_available():     # device memory for training comes from torch
         raise SystemExit("no GPU")
     st = Setup(n_hi=a.n_hi, n_lo=a.n_lo, snapshots=a.snapshots, every=a.every, spinup=a.spinup, epochs=a.epochs,
-               filter_cells=a.filter_cells)
+               lr=a.lr, filter_cells=a.filter_cells)
     res = run(st, log=lambda s: print(s, file=sys.stderr, flush=True))
     line = json.dumps(res)
     print(line, flush=True)
