@@ -22,9 +22,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 # Algorithmic HBM bytes per launch of the RHS passes, counting only the kept
-# (unpadded) modes: rs = bytes of one complex of the pass's arithmetic type,
-# 16 for the FP64 compact state q.  ncmp = nz*Ky*Kx compact modes,
-# nmix = nz*ny*Kx mixed values.
+# (unpadded) modes: rs = bytes of one complex of the pass's arithmetic type;
+# the stage value q is read as the projection kernel's FP32 copy (rs bytes)
+# on the mixed path and as the FP64 state (16 bytes) on the FP64 path.
+# ncmp = nz*Ky*Kx compact modes, nmix = nz*ny*Kx mixed values.
 def kernel_bytes(kernel, g, rs):
     nz, ny, Kx, Ky = g["nz"], g["ny"], g["Kx"], g["Ky"]
     ncmp = nz * Ky * Kx
@@ -33,9 +34,9 @@ def kernel_bytes(kernel, g, rs):
     if kernel == "row_jac":     # read u, v, q_x, q_y; write J (+ |u|u, |u|v on layer nz)
         return 5 * nmix * rs + drag
     if kernel == "col_inv":     # read q, psi; write u, v, q_x, q_y
-        return ncmp * (16 + rs) + 4 * nmix * rs
+        return 2 * ncmp * rs + 4 * nmix * rs
     if kernel == "col_rhs":     # read J, q, psi; write N (+ read |u|u, |u|v for layer nz)
-        return nmix * rs + ncmp * (16 + 2 * rs) + drag
+        return nmix * rs + 3 * ncmp * rs + drag
     raise ValueError(kernel)
 
 

@@ -184,6 +184,7 @@ struct RowJacArgs {
 struct ColRhsArgs {
   const void* J;         // mixed cplx<R>
   const double2* q;      // compact
+  const void* q32;       // compact cplx<R>, q rounded to R (null: read q)
   const void* psi;       // compact cplx<R>
   void* out;             // compact cplx<S>
   const double* ubar; const double* qybar;   // per layer
@@ -270,6 +271,7 @@ struct StageArgs {
   double A8, A2;
   int implicit_visc;      // use Mq and L_j terms
   double2* out;
+  float2* out32;          // MFMA projection kernels: FP32 copy of the stage value (may be null)
 };
 
 // ARK stage value of element e (layer kl of the batch, point pt of its
@@ -424,19 +426,20 @@ __device__ __forceinline__ qgn_float4 mfma3(const qgn_short4& ah, const qgn_shor
 }
 
 // Stage combination (or, with a.ns = 0 and a.out = nullptr, a plain copy of
-// q) followed by the MFMA projection, nz <= 16.  Workgroup: 32 points x 16
-// layers, two (point, layer) elements per thread for the FP64 stage sums
-// (consecutive threads on consecutive points: 512 contiguous bytes of q per
-// layer); then each of the 4 waves projects 8 points (16 columns: re, im).
-// Every wave builds the split matrix operands before the stage loads, so
-// those small loads overlap them (measured faster at 16 layers than holding
-// the operands across tiles, which needs twice the registers).
-template <typename S, int NS>
+// q) followed by the MFMA projection, nz <= 16.  Workgroup: T tiles of 32
+// points x 16 layers, 2T (point, layer) elements per thread for the FP64
+// stage sums (consecutive threads on consecutive points: 512 contiguous bytes
+// of q per layer), all of a thread's loads issued before its first store;
+// then, per tile, each of the 4 waves projects 8 points (16 columns: re, im).
+// Every wave builds the split matrix operands once, before the stage loads, so
+// those small loads overlap them.  T > 1 keeps more bytes in flight where a
+// stage has few input arrays.
+template <typename S, int NS, int T>
 __global__ void __launch_bounds__(256) k_stage_proj_mfma16(StageArgs a, Geom g, const double* __restrict__ HM,
                                                          const double* __restrict__ MD,
                                                          const double* __restrict__ evals, double inv_htot,
                                                          float2* __restrict__ psi) {
-  __shared__ double2 sq[16][32];    // [layer][point]
+  __shared__ double2 sq[T][16][32];    // [tile][layer][point]
   const int nz = g.nzm;
   const int mb = blockIdx.y;         // member
   HM += mb * nz * nz;
@@ -446,8 +449,9 @@ __global__ void __launch_bounds__(256) k_stage_proj_mfma16(StageArgs a, Geom g, 
   const double dt = a.dtv ? a.dtv[mb] : a.dt;
   const int tid = threadIdx.x;
   const long plane = (long)g.Ky * g.Kxp;
+  const long p0 = (long)blockIdx.x * (32 * T);       // first point of the workgroup
   const int l = tid & 63;
-  const int tile = tid >> 6;                  // wave: points 8*tile .. 8*tile+7
+  const int wv = tid >> 6;                    // wave: points 8*wv .. 8*wv+7 of each tile
   const int row = l & 15, kb = 4 * (l >> 4);  // A: i = row; B, D: j = row
   qgn_short4 a1h, a1l, a2h, a2l, bh, bl;
   double ev[4];
@@ -464,59 +468,76 @@ __global__ void __launch_bounds__(256) k_stage_proj_mfma16(StageArgs a, Geom g, 
   }
   {
     const int pp = tid & 31, k0 = tid >> 5;   // layers k0 and k0 + 8
-    const long pt = (long)blockIdx.x * 32 + pp;
-    double2 qt[2];
+    double2 qt[T][2];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int k = k0 + 8 * h;
-      qt[h] = make_double2(0.0, 0.0);
-      if (k < nz && pt < plane) qt[h] = stage_value<S, NS>(a, g, moff + k * plane + pt, pt, mb * nz + k, dt);
+    for (int t = 0; t < T; ++t) {
+      const long pt = p0 + 32 * t + pp;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int k = k0 + 8 * h;
+        qt[t][h] = make_double2(0.0, 0.0);
+        if (k < nz && pt < plane) qt[t][h] = stage_value<S, NS>(a, g, moff + k * plane + pt, pt, mb * nz + k, dt);
+      }
     }
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int k = k0 + 8 * h;
-      if (a.out && k < nz && pt < plane) a.out[moff + k * plane + pt] = qt[h];
-      sq[k][pp] = qt[h];
+    for (int t = 0; t < T; ++t) {
+      const long pt = p0 + 32 * t + pp;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int k = k0 + 8 * h;
+        if (k < nz && pt < plane) {
+          if (a.out) a.out[moff + k * plane + pt] = qt[t][h];
+          if (a.out32) a.out32[moff + k * plane + pt] = make_float2((float)qt[t][h].x, (float)qt[t][h].y);
+        }
+        sq[t][k][pp] = qt[t][h];
+      }
     }
   }
   __syncthreads();
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const double2 v = sq[kb + r][8 * tile + (row >> 1)];
-    const float x = (float)((row & 1) ? v.y : v.x);
-    bh[r] = bf16_rne(x);
-    bl[r] = bf16_lo(x, bh[r]);
-  }
-  qgn_float4 d = {0.f, 0.f, 0.f, 0.f};
-  d = mfma3(a1h, a1l, bh, bl, d);            // modal amplitudes, D[mode][column]
-  const long pt = (long)blockIdx.x * 32 + 8 * tile + (row >> 1);
-  bool zero_mode = pt >= plane;
-  double k2 = 0.0;
-  if (!zero_mode) {
-    const int i = (int)pt % g.Kxp;
-    const int j = compact_row_to_full((int)pt / g.Kxp, g);
-    const double kx = kx_of(i, g), ky = ky_of(j, g);
-    k2 = kx * kx + ky * ky;
-    zero_mode = (i == 0 && j == 0) || i >= g.Kx;
-  }
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int mode = kb + r;
-    const float pm = (zero_mode || mode >= nz) ? 0.f : d[r] / (float)(-k2 + ev[r]);
-    bh[r] = bf16_rne(pm);
-    bl[r] = bf16_lo(pm, bh[r]);
-  }
-  qgn_float4 p = {0.f, 0.f, 0.f, 0.f};
-  p = mfma3(a2h, a2l, bh, bl, p);            // D[layer][column]
-  if (pt < plane) {
-    float* out = reinterpret_cast<float*>(psi);
+  for (int t = 0; t < T; ++t) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int layer = kb + r;
-      if (layer < nz) out[2 * (moff + layer * plane + pt) + (row & 1)] = p[r];
+      const double2 v = sq[t][kb + r][8 * wv + (row >> 1)];
+      const float x = (float)((row & 1) ? v.y : v.x);
+      bh[r] = bf16_rne(x);
+      bl[r] = bf16_lo(x, bh[r]);
+    }
+    qgn_float4 d = {0.f, 0.f, 0.f, 0.f};
+    d = mfma3(a1h, a1l, bh, bl, d);            // modal amplitudes, D[mode][column]
+    const long pt = p0 + 32 * t + 8 * wv + (row >> 1);
+    bool zero_mode = pt >= plane;
+    double k2 = 0.0;
+    if (!zero_mode) {
+      const int i = (int)pt % g.Kxp;
+      const int j = compact_row_to_full((int)pt / g.Kxp, g);
+      const double kx = kx_of(i, g), ky = ky_of(j, g);
+      k2 = kx * kx + ky * ky;
+      zero_mode = (i == 0 && j == 0) || i >= g.Kx;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int mode = kb + r;
+      const float pm = (zero_mode || mode >= nz) ? 0.f : d[r] / (float)(-k2 + ev[r]);
+      bh[r] = bf16_rne(pm);
+      bl[r] = bf16_lo(pm, bh[r]);
+    }
+    qgn_float4 p = {0.f, 0.f, 0.f, 0.f};
+    p = mfma3(a2h, a2l, bh, bl, p);            // D[layer][column]
+    if (pt < plane) {
+      float* out = reinterpret_cast<float*>(psi);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int layer = kb + r;
+        if (layer < nz) out[2 * (moff + layer * plane + pt) + (row & 1)] = p[r];
+      }
     }
   }
 }
+
+// tiles per workgroup of k_stage_proj_mfma16 for a stage with NS earlier
+// stages (measured at 1024^2 x 16: 4 and 2 beat 1, 8 and 4)
+template <int NS> struct ProjTiles { static constexpr int T = NS == 0 ? 4 : 2; };
 
 // Stage combination (or, with a.ns = 0 and a.out = nullptr, a plain copy of
 // q) followed by the MFMA projection, for nz <= NZT layers (NZT = 16 or 32:
@@ -584,6 +605,7 @@ __global__ void __launch_bounds__(256) k_stage_proj_mfma(StageArgs a, Geom g, co
         const long e = moff + k * plane + pt;
         qt = stage_value<S, NS>(a, g, e, pt, mb * nz + k, dt);
         if (a.out) a.out[e] = qt;
+        if (a.out32) a.out32[e] = make_float2((float)qt.x, (float)qt.y);
       }
       sq[half][k][pp] = qt;
     }

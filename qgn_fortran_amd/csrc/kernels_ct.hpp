@@ -414,16 +414,30 @@ __global__ void __launch_bounds__(NTHR) k_col_rhs_ct(ColRhsArgs a, Geom g, const
   // zero)
   const long lay = (long)z * g.Ky * g.Kxp;
   const int plane = g.Ky * g.Kxp;
-  const __amdgpu_buffer_rsrc_t rq = row_rsrc(a.q + lay, plane * (int)sizeof(double2));
+  // (q rounded to R, when the stage kernel wrote that copy: half the bytes)
+  const bool q_r = a.q32 != nullptr;
+  const __amdgpu_buffer_rsrc_t rq =
+      q_r ? row_rsrc(reinterpret_cast<const Cx*>(a.q32) + lay, plane * (int)sizeof(Cx))
+          : row_rsrc(a.q + lay, plane * (int)sizeof(double2));
   const __amdgpu_buffer_rsrc_t rp = row_rsrc(reinterpret_cast<const Cx*>(a.psi) + lay, plane * (int)sizeof(Cx));
-  double2 qv[MAXO];
+  Cx qv[MAXO];
   Cx pv[MAXO];
+  if (q_r) {
 #pragma unroll
-  for (int u = 0; u < MAXO; ++u) {
-    const int o = threadIdx.x + u * NTHR;
-    const int off = o < nown ? (o / C) * g.Kxp + i0 + o % C : -1;
-    qv[u] = rsrc_ld<double2>(rq, off >= 0 ? off * (int)sizeof(double2) : kRowOob);
-    pv[u] = rsrc_ld<Cx>(rp, off >= 0 ? off * (int)sizeof(Cx) : kRowOob);
+    for (int u = 0; u < MAXO; ++u) {
+      const int o = threadIdx.x + u * NTHR;
+      const int off = o < nown ? (o / C) * g.Kxp + i0 + o % C : -1;
+      qv[u] = rsrc_ld<Cx>(rq, off >= 0 ? off * (int)sizeof(Cx) : kRowOob);
+      pv[u] = rsrc_ld<Cx>(rp, off >= 0 ? off * (int)sizeof(Cx) : kRowOob);
+    }
+  } else {
+#pragma unroll
+    for (int u = 0; u < MAXO; ++u) {
+      const int o = threadIdx.x + u * NTHR;
+      const int off = o < nown ? (o / C) * g.Kxp + i0 + o % C : -1;
+      qv[u] = to_c<R>(rsrc_ld<double2>(rq, off >= 0 ? off * (int)sizeof(double2) : kRowOob));
+      pv[u] = rsrc_ld<Cx>(rp, off >= 0 ? off * (int)sizeof(Cx) : kRowOob);
+    }
   }
   Cx keep[MAXO];
 #pragma unroll
@@ -433,7 +447,7 @@ __global__ void __launch_bounds__(NTHR) k_col_rhs_ct(ColRhsArgs a, Geom g, const
     const int c = o % C, jc = o / C, i = i0 + c;
     const int j = compact_row_to_full(jc, g);
     const Cx v = buf[j * C + c];
-    const Cx q = to_c<R>(qv[u]);
+    const Cx q = qv[u];
     const Cx ps = pv[u];
     const R kx = kx_r<R>(i, g), ky = ky_r<R>(j, g);
     Cx r = cmk<Cx>(-v.x, -v.y);

@@ -261,6 +261,7 @@ struct qgn_ctx {
   double2 *q = nullptr, *qn = nullptr, *E = nullptr;
   double2* psi64 = nullptr;
   void* psi = nullptr;   // cplx<R>
+  void* q32 = nullptr;   // mixed: the RHS input rounded to FP32, written by the MFMA projection kernels
   void* N[6] = {};       // cplx<R>
   double2* Y[6] = {};    // Y[0] unused
   void* hist[3] = {};    // AB3 / BSRK3 saved RHS, cplx<R>
@@ -521,18 +522,19 @@ int with_ns(int ns, F&& f) {
 // Mixed precision with nz <= 32: the stage + split-BF16 MFMA projection.
 int launch_stage_proj_mfma(qgn_ctx* c, const StageArgs& a, int kernel_id) {
   const long plane = (long)c->g.Ky * c->g.Kxp;
-  // nz <= 16: one tile of 32 points per workgroup; nz <= 32: workgroups walk
+  // nz <= 16: ProjTiles<NS>::T tiles of 32 points per workgroup; nz <= 32: workgroups walk
   // tiles of 16 points, 8 per CU over all members, so the 4x larger split
   // matrices are built once per workgroup
-  const long tiles = c->g.nzm <= 16 ? (plane + 31) / 32 : (plane + 15) / 16;
-  const long want = c->g.nzm <= 16 ? tiles : std::max(1L, (8L * c->ncu + c->nm - 1) / c->nm);
-  const int blocks = (int)std::min(tiles, want);
   return with_ns(a.ns, [&](auto nsc) {
     constexpr int NS = decltype(nsc)::value;
+    constexpr int T = ProjTiles<NS>::T;
+    const long tiles = c->g.nzm <= 16 ? (plane + 32 * T - 1) / (32 * T) : (plane + 15) / 16;
+    const long want = c->g.nzm <= 16 ? tiles : std::max(1L, (8L * c->ncu + c->nm - 1) / c->nm);
+    const int blocks = (int)std::min(tiles, want);
     return timed(c, kernel_id, [&] {
       if (c->g.nzm <= 16)
-        hipLaunchKernelGGL((k_stage_proj_mfma16<float, NS>), dim3(blocks, c->nm), dim3(256), 0, c->stream, a, c->g,
-                           c->HM, c->MD, c->evals, 1.0 / c->p.Htot, reinterpret_cast<float2*>(c->psi));
+        hipLaunchKernelGGL((k_stage_proj_mfma16<float, NS, T>), dim3(blocks, c->nm), dim3(256), 0, c->stream, a,
+                           c->g, c->HM, c->MD, c->evals, 1.0 / c->p.Htot, reinterpret_cast<float2*>(c->psi));
       else
         hipLaunchKernelGGL((k_stage_proj_mfma<float, 32, NS>), dim3(blocks, c->nm), dim3(256), 0, c->stream, a, c->g,
                            c->HM, c->MD, c->evals, 1.0 / c->p.Htot, reinterpret_cast<float2*>(c->psi));
@@ -638,6 +640,10 @@ template <typename R>
 int rhs_impl(qgn_ctx* c, const double2* qin, void* out, bool psi_ready) {
   void* psi = std::is_same<R, double>::value ? (void*)c->psi64 : c->psi;
   const int isd = std::is_same<R, double>::value;
+  // the MFMA projection kernels also write qin rounded to FP32 (the
+  // projection below, or the stage kernel that computed psi): P1 and P3
+  // read that copy, as they would round qin to FP32 themselves
+  void* q32 = std::is_same<R, float>::value && use_mfma_projection(c) ? c->q32 : nullptr;
   if (!psi_ready) {
     if (use_mfma_projection(c)) {
       StageArgs pa{};
@@ -645,6 +651,7 @@ int rhs_impl(qgn_ctx* c, const double2* qin, void* out, bool psi_ready) {
       pa.ns = 0;
       pa.dt = 0.0;
       pa.out = nullptr;
+      pa.out32 = reinterpret_cast<float2*>(c->q32);
       CHK(launch_stage_proj_mfma(c, pa, QGN_K_PROJECT));
     } else {
       CHK(launch_project<R>(c, qin, psi));
@@ -654,15 +661,16 @@ int rhs_impl(qgn_ctx* c, const double2* qin, void* out, bool psi_ready) {
   ca.nf = 4;
   ca.src[0] = psi; ca.src_is_double[0] = isd; ca.mult[0] = 1; ca.dst[0] = c->Um;    // u = -i ky psi
   ca.src[1] = psi; ca.src_is_double[1] = isd; ca.mult[1] = 2; ca.dst[1] = c->Vm;    // v =  i kx psi
-  ca.src[2] = qin; ca.src_is_double[2] = 1; ca.mult[2] = 2; ca.dst[2] = c->QXm;     // q_x
-  ca.src[3] = qin; ca.src_is_double[3] = 1; ca.mult[3] = 3; ca.dst[3] = c->QYm;     // q_y
+  const void* qsrc = q32 ? q32 : (const void*)qin;
+  ca.src[2] = qsrc; ca.src_is_double[2] = !q32; ca.mult[2] = 2; ca.dst[2] = c->QXm;   // q_x
+  ca.src[3] = qsrc; ca.src_is_double[3] = !q32; ca.mult[3] = 3; ca.dst[3] = c->QYm;   // q_y
   CHK(launch_col_inv<R>(c, ca));
   RowJacArgs ra{};
   ra.U = c->Um; ra.V = c->Vm; ra.QX = c->QXm; ra.QY = c->QYm; ra.J = c->Jm; ra.DU = c->DUm; ra.DV = c->DVm;
   ra.drag = c->p.C_d != 0.0;
   CHK(launch_row_jac<R>(c, ra));
   ColRhsArgs ka{};
-  ka.J = c->Jm; ka.q = qin; ka.psi = psi; ka.out = out;
+  ka.J = c->Jm; ka.q = qin; ka.q32 = q32; ka.psi = psi; ka.out = out;
   ka.ubar = c->ubar; ka.qybar = c->qybar;
   ka.beta = c->p.beta;
   ka.ekman = c->ekm;
@@ -874,7 +882,9 @@ int launch_stage_proj(qgn_ctx* c, const StageArgs& a, bool* done) {
 int stage_proj(qgn_ctx* c, const StageArgs& a, bool* done) {
   if (use_mfma_projection(c)) {
     *done = true;
-    return launch_stage_proj_mfma(c, a, QGN_K_STAGE);
+    StageArgs b = a;
+    b.out32 = reinterpret_cast<float2*>(c->q32);   // read by the GetRHS of this stage
+    return launch_stage_proj_mfma(c, b, QGN_K_STAGE);
   }
   return c->p.precision == QGN_MIXED ? launch_stage_proj<float>(c, a, done) : launch_stage_proj<double>(c, a, done);
 }
@@ -1239,7 +1249,7 @@ void destroy_ctx(qgn_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   void* ptrs[] = {c->HM, c->MD, c->evals, c->ubar, c->qybar, c->a8l, c->partial, c->q, c->qn, c->E, c->psi64,
-                  c->psi, c->Um, c->Vm, c->QXm, c->QYm, c->Jm, c->DUm, c->DVm, c->Em, c->errslot, c->errpart, c->fa, c->fb,
+                  c->psi, c->q32, c->Um, c->Vm, c->QXm, c->QYm, c->Jm, c->DUm, c->DVm, c->Em, c->errslot, c->errpart, c->fa, c->fb,
                   c->phys};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -1443,6 +1453,7 @@ int create_impl(qgn_ctx* c, const qgn_params* p, int members, const double* H, c
   CHK(alloc((void**)&c->E, cb));
   CHK(alloc((void**)&c->psi64, cb));
   CHK(alloc(&c->psi, (size_t)c->ncmp * rs));
+  if (p->precision == QGN_MIXED) CHK(alloc(&c->q32, (size_t)c->ncmp * sizeof(float2)));
   for (int i = 0; i < 6; ++i) CHK(alloc(&c->N[i], (size_t)c->ncmp * rs));
   for (int i = 1; i < 6; ++i) CHK(alloc((void**)&c->Y[i], cb));
   for (int i = 0; i < 3; ++i) CHK(alloc(&c->hist[i], (size_t)c->ncmp * rs));
