@@ -202,18 +202,22 @@ struct ColRhsArgs {
 // and kx = nx/2 bins are ignored, as DFTI does.  z = IFFT_N2(Z) then holds
 // x[2m] + i x[2m+1].
 // ---------------------------------------------------------------------------
-// From the already loaded bins a = X[k], b = X[N2-k].
+// From the already loaded bins a = X[k], b = X[N2-k] and table value
+// w = twr[k] = exp(-2 pi i k / nx).
 template <typename R>
-__device__ __forceinline__ cplx<R> c2r_pre_ab(cplx<R> a, cplx<R> b, int k, const cplx<R>* __restrict__ twr,
-                                              R scale) {
+__device__ __forceinline__ cplx<R> c2r_pre_w(cplx<R> a, cplx<R> b, int k, cplx<R> w, R scale = (R)1) {
   using Cx = cplx<R>;
   if (k == 0) { a.y = (R)0; b.y = (R)0; }
   const Cx s = cmk<Cx>(a.x + b.x, a.y - b.y);   // X[k] + conj X[N2-k]
   const Cx d = cmk<Cx>(a.x - b.x, a.y + b.y);   // X[k] - conj X[N2-k]
-  Cx w = twr[k];
   w.y = -w.y;                                   // e^{+2 pi i k / nx}
   const Cx wd = cmul(w, d);
   return cmk<Cx>((s.x - wd.y) * scale, (s.y + wd.x) * scale);  // s + i*wd
+}
+template <typename R>
+__device__ __forceinline__ cplx<R> c2r_pre_ab(cplx<R> a, cplx<R> b, int k, const cplx<R>* __restrict__ twr,
+                                              R scale) {
+  return c2r_pre_w<R>(a, b, k, twr[k], scale);
 }
 
 template <typename R>
@@ -270,9 +274,52 @@ struct StageArgs {
   const double* a8l;      // per-layer A8 (nullptr -> A8 below)
   double A8, A2;
   int implicit_visc;      // use Mq and L_j terms
-  double2* out;
+  double2* out;           // may be null where nothing reads the stage value in FP64
   float2* out32;          // MFMA projection kernels: FP32 copy of the stage value (may be null)
+  // recompute != 0 (implicit_visc only): the earlier stage values Y_2..Y_ns
+  // are recomputed from q and N instead of read from Y, with the tables
+  // ye[j][k] = ae, yi[j][k] = ai of stage j+1 on stage k+1 (k < j)
+  int recompute;
+  double ye[6][6], yi[6][6];
 };
+
+// One element's stage combination over the first n earlier stages:
+//   mq (q + dt (sum_k ae_k N_k + sum_k ai_k (-lam Y_k))),  Y_0 = q,
+// explicit sum first, as the stage kernel that stored a stage value formed
+// it, so a recomputed Y_k repeats its arithmetic.
+__device__ __forceinline__ double2 stage_combine(int n, const double* ae, const double* ai, const double2* nd,
+                                                 const double2* y, double2 q, bool impl, double lam, double mq,
+                                                 double dt) {
+  double sx = 0.0, sy = 0.0;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    if (k >= n) break;
+    sx += ae[k] * nd[k].x;
+    sy += ae[k] * nd[k].y;
+  }
+  if (impl) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      if (k >= n) break;
+      const double2 yk = k == 0 ? q : y[k];
+      sx += ai[k] * (-lam * yk.x);
+      sy += ai[k] * (-lam * yk.y);
+    }
+  }
+  return make_double2(mq * (q.x + dt * sx), mq * (q.y + dt * sy));
+}
+
+// lam = A2 k2 + A8 k2^4 of compact point pt, layer kl of the batch
+__device__ __forceinline__ double visc_lambda(const Geom& g, long pt, int kl, const double* a8l, double A8,
+                                              double A2) {
+  const int i = (int)pt % g.Kxp;
+  const int j = compact_row_to_full((int)pt / g.Kxp, g);
+  const double kx = kx_of(i, g), ky = ky_of(j, g);
+  const double k2 = kx * kx + ky * ky;
+  const double k4 = k2 * k2;
+  const double a8 = a8l ? a8l[kl] : A8;
+  return A2 * k2 + a8 * (k4 * k4);
+}
 
 // ARK stage value of element e (layer kl of the batch, point pt of its
 // plane), the stage blocks of ARK43*.f90:
@@ -284,39 +331,31 @@ struct StageArgs {
 template <typename S, int NS>
 __device__ __forceinline__ double2 stage_value(const StageArgs& a, const Geom& g, long e, long pt, int kl,
                                                double dt) {
+  constexpr int M = NS > 0 ? NS : 1;
   const double2 q = a.q[e];
-  cplx<S> nv[NS > 0 ? NS : 1];
+  cplx<S> nv[M];
 #pragma unroll
   for (int jj = 0; jj < NS; ++jj) nv[jj] = reinterpret_cast<const cplx<S>*>(a.N[jj])[e];
-  double2 yv[NS > 1 ? NS : 1];
-  if (a.implicit_visc) {
+  double2 yv[M];
+  const bool rec = a.recompute != 0;
+  if (a.implicit_visc && !rec) {
 #pragma unroll
     for (int jj = 1; jj < NS; ++jj) yv[jj] = a.Y[jj][e];
   }
-  double sx = 0.0, sy = 0.0;
+  double2 nd[M];
 #pragma unroll
-  for (int jj = 0; jj < NS; ++jj) {
-    sx += a.ae[jj] * (double)nv[jj].x;
-    sy += a.ae[jj] * (double)nv[jj].y;
-  }
-  double mq = 1.0;
+  for (int jj = 0; jj < NS; ++jj) nd[jj] = make_double2((double)nv[jj].x, (double)nv[jj].y);
+  double lam = 0.0, mq = 1.0;
   if (a.implicit_visc) {
-    const int i = (int)pt % g.Kxp;
-    const int j = compact_row_to_full((int)pt / g.Kxp, g);
-    const double kx = kx_of(i, g), ky = ky_of(j, g);
-    const double k2 = kx * kx + ky * ky;
-    const double k4 = k2 * k2;
-    const double a8 = a.a8l ? a.a8l[kl] : a.A8;
-    const double lam = a.A2 * k2 + a8 * (k4 * k4);
-#pragma unroll
-    for (int jj = 0; jj < NS; ++jj) {
-      const double2 y = jj == 0 ? q : yv[jj];
-      sx += a.ai[jj] * (-lam * y.x);
-      sy += a.ai[jj] * (-lam * y.y);
-    }
+    lam = visc_lambda(g, pt, kl, a.a8l, a.A8, a.A2);
     mq = 1.0 / (1.0 + 0.25 * dt * lam);
+    if (rec) {
+      yv[0] = q;
+#pragma unroll
+      for (int jj = 1; jj < NS; ++jj) yv[jj] = stage_combine(jj, a.ye[jj], a.yi[jj], nd, yv, q, true, lam, mq, dt);
+    }
   }
-  return make_double2(mq * (q.x + dt * sx), mq * (q.y + dt * sy));
+  return stage_combine(NS, a.ae, a.ai, nd, yv, q, a.implicit_visc != 0, lam, mq, dt);
 }
 
 template <typename S, int NS>
@@ -683,34 +722,49 @@ struct UpdateArgs {
   int implicit_visc;
   double2* E;     // may be null
   double2* qn;
+  int recompute;  // as in StageArgs: Y_2..Y_ns from q and N
+  double ye[6][6], yi[6][6];
 };
 
+// grid (x, nz): blockIdx.y is the layer, x strides over its plane
 template <typename S>
 __global__ void k_update(UpdateArgs a, Geom g) {
-  const long plane = (long)g.Ky * g.Kxp;
-  const long n = plane * g.nz;
-  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+  const int plane = g.Ky * g.Kxp;
+  const int z = blockIdx.y;
+  const bool rec = a.recompute != 0 && a.implicit_visc;
+  const double dt = a.dtv ? a.dtv[z / g.nzm] : a.dt;
+  for (int pt = blockIdx.x * blockDim.x + threadIdx.x; pt < plane; pt += gridDim.x * blockDim.x) {
+    const long e = (long)z * plane + pt;
     const double2 q = a.q[e];
-    const int z = (int)(e / plane);
-    const long pt = e % plane;
-    const double dt = a.dtv ? a.dtv[z / g.nzm] : a.dt;
+    // all of the element's loads first
+    double2 nd[6], yv[6];
+#pragma unroll
+    for (int jj = 0; jj < 6; ++jj) {
+      if (jj >= a.ns) break;
+      const cplx<S> v = reinterpret_cast<const cplx<S>*>(a.N[jj])[e];
+      nd[jj] = make_double2((double)v.x, (double)v.y);
+      if (a.implicit_visc && !rec && jj > 0) yv[jj] = a.Y[jj][e];
+    }
     double lam = 0.0;
     if (a.implicit_visc) {
-      const int i = (int)pt % g.Kxp;
-      const int j = compact_row_to_full((int)pt / g.Kxp, g);
-      const double kx = kx_of(i, g), ky = ky_of(j, g);
-      const double k2 = kx * kx + ky * ky;
-      const double k4 = k2 * k2;
-      const double a8 = a.a8l ? a.a8l[z] : a.A8;
-      lam = a.A2 * k2 + a8 * (k4 * k4);
+      lam = visc_lambda(g, pt, z, a.a8l, a.A8, a.A2);
+      if (rec) {
+        const double mq = 1.0 / (1.0 + 0.25 * dt * lam);
+#pragma unroll
+        for (int jj = 1; jj < 6; ++jj) {
+          if (jj >= a.ns) break;
+          yv[jj] = stage_combine(jj, a.ye[jj], a.yi[jj], nd, yv, q, true, lam, mq, dt);
+        }
+      }
     }
     double ex = 0.0, ey = 0.0, ux = 0.0, uy = 0.0;
-    for (int jj = 0; jj < a.ns; ++jj) {
+#pragma unroll
+    for (int jj = 0; jj < 6; ++jj) {
+      if (jj >= a.ns) break;
       if (a.b[jj] == 0.0 && a.be[jj] == 0.0) continue;
-      const cplx<S> v = reinterpret_cast<const cplx<S>*>(a.N[jj])[e];
-      double tx = (double)v.x, ty = (double)v.y;
+      double tx = nd[jj].x, ty = nd[jj].y;
       if (a.implicit_visc) {
-        const double2 y = jj == 0 ? q : a.Y[jj][e];
+        const double2 y = jj == 0 ? q : yv[jj];
         tx += -lam * y.x;
         ty += -lam * y.y;
       }

@@ -545,18 +545,36 @@ __global__ void __launch_bounds__(NTHR) k_row_maxabs_ct(const cplx<R>* __restric
   using Cx = cplx<R>;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   constexpr int PS = row_pad_shift<R>();
+  constexpr int TOT = ROWS * N2, EPT = (TOT + NTHR - 1) / NTHR;
+  constexpr bool EXACT = EPT * NTHR == TOT;
+  constexpr int SZ = (int)sizeof(Cx);
   Cx* buf = reinterpret_cast<Cx*>(lds_raw);
   Cx* ts = buf + row_lds_slots<R>(ROWS * N2);
-  Cx* tr = ts + ct_tw_size(N2);
   __shared__ double red[16];
   __shared__ int bad;
   if (threadIdx.x == 0) bad = 0;
-  load_table<R>(ts, twx, ct_tw_size(N2));
-  load_table<R>(tr, twr, N2 + 1);
-  __syncthreads();
+  // the rows' bins and each element's pre-processing twiddle exp(+2 pi i k/nx)
+  // are loaded straight into registers, all in flight while the FFT's
+  // per-pass tables are copied into LDS
   const long row0 = (long)blockIdx.x * ROWS;
-  const __amdgpu_buffer_rsrc_t rs = row_rsrc(mixed + row0 * stride, ROWS * stride * (int)sizeof(Cx));
-  c2r_pre_rows<R, N2, ROWS, NTHR, PS>(buf, rs, stride, Kin, tr, (R)1);
+  const __amdgpu_buffer_rsrc_t rs = row_rsrc(mixed + row0 * stride, ROWS * stride * SZ);
+  const __amdgpu_buffer_rsrc_t rt = row_rsrc(twr, (N2 + 1) * SZ);
+  Cx va[EPT], vb[EPT], w[EPT];
+#pragma unroll
+  for (int u = 0; u < EPT; ++u) {
+    const int e = threadIdx.x + u * NTHR;
+    const bool ok = EXACT || e < TOT;
+    const int r = e / N2, k = e - r * N2, kk = N2 - k;
+    va[u] = rsrc_ld<Cx>(rs, ok && k < Kin ? (r * stride + k) * SZ : kRowOob);
+    vb[u] = rsrc_ld<Cx>(rs, ok && kk < Kin ? (r * stride + kk) * SZ : kRowOob);
+    w[u] = rsrc_ld<Cx>(rt, ok ? k * SZ : kRowOob);
+  }
+  load_table<R>(ts, twx, ct_tw_size(N2));
+#pragma unroll
+  for (int u = 0; u < EPT; ++u) {
+    const int e = threadIdx.x + u * NTHR;
+    if (EXACT || e < TOT) buf[padded<PS>(e)] = c2r_pre_w<R>(va[u], vb[u], e % N2, w[u]);
+  }
   __syncthreads();
   ct_fft<R, N2, CtLayout<ROWS, N2, 1, false, PS>, NTHR>(buf, ts, +1);
   double m = 0.0;

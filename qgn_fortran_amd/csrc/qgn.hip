@@ -802,7 +802,7 @@ int errnorm_launch(qgn_ctx* c, const double2* Ecmp) {
       constexpr int RW = Cfg::ROWS * 4 > 16 ? 16 : Cfg::ROWS * 4;
       constexpr int NT0 = RW * N2 / 8;
       constexpr int NT = NT0 < 64 ? 64 : (NT0 > 1024 ? 1024 : ((NT0 + 63) / 64) * 64);
-      const size_t lds = (size_t)(row_lds_slots<double>(RW * N2) + ct_tw_size(N2) + N2 + 1) * sizeof(double2);
+      const size_t lds = (size_t)(row_lds_slots<double>(RW * N2) + ct_tw_size(N2)) * sizeof(double2);
       auto kern = k_row_maxabs_ct<double, N2, RW, NT>;
       rc = allow_lds(kern, lds);
       if (rc) return;
@@ -890,12 +890,13 @@ int stage_proj(qgn_ctx* c, const StageArgs& a, bool* done) {
 }
 
 int update(qgn_ctx* c, const UpdateArgs& a) {
-  const long n = c->ncmp;
+  const long plane = (long)c->g.Ky * c->g.Kxp;
+  const dim3 grid(grid1d(plane), c->g.nz);
   return timed(c, QGN_K_UPDATE, [&] {
     if (c->p.precision == QGN_MIXED)
-      hipLaunchKernelGGL((k_update<float>), dim3(grid1d(n)), dim3(256), 0, c->stream, a, c->g);
+      hipLaunchKernelGGL((k_update<float>), grid, dim3(256), 0, c->stream, a, c->g);
     else
-      hipLaunchKernelGGL((k_update<double>), dim3(grid1d(n)), dim3(256), 0, c->stream, a, c->g);
+      hipLaunchKernelGGL((k_update<double>), grid, dim3(256), 0, c->stream, a, c->g);
   });
 }
 
@@ -930,6 +931,18 @@ int lincomb(qgn_ctx* c, const void* X0, const void* X1, const void* X2, const vo
 // dtv: per-member dt on the device (a batch), else dt.
 int ark43_stages(qgn_ctx* c, double dt, const double* dtv, const double* a8l) {
   const qgn_params& p = c->p;
+  // The stage and update kernels recompute the earlier stage values from q
+  // and N (per mode, the implicit term is diagonal) instead of reading them.
+  // A stage value is then stored only where its GetRHS reads it in FP64:
+  // the mixed MFMA path reads the FP32 copy the stage kernel writes, unless
+  // the closure network takes its input from it.
+  const bool store_y = !use_mfma_projection(c) || c->net;
+  double ye[6][6] = {}, yi[6][6] = {};
+  for (int j = 1; j < 6; ++j)
+    for (int k = 0; k < j; ++k) {
+      ye[j][k] = kArk.ae[j + 1][k + 1];
+      yi[j][k] = kArk.ai[j + 1][k + 1];
+    }
   for (int i = 2; i <= 6; ++i) {
     StageArgs a{};
     a.q = c->q;
@@ -946,7 +959,10 @@ int ark43_stages(qgn_ctx* c, double dt, const double* dtv, const double* a8l) {
     a.A8 = p.A8;
     a.A2 = p.A2;
     a.implicit_visc = 1;
-    a.out = c->Y[i - 1];
+    a.recompute = 1;
+    std::memcpy(a.ye, ye, sizeof(ye));
+    std::memcpy(a.yi, yi, sizeof(yi));
+    a.out = store_y ? c->Y[i - 1] : nullptr;
     bool fused = false;
     CHK(stage_proj(c, a, &fused));
     if (!fused) CHK(stage(c, a));
@@ -967,6 +983,9 @@ int ark43_stages(qgn_ctx* c, double dt, const double* dtv, const double* a8l) {
   u.A8 = p.A8;
   u.A2 = p.A2;
   u.implicit_visc = 1;
+  u.recompute = 1;
+  std::memcpy(u.ye, ye, sizeof(ye));
+  std::memcpy(u.yi, yi, sizeof(yi));
   u.E = c->E;
   u.qn = c->qn;
   return update(c, u);
