@@ -162,10 +162,10 @@ __global__ void k_project(const double2* __restrict__ q, cplx<Tout>* __restrict_
 // way).  mult: 0 -> x1, 1 -> -i ky, 2 -> +i kx, 3 -> +i ky.
 // ---------------------------------------------------------------------------
 struct ColInvArgs {
-  const void* src[4];   // compact inputs (double2 when src_is_double, else cplx<R>)
-  int src_is_double[4];
-  int mult[4];
-  void* dst[4];         // mixed outputs (cplx<R>)
+  const void* src[6];   // compact inputs (double2 when src_is_double, else cplx<R>)
+  int src_is_double[6];
+  int mult[6];
+  void* dst[6];         // mixed outputs (cplx<R>)
   int nf;
 };
 

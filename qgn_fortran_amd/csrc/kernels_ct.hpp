@@ -600,10 +600,16 @@ __global__ void __launch_bounds__(NTHR) k_row_maxabs_ct(const cplx<R>* __restric
 
 // Rows of mixed arrays to physical rows: out[row][x] = c2r(row)(x) * scale
 // (the c2r of k_row_maxabs_ct, stored).  out holds nx reals per row.
+// grid.y = 2 transforms a second array, mixed2 into out2, in the same launch.
 template <typename R, int N2, int ROWS, int NTHR>
 __global__ void __launch_bounds__(NTHR) k_row_c2r_ct(const cplx<R>* __restrict__ mixed, int Kin, int stride,
                                                      R* __restrict__ out, R scale, Geom g,
-                                                     const cplx<R>* __restrict__ twx, const cplx<R>* __restrict__ twr) {
+                                                     const cplx<R>* __restrict__ twx, const cplx<R>* __restrict__ twr,
+                                                     const cplx<R>* __restrict__ mixed2, R* __restrict__ out2) {
+  if (blockIdx.y) {
+    mixed = mixed2;
+    out = out2;
+  }
   using Cx = cplx<R>;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   constexpr int PS = row_pad_shift<R>();

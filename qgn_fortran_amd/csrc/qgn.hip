@@ -283,6 +283,7 @@ struct qgn_ctx {
   qgn_cnn* net = nullptr;
   double *cl_in = nullptr, *cl_out = nullptr;   // 2 nzm and nzm scales
   double *cl_pq = nullptr, *cl_pp = nullptr;
+  void *cl_mq = nullptr, *cl_mp = nullptr;      // mixed q and psi (cplx<R>)
   short* cl_x = nullptr;
   float* cl_y = nullptr;
   // launch choices of the FP32 passes (0: the fixed rule), picked by
@@ -664,6 +665,11 @@ int rhs_impl(qgn_ctx* c, const double2* qin, void* out, bool psi_ready) {
   const void* qsrc = q32 ? q32 : (const void*)qin;
   ca.src[2] = qsrc; ca.src_is_double[2] = !q32; ca.mult[2] = 2; ca.dst[2] = c->QXm;   // q_x
   ca.src[3] = qsrc; ca.src_is_double[3] = !q32; ca.mult[3] = 3; ca.dst[3] = c->QYm;   // q_y
+  if (c->net) {   // the closure network's inputs q and psi, mixed (closure_add follows)
+    ca.nf = 6;
+    ca.src[4] = qsrc; ca.src_is_double[4] = !q32; ca.mult[4] = 0; ca.dst[4] = c->cl_mq;
+    ca.src[5] = psi; ca.src_is_double[5] = isd; ca.mult[5] = 0; ca.dst[5] = c->cl_mp;
+  }
   CHK(launch_col_inv<R>(c, ca));
   RowJacArgs ra{};
   ra.U = c->Um; ra.V = c->Vm; ra.QX = c->QXm; ra.QY = c->QYm; ra.J = c->Jm; ra.DU = c->DUm; ra.DV = c->DVm;
@@ -683,9 +689,11 @@ int rhs_impl(qgn_ctx* c, const double2* qin, void* out, bool psi_ready) {
 
 // Row passes of the closure in the RHS type R: c2r of mixed rows to
 // physical rows (times scale), and r2c of physical rows to the Kx bins of
-// mixed rows; RW rows per workgroup (RW divides ny).
+// mixed rows; RW rows per workgroup (RW divides ny).  The inverse takes a
+// second array pair (src2 -> dst2) in the same launch when src2 is given.
 template <typename R, bool INVERSE>
-int launch_row_tx(qgn_ctx* c, const void* src, void* dst, R scale) {
+int launch_row_tx(qgn_ctx* c, const void* src, void* dst, R scale, const void* src2 = nullptr,
+                  void* dst2 = nullptr) {
   const Geom& g = c->g;
   const Tw<R>& tw = tw_of<R>(c);
   int rc = 0;
@@ -701,8 +709,10 @@ int launch_row_tx(qgn_ctx* c, const void* src, void* dst, R scale) {
       auto kern = k_row_c2r_ct<R, N2, RW, NT>;
       rc = allow_lds(kern, lds);
       if (rc) return;
-      hipLaunchKernelGGL(kern, grid, dim3(NT), lds, c->stream, reinterpret_cast<const cplx<R>*>(src), g.Kx, g.Kxp,
-                         reinterpret_cast<R*>(dst), scale, g, tw.sx, tw.r);
+      const dim3 grid2(grid.x, src2 ? 2 : 1);
+      hipLaunchKernelGGL(kern, grid2, dim3(NT), lds, c->stream, reinterpret_cast<const cplx<R>*>(src), g.Kx, g.Kxp,
+                         reinterpret_cast<R*>(dst), scale, g, tw.sx, tw.r,
+                         reinterpret_cast<const cplx<R>*>(src2 ? src2 : src), reinterpret_cast<R*>(src2 ? dst2 : dst));
     } else {
       auto kern = k_row_r2c_ct<R, N2, RW, NT>;
       rc = allow_lds(kern, lds);
@@ -738,26 +748,20 @@ int launch_col_fwd_add(qgn_ctx* c, const void* mixed, void* out) {
 
 // Learned closure: out += r2c(net(q, psi) * scale) on the kept modes, with
 // q, psi the physical fields of qin and of the psi the RHS pass just formed,
-// all in the RHS's arithmetic type (the network itself takes bf16).  Reuses
-// the RHS pass's mixed buffers, which it has finished with.
+// all in the RHS's arithmetic type (the network itself takes bf16).  Runs
+// right after that RHS pass, whose P1 also formed the mixed q and psi;
+// reuses its mixed J buffer, which it has finished with.
 template <typename S>
 int closure_add(qgn_ctx* c, const double2* qin, void* out) {
   using R = S;
   const Geom& g = c->g;
   hipStream_t s = c->stream;
   const long np = (long)g.nz * g.ny * g.nx;
-  const int isd = std::is_same<R, double>::value;
-  ColInvArgs ca{};
-  ca.nf = 2;
-  ca.src[0] = qin; ca.src_is_double[0] = 1; ca.mult[0] = 0; ca.dst[0] = c->Um;
-  ca.src[1] = isd ? (const void*)c->psi64 : (const void*)c->psi; ca.src_is_double[1] = isd; ca.mult[1] = 0;
-  ca.dst[1] = c->Vm;
-  CHK(launch_col_inv<R>(c, ca));
+  // the RHS pass's P1 formed the mixed q and psi (cl_mq, cl_mp)
   const R nn = (R)(1.0 / ((double)g.nx * (double)g.ny));
   R* pq = reinterpret_cast<R*>(c->cl_pq);
   R* pp = reinterpret_cast<R*>(c->cl_pp);
-  CHK((launch_row_tx<R, true>(c, c->Um, pq, nn)));
-  CHK((launch_row_tx<R, true>(c, c->Vm, pp, nn)));
+  CHK((launch_row_tx<R, true>(c, c->cl_mq, pq, nn, c->cl_mp, pp)));
   const long npx = np / g.nzm;   // members x pixels: one thread each
   if (g.nzm == 8)
     hipLaunchKernelGGL((k_closure_in<R, 8>), dim3(grid1d(npx)), dim3(256), 0, s, pq, pp, c->cl_in, c->cl_x, g);
@@ -1283,7 +1287,7 @@ void destroy_ctx(qgn_ctx* c) {
   for (int i = 0; i < 5; ++i)
     if (c->mp[i]) (void)hipFree(c->mp[i]);
   if (c->spec) (void)hipFree(c->spec);
-  void* cl[] = {c->cl_in, c->cl_out, c->cl_pq, c->cl_pp, c->cl_x, c->cl_y};
+  void* cl[] = {c->cl_in, c->cl_out, c->cl_pq, c->cl_pp, c->cl_mq, c->cl_mp, c->cl_x, c->cl_y};
   for (void* x : cl)
     if (x) (void)hipFree(x);
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
@@ -1724,6 +1728,8 @@ int qgn_set_closure(qgn_ctx* c, qgn_cnn* net, const double* in_scale, const doub
     CHK(alloc((void**)&c->cl_out, (size_t)g.nzm * sizeof(double)));
     CHK(alloc((void**)&c->cl_pq, np * sizeof(double)));
     CHK(alloc((void**)&c->cl_pp, np * sizeof(double)));
+    CHK(alloc(&c->cl_mq, (size_t)c->nmix * c->rsize));
+    CHK(alloc(&c->cl_mp, (size_t)c->nmix * c->rsize));
     CHK(alloc((void**)&c->cl_x, pix * 2 * g.nzm * sizeof(short)));
     CHK(alloc((void**)&c->cl_y, pix * g.nzm * sizeof(float)));
   }
