@@ -1070,58 +1070,6 @@ __global__ void k_batch_commit(double2* __restrict__ q, const double2* __restric
 }
 
 // ---------------------------------------------------------------------------
-// Learned closure plumbing: physical q and psi (planar, layers of all
-// members) -> network input (member, y, x, channel) in bf16, channels
-// q_1..q_nzm then psi_1..psi_nzm, each times its scale; and the network
-// output (member, y, x, layer) times its scale -> planar physical forcing.
-// ---------------------------------------------------------------------------
-template <typename R, int NZM>
-__global__ void k_closure_in(const R* __restrict__ q, const R* __restrict__ psi,
-                             const double* __restrict__ scale, short* __restrict__ x, Geom g) {
-  constexpr int C = 2 * NZM;               // channels: q then psi of each layer
-  const int plane = g.ny * g.nx;
-  const int npix = plane * (g.nz / NZM);   // member * plane + pixel
-  for (int pm = blockIdx.x * blockDim.x + threadIdx.x; pm < npix; pm += gridDim.x * blockDim.x) {
-    const int m = pm / plane, pix = pm - m * plane;
-    // each layer read is coalesced across the threads (consecutive pixels)
-    unsigned w[C / 2];
-#pragma unroll
-    for (int c = 0; c < C; c += 2) {
-      const R* f0 = c < NZM ? q : psi;
-      const int l0 = c < NZM ? c : c - NZM;   // c and c + 1 are in the same field (NZM even)
-      const long i0 = (long)(m * NZM + l0) * plane + pix;
-      const short h0 = bf16_rne((float)((double)f0[i0] * scale[c]));
-      const short h1 = bf16_rne((float)((double)f0[i0 + plane] * scale[c + 1]));
-      w[c / 2] = (unsigned)(unsigned short)h0 | ((unsigned)(unsigned short)h1 << 16);
-    }
-    uint4* xo = reinterpret_cast<uint4*>(x + (long)pm * C);   // C / 8 16-byte stores
-#pragma unroll
-    for (int k = 0; k < C / 8; ++k) xo[k] = make_uint4(w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]);
-  }
-}
-
-template <typename R, int NZM>
-__global__ void k_closure_out(const float* __restrict__ y, const double* __restrict__ scale,
-                              R* __restrict__ f, Geom g) {
-  const int plane = g.ny * g.nx;
-  const int npix = plane * (g.nz / NZM);
-  for (int pm = blockIdx.x * blockDim.x + threadIdx.x; pm < npix; pm += gridDim.x * blockDim.x) {
-    const int m = pm / plane, pix = pm - m * plane;
-    const float4* yi = reinterpret_cast<const float4*>(y + (long)pm * NZM);   // NZM / 4 16-byte loads
-#pragma unroll
-    for (int k = 0; k < NZM / 4; ++k) {
-      const float4 v = yi[k];
-      const float vv[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int c = 4 * k + r;
-        f[(long)(m * NZM + c) * plane + pix] = (R)((double)vv[r] * scale[c]);
-      }
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
 // Isotropic KE and enstrophy spectra of the dealiased state, per layer:
 //   KE(b) = 1/2 sum w k^2 |psi_hat|^2 / (nx ny)^2,  Z(b) = 1/2 sum w |q_hat|^2 / (nx ny)^2
 // over the half plane with w = 2 for kx > 0; bin b = floor(|k|/dk + 1/2).

@@ -598,57 +598,124 @@ __global__ void __launch_bounds__(NTHR) k_row_maxabs_ct(const cplx<R>* __restric
   }
 }
 
-// Rows of mixed arrays to physical rows: out[row][x] = c2r(row)(x) * scale
-// (the c2r of k_row_maxabs_ct, stored).  out holds nx reals per row.
-// grid.y = 2 transforms a second array, mixed2 into out2, in the same launch.
-template <typename R, int N2, int ROWS, int NTHR>
-__global__ void __launch_bounds__(NTHR) k_row_c2r_ct(const cplx<R>* __restrict__ mixed, int Kin, int stride,
-                                                     R* __restrict__ out, R scale, Geom g,
-                                                     const cplx<R>* __restrict__ twx, const cplx<R>* __restrict__ twr,
-                                                     const cplx<R>* __restrict__ mixed2, R* __restrict__ out2) {
-  if (blockIdx.y) {
-    mixed = mixed2;
-    out = out2;
-  }
+// Closure network input.  One workgroup per (member, row y, group of G
+// channels); channel ch < nzm is layer ch of q, else layer ch - nzm of psi,
+// so a group's rows all come from one field, one layer apart.  The rows'
+// x-c2r times nn (the 1/(nx ny) of the unnormalised transforms), times the
+// channel's input scale, rounded to bf16 and stored channels-last:
+//   x[((m ny + y) nx + px) C + ch],  C = 2 nzm.
+// The rows' bins and their pre-processing twiddles are loaded into
+// registers, all in flight while the FFT tables are copied into LDS.
+template <typename R, int N2, int G, int NTHR>
+__global__ void __launch_bounds__(NTHR) k_closure_rows_in(const cplx<R>* __restrict__ mq,
+                                                          const cplx<R>* __restrict__ mp,
+                                                          const double* __restrict__ scale, short* __restrict__ x,
+                                                          R nn, Geom g, const cplx<R>* __restrict__ twx,
+                                                          const cplx<R>* __restrict__ twr) {
   using Cx = cplx<R>;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   constexpr int PS = row_pad_shift<R>();
+  constexpr int TOT = G * N2, EPT = (TOT + NTHR - 1) / NTHR;
+  constexpr bool EXACT = EPT * NTHR == TOT;
+  constexpr int SZ = (int)sizeof(Cx);
   Cx* buf = reinterpret_cast<Cx*>(lds_raw);
-  Cx* ts = buf + row_lds_slots<R>(ROWS * N2);
-  Cx* tr = ts + ct_tw_size(N2);
+  Cx* ts = buf + row_lds_slots<R>(TOT);
+  const int C = 2 * g.nzm, ngrp = C / G;
+  const int cg = blockIdx.x % ngrp;
+  const int my = blockIdx.x / ngrp;            // member * ny + y
+  const int m = my / g.ny, y = my - m * g.ny;
+  const int c0 = cg * G;
+  const bool isq = c0 < g.nzm;
+  const int l0 = m * g.nzm + (isq ? c0 : c0 - g.nzm);   // the group's first layer in the batch
+  const int stride = g.ny * g.Kxp;                       // complex values from one layer's row to the next's
+  const __amdgpu_buffer_rsrc_t rs =
+      row_rsrc((isq ? mq : mp) + ((long)l0 * g.ny + y) * g.Kxp, G * stride * SZ);
+  const __amdgpu_buffer_rsrc_t rt = row_rsrc(twr, (N2 + 1) * SZ);
+  const int Kin = g.Kx;
+  Cx va[EPT], vb[EPT], w[EPT];
+#pragma unroll
+  for (int u = 0; u < EPT; ++u) {
+    const int e = threadIdx.x + u * NTHR;
+    const bool ok = EXACT || e < TOT;
+    const int r = e / N2, k = e - r * N2, kk = N2 - k;
+    va[u] = rsrc_ld<Cx>(rs, ok && k < Kin ? (r * stride + k) * SZ : kRowOob);
+    vb[u] = rsrc_ld<Cx>(rs, ok && kk < Kin ? (r * stride + kk) * SZ : kRowOob);
+    w[u] = rsrc_ld<Cx>(rt, ok ? k * SZ : kRowOob);
+  }
   load_table<R>(ts, twx, ct_tw_size(N2));
-  load_table<R>(tr, twr, N2 + 1);
+#pragma unroll
+  for (int u = 0; u < EPT; ++u) {
+    const int e = threadIdx.x + u * NTHR;
+    if (EXACT || e < TOT) buf[padded<PS>(e)] = c2r_pre_w<R>(va[u], vb[u], e % N2, w[u], nn);
+  }
   __syncthreads();
-  const long row0 = (long)blockIdx.x * ROWS;
-  const __amdgpu_buffer_rsrc_t rs = row_rsrc(mixed + row0 * stride, ROWS * stride * (int)sizeof(Cx));
-  c2r_pre_rows<R, N2, ROWS, NTHR, PS>(buf, rs, stride, Kin, tr, scale);
-  __syncthreads();
-  ct_fft<R, N2, CtLayout<ROWS, N2, 1, false, PS>, NTHR>(buf, ts, +1);
-  Cx* o = reinterpret_cast<Cx*>(out) + row0 * N2;
-  for (int e = threadIdx.x; e < ROWS * N2; e += NTHR) o[e] = buf[padded<PS>(e)];
+  ct_fft<R, N2, CtLayout<G, N2, 1, false, PS>, NTHR>(buf, ts, +1);
+  // pixel px is component px & 1 of complex entry px / 2 of each row
+  short* xo = x + (long)my * g.nx * C + c0;
+  for (int px = threadIdx.x; px < 2 * N2; px += NTHR) {
+    unsigned wd[G / 2];
+#pragma unroll
+    for (int k = 0; k < G / 2; ++k) {
+      short h[2];
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const int ch = 2 * k + t;
+        const Cx z = buf[padded<PS>(ch * N2 + (px >> 1))];
+        const R v = (px & 1) ? z.y : z.x;
+        h[t] = bf16_rne((float)((double)v * scale[c0 + ch]));
+      }
+      wd[k] = (unsigned)(unsigned short)h[0] | ((unsigned)(unsigned short)h[1] << 16);
+    }
+    if constexpr (G == 8)
+      *reinterpret_cast<uint4*>(xo + (long)px * C) = make_uint4(wd[0], wd[1], wd[2], wd[3]);
+    else
+      *reinterpret_cast<uint2*>(xo + (long)px * C) = make_uint2(wd[0], wd[1]);
+  }
 }
 
-// Physical rows to mixed rows: out[row][k] = r2c(row)[k] for k < Kx.
-template <typename R, int N2, int ROWS, int NTHR>
-__global__ void __launch_bounds__(NTHR) k_row_r2c_ct(const R* __restrict__ in, cplx<R>* __restrict__ mixed, Geom g,
-                                                     const cplx<R>* __restrict__ twx, const cplx<R>* __restrict__ twr) {
+// Closure network output.  One workgroup per (member, row y, group of G
+// channels): the network's channels-last output y[((m ny + y) nx + px) nzm
+// + ch] times the channel's output scale, x-r2c, into the Kx bins of the
+// mixed rows of layers m nzm + ch.
+template <typename R, int N2, int G, int NTHR>
+__global__ void __launch_bounds__(NTHR) k_closure_rows_out(const float* __restrict__ yn,
+                                                           const double* __restrict__ scale,
+                                                           cplx<R>* __restrict__ mixed, Geom g,
+                                                           const cplx<R>* __restrict__ twx,
+                                                           const cplx<R>* __restrict__ twr) {
   using Cx = cplx<R>;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   constexpr int PS = row_pad_shift<R>();
   Cx* buf = reinterpret_cast<Cx*>(lds_raw);
-  Cx* ts = buf + row_lds_slots<R>(ROWS * N2);
+  Cx* ts = buf + row_lds_slots<R>(G * N2);
   Cx* tr = ts + ct_tw_size(N2);
   load_table<R>(ts, twx, ct_tw_size(N2));
   load_table<R>(tr, twr, N2 + 1);
-  const long row0 = (long)blockIdx.x * ROWS;
-  const Cx* src = reinterpret_cast<const Cx*>(in) + row0 * N2;
-  for (int e = threadIdx.x; e < ROWS * N2; e += NTHR) buf[padded<PS>(e)] = src[e];
+  const int ngrp = g.nzm / G;
+  const int cg = blockIdx.x % ngrp;
+  const int my = blockIdx.x / ngrp;            // member * ny + y
+  const int m = my / g.ny, y = my - m * g.ny;
+  const int c0 = cg * G;
+  // pixel px goes to component px & 1 of complex entry px / 2 of each row
+  R* bre = reinterpret_cast<R*>(buf);
+  const float* yi = yn + (long)my * g.nx * g.nzm + c0;
+  for (int px = threadIdx.x; px < 2 * N2; px += NTHR) {
+    float v[G];
+#pragma unroll
+    for (int k = 0; k < G / 4; ++k) {
+      const float4 f = *reinterpret_cast<const float4*>(yi + (long)px * g.nzm + 4 * k);
+      v[4 * k] = f.x; v[4 * k + 1] = f.y; v[4 * k + 2] = f.z; v[4 * k + 3] = f.w;
+    }
+#pragma unroll
+    for (int ch = 0; ch < G; ++ch)
+      bre[2 * padded<PS>(ch * N2 + (px >> 1)) + (px & 1)] = (R)((double)v[ch] * scale[c0 + ch]);
+  }
   __syncthreads();
-  ct_fft<R, N2, CtLayout<ROWS, N2, 1, false, PS>, NTHR>(buf, ts, -1);
+  ct_fft<R, N2, CtLayout<G, N2, 1, false, PS>, NTHR>(buf, ts, -1);
   const int Kx = g.Kx;
-  for (int e = threadIdx.x; e < ROWS * Kx; e += NTHR) {
+  for (int e = threadIdx.x; e < G * Kx; e += NTHR) {
     const int r = e / Kx, k = e - r * Kx;
-    mixed[(row0 + r) * g.Kxp + k] = r2c_post_pad<R, PS>(buf, r * N2, k, N2, tr);
+    mixed[((long)(m * g.nzm + c0 + r) * g.ny + y) * g.Kxp + k] = r2c_post_pad<R, PS>(buf, r * N2, k, N2, tr);
   }
 }
 

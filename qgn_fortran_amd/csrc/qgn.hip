@@ -282,7 +282,6 @@ struct qgn_ctx {
   // learned closure: network, scales and work arrays
   qgn_cnn* net = nullptr;
   double *cl_in = nullptr, *cl_out = nullptr;   // 2 nzm and nzm scales
-  double *cl_pq = nullptr, *cl_pp = nullptr;
   void *cl_mq = nullptr, *cl_mp = nullptr;      // mixed q and psi (cplx<R>)
   short* cl_x = nullptr;
   float* cl_y = nullptr;
@@ -687,42 +686,41 @@ int rhs_impl(qgn_ctx* c, const double2* qin, void* out, bool psi_ready) {
   return 0;
 }
 
-// Row passes of the closure in the RHS type R: c2r of mixed rows to
-// physical rows (times scale), and r2c of physical rows to the Kx bins of
-// mixed rows; RW rows per workgroup (RW divides ny).  The inverse takes a
-// second array pair (src2 -> dst2) in the same launch when src2 is given.
+// Row passes of the closure in the RHS type R, one workgroup per (member,
+// row, group of G channels): INVERSE, the mixed q and psi to the network's
+// bf16 channels-last input; otherwise the network's output to the mixed
+// rows of `mixed`.  G = 8, or 4 for FP64 rows of 1024 bins (LDS).
 template <typename R, bool INVERSE>
-int launch_row_tx(qgn_ctx* c, const void* src, void* dst, R scale, const void* src2 = nullptr,
-                  void* dst2 = nullptr) {
+int launch_closure_rows(qgn_ctx* c, void* mixed) {
   const Geom& g = c->g;
   const Tw<R>& tw = tw_of<R>(c);
   int rc = 0;
   bool hit = with_size(g.N2, [&](auto nc) {
     constexpr int N2 = decltype(nc)::value;
-    using Cfg = RowCfg<float2, N2, 4>;
-    constexpr int RW = Cfg::ROWS * 4 > 16 ? 16 : Cfg::ROWS * 4;
-    constexpr int NT0 = RW * N2 / 8;
+    constexpr int G = sizeof(R) == 8 && N2 >= 1024 ? 4 : 8;
+    constexpr int NT0 = G * N2 / 8;
     constexpr int NT = NT0 < 64 ? 64 : (NT0 > 1024 ? 1024 : ((NT0 + 63) / 64) * 64);
-    const size_t lds = (size_t)(row_lds_slots<R>(RW * N2) + ct_tw_size(N2) + N2 + 1) * sizeof(cplx<R>);
-    const dim3 grid(g.ny * g.nz / RW);
+    const int nch = INVERSE ? 2 * g.nzm : g.nzm;
+    const dim3 grid(c->nm * g.ny * (nch / G));
     if constexpr (INVERSE) {
-      auto kern = k_row_c2r_ct<R, N2, RW, NT>;
+      const size_t lds = (size_t)(row_lds_slots<R>(G * N2) + ct_tw_size(N2)) * sizeof(cplx<R>);
+      auto kern = k_closure_rows_in<R, N2, G, NT>;
       rc = allow_lds(kern, lds);
       if (rc) return;
-      const dim3 grid2(grid.x, src2 ? 2 : 1);
-      hipLaunchKernelGGL(kern, grid2, dim3(NT), lds, c->stream, reinterpret_cast<const cplx<R>*>(src), g.Kx, g.Kxp,
-                         reinterpret_cast<R*>(dst), scale, g, tw.sx, tw.r,
-                         reinterpret_cast<const cplx<R>*>(src2 ? src2 : src), reinterpret_cast<R*>(src2 ? dst2 : dst));
+      hipLaunchKernelGGL(kern, grid, dim3(NT), lds, c->stream, reinterpret_cast<const cplx<R>*>(c->cl_mq),
+                         reinterpret_cast<const cplx<R>*>(c->cl_mp), c->cl_in, c->cl_x,
+                         (R)(1.0 / ((double)g.nx * (double)g.ny)), g, tw.sx, tw.r);
     } else {
-      auto kern = k_row_r2c_ct<R, N2, RW, NT>;
+      const size_t lds = (size_t)(row_lds_slots<R>(G * N2) + ct_tw_size(N2) + N2 + 1) * sizeof(cplx<R>);
+      auto kern = k_closure_rows_out<R, N2, G, NT>;
       rc = allow_lds(kern, lds);
       if (rc) return;
-      hipLaunchKernelGGL(kern, grid, dim3(NT), lds, c->stream, reinterpret_cast<const R*>(src),
-                         reinterpret_cast<cplx<R>*>(dst), g, tw.sx, tw.r);
+      hipLaunchKernelGGL(kern, grid, dim3(NT), lds, c->stream, c->cl_y, c->cl_out,
+                         reinterpret_cast<cplx<R>*>(mixed), g, tw.sx, tw.r);
     }
-    rc = hipGetLastError() == hipSuccess ? 0 : fail(QGN_EHIP, "row transform launch failed");
+    rc = hipGetLastError() == hipSuccess ? 0 : fail(QGN_EHIP, "closure row pass launch failed");
   }, CtSizes{});
-  if (!hit) return fail(QGN_EINVAL, "nx/2 not supported by the row transforms");
+  if (!hit) return fail(QGN_EINVAL, "nx/2 not supported by the closure row passes");
   return rc;
 }
 
@@ -756,28 +754,12 @@ int closure_add(qgn_ctx* c, const double2* qin, void* out) {
   using R = S;
   const Geom& g = c->g;
   hipStream_t s = c->stream;
-  const long np = (long)g.nz * g.ny * g.nx;
   // the RHS pass's P1 formed the mixed q and psi (cl_mq, cl_mp)
-  const R nn = (R)(1.0 / ((double)g.nx * (double)g.ny));
-  R* pq = reinterpret_cast<R*>(c->cl_pq);
-  R* pp = reinterpret_cast<R*>(c->cl_pp);
-  CHK((launch_row_tx<R, true>(c, c->cl_mq, pq, nn, c->cl_mp, pp)));
-  const long npx = np / g.nzm;   // members x pixels: one thread each
-  if (g.nzm == 8)
-    hipLaunchKernelGGL((k_closure_in<R, 8>), dim3(grid1d(npx)), dim3(256), 0, s, pq, pp, c->cl_in, c->cl_x, g);
-  else if (g.nzm == 16)
-    hipLaunchKernelGGL((k_closure_in<R, 16>), dim3(grid1d(npx)), dim3(256), 0, s, pq, pp, c->cl_in, c->cl_x, g);
-  else
-    return fail(QGN_EINVAL, "closure network: nz must be 8 or 16");
-  HIPCHK(hipGetLastError());
+  if (g.nzm != 8 && g.nzm != 16) return fail(QGN_EINVAL, "closure network: nz must be 8 or 16");
+  CHK((launch_closure_rows<R, true>(c, nullptr)));
   if (qcnn::forward(c->net, reinterpret_cast<const unsigned short*>(c->cl_x), c->nm, g.ny, g.nx, c->cl_y, s))
     return fail(QGN_EINVAL, std::string("closure network: ") + qcnn::last_error());
-  if (g.nzm == 8)
-    hipLaunchKernelGGL((k_closure_out<R, 8>), dim3(grid1d(npx)), dim3(256), 0, s, c->cl_y, c->cl_out, pq, g);
-  else
-    hipLaunchKernelGGL((k_closure_out<R, 16>), dim3(grid1d(npx)), dim3(256), 0, s, c->cl_y, c->cl_out, pq, g);
-  HIPCHK(hipGetLastError());
-  CHK((launch_row_tx<R, false>(c, pq, c->Jm, (R)1)));
+  CHK((launch_closure_rows<R, false>(c, c->Jm)));
   return launch_col_fwd_add<R, S>(c, c->Jm, out);
 }
 
@@ -1287,7 +1269,7 @@ void destroy_ctx(qgn_ctx* c) {
   for (int i = 0; i < 5; ++i)
     if (c->mp[i]) (void)hipFree(c->mp[i]);
   if (c->spec) (void)hipFree(c->spec);
-  void* cl[] = {c->cl_in, c->cl_out, c->cl_pq, c->cl_pp, c->cl_mq, c->cl_mp, c->cl_x, c->cl_y};
+  void* cl[] = {c->cl_in, c->cl_out, c->cl_mq, c->cl_mp, c->cl_x, c->cl_y};
   for (void* x : cl)
     if (x) (void)hipFree(x);
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
@@ -1723,11 +1705,8 @@ int qgn_set_closure(qgn_ctx* c, qgn_cnn* net, const double* in_scale, const doub
   const long pix = (long)c->nm * g.ny * g.nx;
   if (g.nx % 16 || pix % 64) return fail(QGN_EINVAL, "closure network needs nx % 16 == 0");
   if (!c->cl_in) {
-    const long np = (long)g.nz * g.ny * g.nx;
     CHK(alloc((void**)&c->cl_in, 2 * (size_t)g.nzm * sizeof(double)));
     CHK(alloc((void**)&c->cl_out, (size_t)g.nzm * sizeof(double)));
-    CHK(alloc((void**)&c->cl_pq, np * sizeof(double)));
-    CHK(alloc((void**)&c->cl_pp, np * sizeof(double)));
     CHK(alloc(&c->cl_mq, (size_t)c->nmix * c->rsize));
     CHK(alloc(&c->cl_mp, (size_t)c->nmix * c->rsize));
     CHK(alloc((void**)&c->cl_x, pix * 2 * g.nzm * sizeof(short)));
