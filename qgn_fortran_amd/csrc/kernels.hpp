@@ -183,6 +183,7 @@ struct RowJacArgs {
 // drag cdrag (i ky du_hat - i kx dv_hat) of the row pass's |u| u, |u| v.
 struct ColRhsArgs {
   const void* J;         // mixed cplx<R>
+  const void* M;         // mixed cplx<R>: the closure network's forcing, or null (RHS += FFT_y(M))
   const double2* q;      // compact
   const void* q32;       // compact cplx<R>, q rounded to R (null: read q)
   const void* psi;       // compact cplx<R>

@@ -357,7 +357,8 @@ __device__ __forceinline__ void rhs_tile(int b, int total, int ntile, int nzm, i
   }
 }
 
-// P3: y-forward FFT of J, then
+// P3: y-forward FFT of J (of J - M with the closure network's forcing M),
+// then
 //   RHS = -J_hat - Ubar_k i kx q - (beta + Qy_k) i kx psi
 //         [- (A2 k2 + A8 k8) q  if is_explicit]  [+ r0 Htot/H_nz k2 psi on layer nz]
 //         [+ C_d Htot/H_nz (i ky du - i kx dv) on layer nz, QGN_Module.f90:284-292]
@@ -386,6 +387,28 @@ __device__ __forceinline__ void load_cols(cplx<R>* buf, const cplx<R>* __restric
   }
 }
 
+// The same tile of a - b.
+template <typename R, int C, int NTHR, int TOT>
+__device__ __forceinline__ void load_cols_sub(cplx<R>* buf, const cplx<R>* __restrict__ a,
+                                              const cplx<R>* __restrict__ b, long stride, int i0) {
+  constexpr int EPT = (TOT + NTHR - 1) / NTHR;
+  constexpr bool EXACT = EPT * NTHR == TOT;
+  cplx<R> v[EPT], w[EPT];
+#pragma unroll
+  for (int u = 0; u < EPT; ++u) {
+    const int e = threadIdx.x + u * NTHR;
+    const int ee = (EXACT || e < TOT) ? e : TOT - 1;
+    const long o = (long)(ee / C) * stride + i0 + ee % C;
+    v[u] = a[o];
+    w[u] = b[o];
+  }
+#pragma unroll
+  for (int u = 0; u < EPT; ++u) {
+    const int e = threadIdx.x + u * NTHR;
+    if (EXACT || e < TOT) buf[e] = cmk<cplx<R>>(v[u].x - w[u].x, v[u].y - w[u].y);
+  }
+}
+
 template <typename R, typename S, int NY, int C, int NTHR>
 __global__ void __launch_bounds__(NTHR) k_col_rhs_ct(ColRhsArgs a, Geom g, const cplx<R>* __restrict__ twy) {
   using Cx = cplx<R>;
@@ -400,7 +423,11 @@ __global__ void __launch_bounds__(NTHR) k_col_rhs_ct(ColRhsArgs a, Geom g, const
   rhs_tile(blockIdx.x, gridDim.x, ntile, g.nzm, a.drag, &z, &i0);   // grid = ntile * nz
   i0 *= C;
   load_table<R>(tw, twy, ct_tw_size(NY));
-  load_cols<R, C, NTHR, TOT>(buf, reinterpret_cast<const Cx*>(a.J) + (long)z * NY * g.Kxp, g.Kxp, i0);
+  if (a.M)   // the forcing enters as -FFT_y(J - M)
+    load_cols_sub<R, C, NTHR, TOT>(buf, reinterpret_cast<const Cx*>(a.J) + (long)z * NY * g.Kxp,
+                                   reinterpret_cast<const Cx*>(a.M) + (long)z * NY * g.Kxp, g.Kxp, i0);
+  else
+    load_cols<R, C, NTHR, TOT>(buf, reinterpret_cast<const Cx*>(a.J) + (long)z * NY * g.Kxp, g.Kxp, i0);
   __syncthreads();
   ct_fft<R, NY, L, NTHR>(buf, tw, -1);
   cplx<S>* out = reinterpret_cast<cplx<S>*>(a.out);
@@ -719,41 +746,6 @@ __global__ void __launch_bounds__(NTHR) k_closure_rows_out(const float* __restri
   }
 }
 
-// y-forward FFT of mixed columns, added to a compact array on the kept
-// modes ((0,0) excluded): out += FFT_y(mixed).
-template <typename R, typename S, int NY, int C, int NTHR>
-__global__ void __launch_bounds__(NTHR) k_col_fwd_add_ct(const cplx<R>* __restrict__ mixed, cplx<S>* __restrict__ out,
-                                                         Geom g, const cplx<R>* __restrict__ twy) {
-  using Cx = cplx<R>;
-  extern __shared__ __align__(16) unsigned char lds_raw[];
-  Cx* buf = reinterpret_cast<Cx*>(lds_raw);
-  Cx* tw = buf + C * NY;
-  using L = CtLayout<C, 1, C, true>;
-  constexpr int TOT = C * NY;
-  const int ntile = g.Kxp / C;
-  const int w = xcd_tile(blockIdx.x, gridDim.x);   // grid = ntile * nz
-  const int i0 = (w % ntile) * C;
-  const int z = w / ntile;
-  load_table<R>(tw, twy, ct_tw_size(NY));
-  for (int e = threadIdx.x; e < TOT; e += NTHR) {
-    const int c = e % C, j = e / C;
-    buf[e] = mixed[((long)z * NY + j) * g.Kxp + i0 + c];
-  }
-  __syncthreads();
-  ct_fft<R, NY, L, NTHR>(buf, tw, -1);
-  const int nown = C * g.Ky;
-  for (int o = threadIdx.x; o < nown; o += NTHR) {
-    const int c = o % C, jc = o / C, i = i0 + c;
-    const int j = compact_row_to_full(jc, g);
-    if (i >= g.Kx || (i == 0 && j == 0)) continue;
-    const long idx = ((long)z * g.Ky + jc) * g.Kxp + i;
-    const Cx v = buf[j * C + c];
-    cplx<S> r = out[idx];
-    r.x += (S)v.x;
-    r.y += (S)v.y;
-    out[idx] = r;
-  }
-}
 
 // Final max over the per-block partials: workgroup b reduces partials
 // [b n, (b+1) n) into slot[b] (one per member of a batch).

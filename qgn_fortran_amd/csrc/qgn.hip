@@ -635,7 +635,11 @@ int launch_col_rhs(qgn_ctx* c, const ColRhsArgs& ka0, bool drag) {
   return rc;
 }
 
-// GetRHS on compact arrays: out = RHS(qin)
+template <typename R>
+int closure_forcing(qgn_ctx* c);
+
+// GetRHS on compact arrays: out = RHS(qin), with the closure network's
+// forcing when one is attached
 template <typename R>
 int rhs_impl(qgn_ctx* c, const double2* qin, void* out, bool psi_ready) {
   void* psi = std::is_same<R, double>::value ? (void*)c->psi64 : c->psi;
@@ -664,7 +668,7 @@ int rhs_impl(qgn_ctx* c, const double2* qin, void* out, bool psi_ready) {
   const void* qsrc = q32 ? q32 : (const void*)qin;
   ca.src[2] = qsrc; ca.src_is_double[2] = !q32; ca.mult[2] = 2; ca.dst[2] = c->QXm;   // q_x
   ca.src[3] = qsrc; ca.src_is_double[3] = !q32; ca.mult[3] = 3; ca.dst[3] = c->QYm;   // q_y
-  if (c->net) {   // the closure network's inputs q and psi, mixed (closure_add follows)
+  if (c->net) {   // the closure network's inputs q and psi, mixed (closure_forcing follows)
     ca.nf = 6;
     ca.src[4] = qsrc; ca.src_is_double[4] = !q32; ca.mult[4] = 0; ca.dst[4] = c->cl_mq;
     ca.src[5] = psi; ca.src_is_double[5] = isd; ca.mult[5] = 0; ca.dst[5] = c->cl_mp;
@@ -674,8 +678,9 @@ int rhs_impl(qgn_ctx* c, const double2* qin, void* out, bool psi_ready) {
   ra.U = c->Um; ra.V = c->Vm; ra.QX = c->QXm; ra.QY = c->QYm; ra.J = c->Jm; ra.DU = c->DUm; ra.DV = c->DVm;
   ra.drag = c->p.C_d != 0.0;
   CHK(launch_row_jac<R>(c, ra));
+  if (c->net) CHK(closure_forcing<R>(c));
   ColRhsArgs ka{};
-  ka.J = c->Jm; ka.q = qin; ka.q32 = q32; ka.psi = psi; ka.out = out;
+  ka.J = c->Jm; ka.M = c->net ? c->cl_mq : nullptr; ka.q = qin; ka.q32 = q32; ka.psi = psi; ka.out = out;
   ka.ubar = c->ubar; ka.qybar = c->qybar;
   ka.beta = c->p.beta;
   ka.ekman = c->ekm;
@@ -724,52 +729,24 @@ int launch_closure_rows(qgn_ctx* c, void* mixed) {
   return rc;
 }
 
-// out (compact, type S) += y-forward FFT of mixed (type R) on the kept modes
-template <typename R, typename S>
-int launch_col_fwd_add(qgn_ctx* c, const void* mixed, void* out) {
+// Learned closure forcing, in the RHS's arithmetic type (the network itself
+// takes bf16): M = x-r2c(out_scale net(in_scale (q, psi))) as mixed rows in
+// cl_mq, q and psi the physical fields of the RHS input and of its psi.
+// Runs between P2 and P3 of that RHS, whose P1 also formed the mixed q and
+// psi (cl_mq, cl_mp); P3 then adds FFT_y(M) on the kept modes.
+template <typename R>
+int closure_forcing(qgn_ctx* c) {
   const Geom& g = c->g;
-  const Tw<R>& tw = tw_of<R>(c);
-  int rc = 0;
-  bool hit = with_size(g.ny, [&](auto nyc) {
-    constexpr int NY = decltype(nyc)::value;
-    using Cfg = ColCfg<R, NY>;
-    auto kern = k_col_fwd_add_ct<R, S, NY, Cfg::C, Cfg::NTHR>;
-    rc = allow_lds(kern, Cfg::LDS);
-    if (rc) return;
-    hipLaunchKernelGGL(kern, dim3(g.Kxp / Cfg::C * g.nz), dim3(Cfg::NTHR), Cfg::LDS, c->stream,
-                       reinterpret_cast<const cplx<R>*>(mixed), reinterpret_cast<cplx<S>*>(out), g, tw.sy);
-    rc = hipGetLastError() == hipSuccess ? 0 : fail(QGN_EHIP, "column transform launch failed");
-  }, CtSizes{});
-  if (!hit) return fail(QGN_EINVAL, "ny not supported by the column transforms");
-  return rc;
-}
-
-// Learned closure: out += r2c(net(q, psi) * scale) on the kept modes, with
-// q, psi the physical fields of qin and of the psi the RHS pass just formed,
-// all in the RHS's arithmetic type (the network itself takes bf16).  Runs
-// right after that RHS pass, whose P1 also formed the mixed q and psi;
-// reuses its mixed J buffer, which it has finished with.
-template <typename S>
-int closure_add(qgn_ctx* c, const double2* qin, void* out) {
-  using R = S;
-  const Geom& g = c->g;
-  hipStream_t s = c->stream;
-  // the RHS pass's P1 formed the mixed q and psi (cl_mq, cl_mp)
   if (g.nzm != 8 && g.nzm != 16) return fail(QGN_EINVAL, "closure network: nz must be 8 or 16");
   CHK((launch_closure_rows<R, true>(c, nullptr)));
-  if (qcnn::forward(c->net, reinterpret_cast<const unsigned short*>(c->cl_x), c->nm, g.ny, g.nx, c->cl_y, s))
+  if (qcnn::forward(c->net, reinterpret_cast<const unsigned short*>(c->cl_x), c->nm, g.ny, g.nx, c->cl_y, c->stream))
     return fail(QGN_EINVAL, std::string("closure network: ") + qcnn::last_error());
-  CHK((launch_closure_rows<R, false>(c, c->Jm)));
-  return launch_col_fwd_add<R, S>(c, c->Jm, out);
+  return launch_closure_rows<R, false>(c, c->cl_mq);   // its input rows are consumed
 }
 
 int rhs(qgn_ctx* c, const double2* qin, void* out, bool psi_ready = false) {
-  if (c->p.precision == QGN_MIXED) {
-    CHK(rhs_impl<float>(c, qin, out, psi_ready));
-    return c->net ? closure_add<float>(c, qin, out) : 0;
-  }
-  CHK(rhs_impl<double>(c, qin, out, psi_ready));
-  return c->net ? closure_add<double>(c, qin, out) : 0;
+  if (c->p.precision == QGN_MIXED) return rhs_impl<float>(c, qin, out, psi_ready);
+  return rhs_impl<double>(c, qin, out, psi_ready);
 }
 
 // max over the physical grid of |c2r(E)| (unnormalised), FP64, per member
