@@ -864,7 +864,8 @@ __global__ void k_leith_final(const double2* __restrict__ q, Geom g, const doubl
   __shared__ double red[16];
   const int z = blockIdx.x;
   const long plane = (long)g.Ky * g.Kxp;
-  double s = 0.0;
+  // the partial sums enter the block reduction one per thread (nblk <= blockDim.x)
+  double s = (int)threadIdx.x < nblk ? partial[z * nblk + threadIdx.x] : 0.0;
   for (int j = threadIdx.x; j < g.ny; j += blockDim.x) {
     const int jc = full_row_to_compact(j, g);
     const int jm = (g.ny - j) % g.ny;
@@ -886,7 +887,6 @@ __global__ void k_leith_final(const double2* __restrict__ q, Geom g, const doubl
   if (threadIdx.x == 0) {
     double t = 0.0;
     for (int w = 0; w < (int)(blockDim.x + 63) / 64; ++w) t += red[w];
-    for (int b2 = 0; b2 < nblk; ++b2) t += partial[z * nblk + b2];
     const double nn = (double)g.nx * (double)g.ny;
     a8l[z] = c12 * sqrt(t) / nn;
   }
