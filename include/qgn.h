@@ -252,7 +252,8 @@ const char* qgn_cnn_last_error(void);
  * adds r2c(F) on the kept modes, where F = out_scale * net(in_scale * (q,
  * psi)) is evaluated on the physical fields of each member (input channels
  * q_1..q_nz, psi_1..psi_nz; output one forcing per layer).  The network
- * needs cin = 2 nz and cout = nz; nx % 16 == 0.  The context keeps the
+ * needs cin = 2 nz and cout = nz; nx % 16 == 0 (and nx <= 1024 on the
+ * FP64 path).  The context keeps the
  * pointer: destroy the network only after detaching it.  An ensemble built
  * before this call keeps the attempt it captured. */
 int qgn_set_closure(qgn_ctx* ctx, qgn_cnn* net, const double* in_scale, const double* out_scale);

@@ -629,7 +629,8 @@ __global__ void __launch_bounds__(NTHR) k_row_maxabs_ct(const cplx<R>* __restric
 // channels); channel ch < nzm is layer ch of q, else layer ch - nzm of psi,
 // so a group's rows all come from one field, one layer apart.  The rows'
 // x-c2r times nn (the 1/(nx ny) of the unnormalised transforms), times the
-// channel's input scale, rounded to bf16 and stored channels-last:
+// channel's input scale, rounded to bf16 and stored channels-last (G = 8:
+// one 16-byte store per pixel):
 //   x[((m ny + y) nx + px) C + ch],  C = 2 nzm.
 // The rows' bins and their pre-processing twiddles are loaded into
 // registers, all in flight while the FFT tables are copied into LDS.
@@ -639,6 +640,7 @@ __global__ void __launch_bounds__(NTHR) k_closure_rows_in(const cplx<R>* __restr
                                                           const double* __restrict__ scale, short* __restrict__ x,
                                                           R nn, Geom g, const cplx<R>* __restrict__ twx,
                                                           const cplx<R>* __restrict__ twr) {
+  static_assert(G == 8, "one 16-byte store of 8 bf16 channels per pixel");
   using Cx = cplx<R>;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   constexpr int PS = row_pad_shift<R>();
@@ -693,10 +695,7 @@ __global__ void __launch_bounds__(NTHR) k_closure_rows_in(const cplx<R>* __restr
       }
       wd[k] = (unsigned)(unsigned short)h[0] | ((unsigned)(unsigned short)h[1] << 16);
     }
-    if constexpr (G == 8)
-      *reinterpret_cast<uint4*>(xo + (long)px * C) = make_uint4(wd[0], wd[1], wd[2], wd[3]);
-    else
-      *reinterpret_cast<uint2*>(xo + (long)px * C) = make_uint2(wd[0], wd[1]);
+    *reinterpret_cast<uint4*>(xo + (long)px * C) = make_uint4(wd[0], wd[1], wd[2], wd[3]);
   }
 }
 

@@ -692,9 +692,10 @@ int rhs_impl(qgn_ctx* c, const double2* qin, void* out, bool psi_ready) {
 }
 
 // Row passes of the closure in the RHS type R, one workgroup per (member,
-// row, group of G channels): INVERSE, the mixed q and psi to the network's
-// bf16 channels-last input; otherwise the network's output to the mixed
-// rows of `mixed`.  G = 8, or 4 for FP64 rows of 1024 bins (LDS).
+// row, group of G = 8 channels): INVERSE, the mixed q and psi to the
+// network's bf16 channels-last input; otherwise the network's output to the
+// mixed rows of `mixed`.  (FP64 rows of 1024 complex values would not fit
+// LDS: qgn_set_closure rejects nx > 1024 on the FP64 path.)
 template <typename R, bool INVERSE>
 int launch_closure_rows(qgn_ctx* c, void* mixed) {
   const Geom& g = c->g;
@@ -702,7 +703,7 @@ int launch_closure_rows(qgn_ctx* c, void* mixed) {
   int rc = 0;
   bool hit = with_size(g.N2, [&](auto nc) {
     constexpr int N2 = decltype(nc)::value;
-    constexpr int G = sizeof(R) == 8 && N2 >= 1024 ? 4 : 8;
+    constexpr int G = 8;
     constexpr int NT0 = G * N2 / 8;
     constexpr int NT = NT0 < 64 ? 64 : (NT0 > 1024 ? 1024 : ((NT0 + 63) / 64) * 64);
     const int nch = INVERSE ? 2 * g.nzm : g.nzm;
@@ -1681,6 +1682,8 @@ int qgn_set_closure(qgn_ctx* c, qgn_cnn* net, const double* in_scale, const doub
     return fail(QGN_EINVAL, "closure network needs 2*nz input and nz output channels");
   const long pix = (long)c->nm * g.ny * g.nx;
   if (g.nx % 16 || pix % 64) return fail(QGN_EINVAL, "closure network needs nx % 16 == 0");
+  if (c->p.precision != QGN_MIXED && g.nx > 1024)
+    return fail(QGN_EINVAL, "closure network: the FP64 path supports nx <= 1024 (the mixed path nx <= 2048)");
   if (!c->cl_in) {
     CHK(alloc((void**)&c->cl_in, 2 * (size_t)g.nzm * sizeof(double)));
     CHK(alloc((void**)&c->cl_out, (size_t)g.nzm * sizeof(double)));

@@ -95,8 +95,8 @@ def test_training_reduces_loss_on_a_teacher():
     net.close()
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_closure_forcing_in_rhs(precision):
+@pytest.mark.parametrize("precision,nz", [("fp64", 8), ("mixed", 8), ("fp64", 16), ("mixed", 16)])
+def test_closure_forcing_in_rhs(precision, nz):
     """qgn_set_closure: GetRHS with the network attached minus GetRHS without
     it is r2c(out_scale * net(in_scale * (q, psi))) on the kept modes, with q
     and psi the physical fields of the state (the oracle's WriteQP fields,
@@ -106,20 +106,22 @@ def test_closure_forcing_in_rhs(precision):
     from oracle import qgn_oracle as orc
     from qgn_fortran_amd import QGN
     from qgn_fortran_amd.cnn import CNN
-    case = make_case(64, 64, 8, k_peak=6, precision=precision)
+    case = make_case(64, 64, nz, k_peak=6, precision=precision)
     om = orc.Model(case["po"], case["H"], case["Modes"], case["EVals"], case["uBar"], case["qyBar"])
     qh = om.initialize(case["q0"])
     qp, pp = om.get_psi_phys(qh)
-    s_in = np.concatenate([np.full(8, 1.0 / np.std(qp)), np.full(8, 1.0 / np.std(pp))])
+    # per-channel input scales that differ, so a channel routed to the wrong
+    # place would show
+    s_in = np.concatenate([np.linspace(0.8, 1.2, nz) / np.std(qp), np.linspace(1.1, 0.9, nz) / np.std(pp)])
     base = om.get_rhs(qh)
-    s_out = np.full(8, 0.3 * np.sqrt(np.mean(om.c2r(base) ** 2)) / (64 * 64))
-    params = co.init_params(16, 8, seed=3, last_scale=1.0)
-    x = np.concatenate([qp, pp], axis=0).transpose(1, 2, 0)[None] * s_in      # (1, H, W, 16)
-    y, _ = co.forward(params, x.astype(np.float32), 16, 8, bf16_points=True)
+    s_out = np.linspace(0.2, 0.4, nz) * np.sqrt(np.mean(om.c2r(base) ** 2)) / (64 * 64)
+    params = co.init_params(2 * nz, nz, seed=3, last_scale=1.0)
+    x = np.concatenate([qp, pp], axis=0).transpose(1, 2, 0)[None] * s_in      # (1, H, W, 2 nz)
+    y, _ = co.forward(params, x.astype(np.float32), 2 * nz, nz, bf16_points=True)
     F = orc.dealias(orc.rfft2(y[0].transpose(2, 0, 1) * s_out[:, None, None]), 64, 64)
     gm = QGN(case["pp"], case["H"], case["Modes"], case["EVals"], case["uBar"], case["qyBar"])
     r0 = gm.get_rhs(qh)
-    net = CNN(16, 8, params)
+    net = CNN(2 * nz, nz, params)
     gm.set_closure(net, s_in, s_out)
     r1 = gm.get_rhs(qh)
     err = rel_l2(r1 - r0, F)
@@ -136,6 +138,28 @@ def test_closure_forcing_in_rhs(precision):
     assert rel_l2(s1, g2.get_state()) > 1e-9
     gm.close()
     g2.close()
+    net.close()
+
+
+def test_closure_fp64_row_limit():
+    """The FP64 path's closure row passes hold 8 rows of nx/2 FP64 complex
+    values in LDS, so qgn_set_closure rejects nx > 1024 there with an error
+    (and the mixed path accepts it)."""
+    from conftest import make_case
+    from oracle import cnn_oracle as co
+    from qgn_fortran_amd import QGN
+    from qgn_fortran_amd.cnn import CNN
+    net = CNN(16, 8, co.init_params(16, 8, seed=1))
+    for precision in ("fp64", "mixed"):
+        case = make_case(2048, 64, 8, precision=precision)
+        gm = QGN(case["pp"], case["H"], case["Modes"], case["EVals"], case["uBar"], case["qyBar"])
+        if precision == "fp64":
+            with pytest.raises(RuntimeError, match="nx <= 1024"):
+                gm.set_closure(net, np.ones(16), np.ones(8))
+        else:
+            gm.set_closure(net, np.ones(16), np.ones(8))
+            gm.set_closure(None)
+        gm.close()
     net.close()
 
 
