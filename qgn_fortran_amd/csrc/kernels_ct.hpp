@@ -53,13 +53,19 @@ __global__ void __launch_bounds__(NTHR) k_col_inv_ct(ColInvArgs a, Geom g, const
     // lanes past the tile) load as zero, and all EPT loads of a thread are
     // in flight together
     constexpr int EPT = (TOT + NTHR - 1) / NTHR;
+    // FP32 tiles move two adjacent columns per 16-byte access: element u of
+    // a thread is column pair (tid + (u/2) NTHR), member u & 1
+    constexpr bool PAIR = sizeof(Cx) == 8 && EPT % 2 == 0 && C % 2 == 0;
+    auto elem = [&](int u) {
+      return PAIR ? 2 * ((int)threadIdx.x + (u >> 1) * NTHR) + (u & 1) : (int)threadIdx.x + u * NTHR;
+    };
     const int lbytes = g.Ky * g.Kxp * (isd ? (int)sizeof(double2) : (int)sizeof(Cx));
     const __amdgpu_buffer_rsrc_t rs =
         row_rsrc(static_cast<const unsigned char*>(a.src[f]) + (long)z * lbytes, lbytes);
     int off[EPT];
 #pragma unroll
     for (int u = 0; u < EPT; ++u) {
-      const int e = threadIdx.x + u * NTHR;
+      const int e = elem(u);
       const int c = e % C, j = e / C;
       const int jc = e < TOT ? full_row_to_compact(j, g) : -1;
       off[u] = jc >= 0 ? jc * g.Kxp + i0 + c : -1;
@@ -69,13 +75,21 @@ __global__ void __launch_bounds__(NTHR) k_col_inv_ct(ColInvArgs a, Geom g, const
 #pragma unroll
       for (int u = 0; u < EPT; ++u)
         v[u] = to_c<R>(rsrc_ld<double2>(rs, off[u] >= 0 ? off[u] * (int)sizeof(double2) : kRowOob));
+    } else if constexpr (PAIR) {
+#pragma unroll
+      for (int u = 0; u < EPT; u += 2) {
+        const double2 d = rsrc_ld<double2>(rs, off[u] >= 0 ? off[u] * (int)sizeof(Cx) : kRowOob);
+        const float4 p4 = __builtin_bit_cast(float4, d);
+        v[u] = cmk<Cx>(p4.x, p4.y);
+        v[u + 1] = cmk<Cx>(p4.z, p4.w);
+      }
     } else {
 #pragma unroll
       for (int u = 0; u < EPT; ++u) v[u] = rsrc_ld<Cx>(rs, off[u] >= 0 ? off[u] * (int)sizeof(Cx) : kRowOob);
     }
 #pragma unroll
     for (int u = 0; u < EPT; ++u) {
-      const int e = threadIdx.x + u * NTHR;
+      const int e = elem(u);
       if (EPT * NTHR != TOT && e >= TOT) continue;
       const int c = e % C, j = e / C, i = i0 + c;
       Cx x = v[u];
@@ -94,9 +108,17 @@ __global__ void __launch_bounds__(NTHR) k_col_inv_ct(ColInvArgs a, Geom g, const
     __syncthreads();
     ct_fft<R, NY, L, NTHR>(buf, tw, +1);
     Cx* dst = reinterpret_cast<Cx*>(a.dst[f]);
-    for (int e = threadIdx.x; e < TOT; e += NTHR) {
-      const int c = e % C, j = e / C;
-      dst[((long)z * NY + j) * g.Kxp + i0 + c] = buf[e];
+    if constexpr (PAIR) {
+      for (int e2 = threadIdx.x; 2 * e2 < TOT; e2 += NTHR) {
+        const int e = 2 * e2, c = e % C, j = e / C;
+        *reinterpret_cast<float4*>(dst + ((long)z * NY + j) * g.Kxp + i0 + c) =
+            reinterpret_cast<const float4*>(buf)[e2];
+      }
+    } else {
+      for (int e = threadIdx.x; e < TOT; e += NTHR) {
+        const int c = e % C, j = e / C;
+        dst[((long)z * NY + j) * g.Kxp + i0 + c] = buf[e];
+      }
     }
     __syncthreads();
   }
@@ -373,6 +395,18 @@ template <typename R, int C, int NTHR, int TOT>
 __device__ __forceinline__ void load_cols(cplx<R>* buf, const cplx<R>* __restrict__ src, long stride, int i0) {
   constexpr int EPT = (TOT + NTHR - 1) / NTHR;
   constexpr bool EXACT = EPT * NTHR == TOT;
+  if constexpr (sizeof(cplx<R>) == 8 && C % 2 == 0 && EPT % 2 == 0 && EXACT) {
+    // FP32: two adjacent columns per 16-byte load and LDS store
+    float4 v[EPT / 2];
+#pragma unroll
+    for (int k = 0; k < EPT / 2; ++k) {
+      const int e = 2 * ((int)threadIdx.x + k * NTHR);
+      v[k] = *reinterpret_cast<const float4*>(src + (long)(e / C) * stride + i0 + e % C);
+    }
+#pragma unroll
+    for (int k = 0; k < EPT / 2; ++k) reinterpret_cast<float4*>(buf)[threadIdx.x + k * NTHR] = v[k];
+    return;
+  }
   cplx<R> v[EPT];
 #pragma unroll
   for (int u = 0; u < EPT; ++u) {
@@ -393,6 +427,21 @@ __device__ __forceinline__ void load_cols_sub(cplx<R>* buf, const cplx<R>* __res
                                               const cplx<R>* __restrict__ b, long stride, int i0) {
   constexpr int EPT = (TOT + NTHR - 1) / NTHR;
   constexpr bool EXACT = EPT * NTHR == TOT;
+  if constexpr (sizeof(cplx<R>) == 8 && C % 2 == 0 && EPT % 2 == 0 && EXACT) {
+    float4 v[EPT / 2], w[EPT / 2];
+#pragma unroll
+    for (int k = 0; k < EPT / 2; ++k) {
+      const int e = 2 * ((int)threadIdx.x + k * NTHR);
+      const long o = (long)(e / C) * stride + i0 + e % C;
+      v[k] = *reinterpret_cast<const float4*>(a + o);
+      w[k] = *reinterpret_cast<const float4*>(b + o);
+    }
+#pragma unroll
+    for (int k = 0; k < EPT / 2; ++k)
+      reinterpret_cast<float4*>(buf)[threadIdx.x + k * NTHR] =
+          make_float4(v[k].x - w[k].x, v[k].y - w[k].y, v[k].z - w[k].z, v[k].w - w[k].w);
+    return;
+  }
   cplx<R> v[EPT], w[EPT];
 #pragma unroll
   for (int u = 0; u < EPT; ++u) {
