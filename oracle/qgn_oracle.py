@@ -2,7 +2,8 @@
 
 TEST INFRASTRUCTURE ONLY.  Nothing in the product imports this module: only
 ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
-``bench.py`` use it, and only as the checker (or as the timed CPU baseline).
+``bench.py`` use it: as the reference the GPU results are compared against,
+or as the timed CPU baseline.
 
 Arrays use numpy C order ``(nz, ny, nx//2+1)`` for spectra and ``(nz, ny, nx)``
 for physical fields.  That is byte-for-byte the reference's Fortran
