@@ -285,25 +285,26 @@ __global__ void __launch_bounds__(256) k_conv3x3_tile(const u16* __restrict__ in
   }
 }
 
-// Weight and bias gradients of one layer:
-//   dW[co][tap][ci] += sum_p dY[p][co] X[p + off(tap)][ci],  db[co] += sum_p dY[p][co]
-// X: (n, H, W, CI) bf16 (layer input), dY: (n, H, W, COP) bf16 (gradient of
-// the layer's pre-activation, channels >= CO zero).  Block: `rows` image
-// rows of image blockIdx.y; per row the three input rows and the gradient
-// row are staged in LDS, and each wave accumulates its share of the
-// 9 (COP/16) (CI/16) 16x16 tiles over 16-pixel chunks.
+// Weight and bias gradients of one layer, per workgroup:
+//   dW[co][tap][ci] = sum_p dY[p][co] X[p + off(tap)][ci],  db[co] = sum_p dY[p][co]
+// over the workgroup's pixels, into its slice of `part` (CO 9 CI weights,
+// then CO biases); k_wgrad_reduce adds the slices in a fixed order, so a
+// training step is reproducible.  X: (n, H, W, CI) bf16 (layer input), dY:
+// (n, H, W, COP) bf16 (gradient of the layer's pre-activation, channels >=
+// CO zero).  Block: `rows` image rows of image blockIdx.y; per row the three
+// input rows and the gradient row are staged in LDS, and each wave
+// accumulates its share of the 9 (COP/16) (CI/16) 16x16 tiles over 16-pixel
+// chunks (each weight entry has one owning lane).
 template <int CI, int CO>
 __global__ void __launch_bounds__(256) k_conv_wgrad(const u16* __restrict__ x, const u16* __restrict__ dy,
-                                                    float* __restrict__ dw, float* __restrict__ db, int H, int W,
-                                                    int rows) {
+                                                    float* __restrict__ part, int H, int W, int rows) {
   constexpr int COP = pad16(CO);
   constexpr int NMT = COP / 16, NNT = CI / 16, NT = 9 * NMT * NNT, TPW = (NT + 3) / 4;
   extern __shared__ __align__(16) u16 lds[];
   u16* xs = lds;                       // [3][W][CI]
   u16* ds = lds + 3 * W * CI;          // [W][COP]
-  __shared__ float bsum[COP];
+  __shared__ float bpart[256];
   const int tid = threadIdx.x;
-  if (tid < COP) bsum[tid] = 0.f;
   const int lane = tid & 63, w = tid >> 6;
   const int j = lane & 15, g = lane >> 4;
   const int n = blockIdx.y;
@@ -313,6 +314,7 @@ __global__ void __launch_bounds__(256) k_conv_wgrad(const u16* __restrict__ x, c
 #pragma unroll
   for (int t = 0; t < TPW; ++t) acc[t] = f4v{0.f, 0.f, 0.f, 0.f};
   float bacc = 0.f;   // channel tid % COP (256 is a multiple of COP)
+  part += (long)(blockIdx.y * gridDim.x + blockIdx.x) * (CO * 9 * CI + CO);
   for (int y = y0; y < y1; ++y) {
     __syncthreads();
     for (int e = tid * 8; e < 3 * W * CI; e += 256 * 8) {
@@ -360,12 +362,29 @@ __global__ void __launch_bounds__(256) k_conv_wgrad(const u16* __restrict__ x, c
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int co = mt * 16 + 4 * g + r;
-      if (co < CO) atomicAdd(&dw[(co * 9 + tap) * CI + nt * 16 + j], acc[tt][r]);
+      if (co < CO) part[(co * 9 + tap) * CI + nt * 16 + j] = acc[tt][r];
     }
   }
-  atomicAdd(&bsum[tid % COP], bacc);
+  // bias: thread tid holds channel tid % COP; summed per channel in thread order
+  bpart[tid] = bacc;
   __syncthreads();
-  if (tid < CO) atomicAdd(&db[tid], bsum[tid]);
+  if (tid < CO) {
+    float sb = 0.f;
+    for (int k = tid; k < 256; k += COP) sb += bpart[k];
+    part[CO * 9 * CI + tid] = sb;
+  }
+}
+
+// grad[i] += sum over the nblk workgroup slices of part[b * psz + i], in
+// slice order; i < CO 9 CI are weights (dw), the rest biases (db).
+__global__ void k_wgrad_reduce(const float* __restrict__ part, int nblk, int psz, int nw, float* __restrict__ dw,
+                               float* __restrict__ db) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= psz) return;
+  float sacc = 0.f;
+  for (int b = 0; b < nblk; ++b) sacc += part[(long)b * psz + i];
+  if (i < nw) dw[i] += sacc;
+  else db[i - nw] += sacc;
 }
 
 // Mean-squared error of pred against target over n_pix * CO values; writes
@@ -451,6 +470,8 @@ struct qgn_cnn {
   int n = 0, H = 0, W = 0;
   double* sq = nullptr;
   double* h_sq = nullptr;
+  float* part = nullptr;       // weight-gradient workgroup slices
+  long part_cap = 0;           // floats part holds
   hipStream_t stream = nullptr;
 };
 
@@ -522,14 +543,26 @@ int conv(const u16* in, const u16* w, const float* b, const u16* mask, void* out
   return 0;
 }
 
+constexpr int kWgradRows = 16;   // image rows per weight-gradient workgroup
+
+// workgroup slices of the weight-gradient scratch for a batch of n images of
+// H rows, and floats per slice (the largest layer: 32 x 9 x 32 + 32)
+long wgrad_slices(int n, int H) { return (long)n * ((H + kWgradRows - 1) / kWgradRows); }
+constexpr long kWgradSlice = 32L * 9 * 32 + 32;
+
 template <int CI, int CO>
-int wgrad(const u16* x, const u16* dy, float* dw, float* db, int n, int H, int W, hipStream_t s) {
+int wgrad(const u16* x, const u16* dy, float* dw, float* db, int n, int H, int W, float* part, hipStream_t s) {
   constexpr int COP = pad16(CO);
+  constexpr int PSZ = CO * 9 * CI + CO;
+  static_assert(PSZ <= kWgradSlice, "slice");
   const size_t lds = (size_t)(3 * W * CI + W * COP) * sizeof(u16);
   auto kern = k_conv_wgrad<CI, CO>;
   if (allow_lds((const void*)kern, lds)) return cfail(QGN_EINVAL, "wgrad: grid row too wide for LDS");
-  const int rows = 16;
-  hipLaunchKernelGGL(kern, dim3((H + rows - 1) / rows, n), dim3(256), lds, s, x, dy, dw, db, H, W, rows);
+  const dim3 grid((H + kWgradRows - 1) / kWgradRows, n);
+  hipLaunchKernelGGL(kern, grid, dim3(256), lds, s, x, dy, part, H, W, kWgradRows);
+  CCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_wgrad_reduce, dim3((PSZ + 255) / 256), dim3(256), 0, s, part, (int)(grid.x * grid.y), PSZ,
+                     CO * 9 * CI, dw, db);
   CCHK(hipGetLastError());
   return 0;
 }
@@ -592,7 +625,8 @@ const char* qgn_cnn_last_error(void) { return g_cnn_err.c_str(); }
 int qgn_cnn_destroy(qgn_cnn* h) {
   if (!h) return 0;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
-  void* ptrs[] = {h->w, h->g, h->m, h->v, h->dpred, h->sq, h->hid[0], h->hid[1], h->hid[2], h->dbuf[0], h->dbuf[1]};
+  void* ptrs[] = {h->w,       h->g,       h->m,       h->v,       h->dpred,  h->sq,
+                  h->hid[0],  h->hid[1],  h->hid[2],  h->dbuf[0], h->dbuf[1], h->part};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (int l = 0; l < kLayers; ++l) {
@@ -694,6 +728,16 @@ static int backward_impl(qgn_cnn* h, const float* target, double* loss) {
   const int n = h->n, H = h->H, W = h->W;
   const long npix = (long)n * H * W;
   const Layer& L4 = h->L[kLayers - 1];
+  const long need = wgrad_slices(n, H) * kWgradSlice;
+  if (need > h->part_cap) {
+    CCHK(hipStreamSynchronize(s));
+    if (h->part) (void)hipFree(h->part);
+    h->part = nullptr;
+    h->part_cap = 0;
+    CCHK(hipMalloc((void**)&h->part, (size_t)need * sizeof(float)));
+    h->part_cap = need;
+  }
+  float* part = h->part;
   CCHK(hipMemsetAsync(h->g, 0, (size_t)h->np * sizeof(float), s));
   CCHK(hipMemsetAsync(h->sq, 0, sizeof(double), s));
   const float scale = (float)(2.0 / ((double)npix * L4.co));
@@ -702,8 +746,8 @@ static int backward_impl(qgn_cnn* h, const float* target, double* loss) {
   CCHK(hipGetLastError());
   // layer 4 (no activation): dY = dpred
   float* g = h->g;
-  if (L4.co == 8) CRET((wgrad<32, 8>(h->act[3], h->dpred, g + L4.woff, g + L4.boff, n, H, W, s)));
-  else CRET((wgrad<32, 16>(h->act[3], h->dpred, g + L4.woff, g + L4.boff, n, H, W, s)));
+  if (L4.co == 8) CRET((wgrad<32, 8>(h->act[3], h->dpred, g + L4.woff, g + L4.boff, n, H, W, part, s)));
+  else CRET((wgrad<32, 16>(h->act[3], h->dpred, g + L4.woff, g + L4.boff, n, H, W, part, s)));
   const u16* dy = h->dpred;
   int nb = 0;
   for (int l = kLayers - 1; l >= 1; --l) {
@@ -716,8 +760,8 @@ static int backward_impl(qgn_cnn* h, const float* target, double* loss) {
       CRET((conv<32, 32, 2>(dy, h->wb[l], nullptr, h->act[l], dx, n, H, W, s)));
     }
     const Layer& Lp = h->L[l - 1];
-    if (Lp.ci == 16) CRET((wgrad<16, 32>(h->act[l - 1], dx, g + Lp.woff, g + Lp.boff, n, H, W, s)));
-    else CRET((wgrad<32, 32>(h->act[l - 1], dx, g + Lp.woff, g + Lp.boff, n, H, W, s)));
+    if (Lp.ci == 16) CRET((wgrad<16, 32>(h->act[l - 1], dx, g + Lp.woff, g + Lp.boff, n, H, W, part, s)));
+    else CRET((wgrad<32, 32>(h->act[l - 1], dx, g + Lp.woff, g + Lp.boff, n, H, W, part, s)));
     dy = dx;
   }
   CCHK(hipMemcpyAsync(h->h_sq, h->sq, sizeof(double), hipMemcpyDeviceToHost, s));

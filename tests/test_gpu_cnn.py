@@ -163,6 +163,32 @@ def test_closure_fp64_row_limit():
     net.close()
 
 
+def test_weight_gradients_are_reproducible():
+    """Each workgroup of the weight-gradient kernel writes its own partial
+    sums and a second kernel adds them in a fixed order, so the same
+    backward pass twice gives bit-identical gradients (and a training run
+    is reproducible)."""
+    import torch
+    from oracle import cnn_oracle as co
+    from qgn_fortran_amd.cnn import CNN, to_bf16_bits
+    rng = np.random.default_rng(3)
+    for cin, cout, n, H, W in ((16, 8, 2, 32, 48), (32, 16, 3, 40, 32)):
+        x = rng.standard_normal((n, H, W, cin)).astype(np.float32)
+        t = rng.standard_normal((n, H, W, cout)).astype(np.float32)
+        xd = torch.from_numpy(to_bf16_bits(x)).cuda()
+        td = torch.from_numpy(t).cuda()
+        y = torch.empty((n, H, W, cout), device="cuda", dtype=torch.float32)
+        net = CNN(cin, cout, co.init_params(cin, cout, seed=2, last_scale=1.0))
+        grads = []
+        for _ in range(3):
+            net.forward(xd, y)
+            net.backward(td)
+            grads.append(net.grads())
+        assert np.any(grads[0] != 0)
+        assert all(np.array_equal(grads[0], gr) for gr in grads[1:])
+        net.close()
+
+
 def test_inputs_written_by_torch_just_before_the_call():
     """The network's kernels run on the library's own stream: inputs that a
     torch kernel is still writing when the call is made (an index_select of
