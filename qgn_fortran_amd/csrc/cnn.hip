@@ -375,16 +375,25 @@ __global__ void __launch_bounds__(256) k_conv_wgrad(const u16* __restrict__ x, c
   }
 }
 
-// grad[i] += sum over the nblk workgroup slices of part[b * psz + i], in
-// slice order; i < CO 9 CI are weights (dw), the rest biases (db).
-__global__ void k_wgrad_reduce(const float* __restrict__ part, int nblk, int psz, int nw, float* __restrict__ dw,
-                               float* __restrict__ db) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= psz) return;
+// grad[i] += sum over the nblk workgroup slices of part[b * psz + i]; i <
+// CO 9 CI are weights (dw), the rest biases (db).  A workgroup of 256 takes
+// 64 entries: wave q sums slices q, q + 4, q + 8, ... in order, then the four
+// wave sums are added in wave order (a fixed order throughout).
+__global__ void __launch_bounds__(256) k_wgrad_reduce(const float* __restrict__ part, int nblk, int psz, int nw,
+                                                      float* __restrict__ dw, float* __restrict__ db) {
+  __shared__ float red[4][64];
+  const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int i = blockIdx.x * 64 + lane;
   float sacc = 0.f;
-  for (int b = 0; b < nblk; ++b) sacc += part[(long)b * psz + i];
-  if (i < nw) dw[i] += sacc;
-  else db[i - nw] += sacc;
+  if (i < psz)
+    for (int b = q; b < nblk; b += 4) sacc += part[(long)b * psz + i];
+  red[q][lane] = sacc;
+  __syncthreads();
+  if (q == 0 && i < psz) {
+    const float t = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
+    if (i < nw) dw[i] += t;
+    else db[i - nw] += t;
+  }
 }
 
 // Mean-squared error of pred against target over n_pix * CO values; writes
@@ -561,7 +570,7 @@ int wgrad(const u16* x, const u16* dy, float* dw, float* db, int n, int H, int W
   const dim3 grid((H + kWgradRows - 1) / kWgradRows, n);
   hipLaunchKernelGGL(kern, grid, dim3(256), lds, s, x, dy, part, H, W, kWgradRows);
   CCHK(hipGetLastError());
-  hipLaunchKernelGGL(k_wgrad_reduce, dim3((PSZ + 255) / 256), dim3(256), 0, s, part, (int)(grid.x * grid.y), PSZ,
+  hipLaunchKernelGGL(k_wgrad_reduce, dim3((PSZ + 63) / 64), dim3(256), 0, s, part, (int)(grid.x * grid.y), PSZ,
                      CO * 9 * CI, dw, db);
   CCHK(hipGetLastError());
   return 0;
