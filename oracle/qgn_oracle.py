@@ -196,7 +196,10 @@ class Model:
         p = self.p
         Hm = self.H[:, None] * self.Modes                  # H(kk)*Modes(kk,k)
         q_mode = np.einsum("ak,ayx->kyx", Hm, q_hat) / p.Htot
-        psi_mode = q_mode / (-self.k2[None, :, :] + self.EVals[:, None, None])
+        # the (0,0) mode divides by the barotropic eigenvalue, a round-off
+        # number that may be 0; that mode is zeroed just below (:328)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            psi_mode = q_mode / (-self.k2[None, :, :] + self.EVals[:, None, None])
         psi_mode[:, 0, 0] = 0.0
         return np.einsum("ka,ayx->kyx", self.Modes, psi_mode)
 
@@ -259,7 +262,10 @@ class Model:
         Hm = self.H[:, None] * self.Modes
         q_mode = np.einsum("ak,ayx->kyx", Hm, q_hat) / p.Htot
         j_mode = np.einsum("ak,ayx->kyx", Hm, jaco_hat) / p.Htot
-        psi_mode = q_mode / (-self.k2[None, :, :] + self.EVals[:, None, None])
+        # the (0,0) mode divides by the barotropic eigenvalue, a round-off
+        # number that may be 0; that mode is zeroed just below (:328)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            psi_mode = q_mode / (-self.k2[None, :, :] + self.EVals[:, None, None])
         psi_mode[:, 0, 0] = 0.0
         return self.c2r(q_mode) / nn, self.c2r(psi_mode) / nn, self.c2r(j_mode) / nn
 
