@@ -95,18 +95,20 @@ def test_training_reduces_loss_on_a_teacher():
     net.close()
 
 
-@pytest.mark.parametrize("precision,nz", [("fp64", 8), ("mixed", 8), ("fp64", 16), ("mixed", 16)])
-def test_closure_forcing_in_rhs(precision, nz):
+@pytest.mark.parametrize("precision,nz,n", [("fp64", 8, 64), ("mixed", 8, 64), ("fp64", 16, 64), ("mixed", 16, 64),
+                                            ("fp64", 8, 256), ("mixed", 8, 256)])
+def test_closure_forcing_in_rhs(precision, nz, n):
     """qgn_set_closure: GetRHS with the network attached minus GetRHS without
     it is r2c(out_scale * net(in_scale * (q, psi))) on the kept modes, with q
     and psi the physical fields of the state (the oracle's WriteQP fields,
-    QGN_Module.f90:353-361) and the network the bf16-emulating oracle."""
+    QGN_Module.f90:353-361) and the network the bf16-emulating oracle; at
+    64^2 and at the learned-closure run's 256^2 x 8."""
     from conftest import make_case
     from oracle import cnn_oracle as co
     from oracle import qgn_oracle as orc
     from qgn_fortran_amd import QGN
     from qgn_fortran_amd.cnn import CNN
-    case = make_case(64, 64, nz, k_peak=6, precision=precision)
+    case = make_case(n, n, nz, k_peak=6, precision=precision)
     om = orc.Model(case["po"], case["H"], case["Modes"], case["EVals"], case["uBar"], case["qyBar"])
     qh = om.initialize(case["q0"])
     qp, pp = om.get_psi_phys(qh)
@@ -114,18 +116,18 @@ def test_closure_forcing_in_rhs(precision, nz):
     # place would show
     s_in = np.concatenate([np.linspace(0.8, 1.2, nz) / np.std(qp), np.linspace(1.1, 0.9, nz) / np.std(pp)])
     base = om.get_rhs(qh)
-    s_out = np.linspace(0.2, 0.4, nz) * np.sqrt(np.mean(om.c2r(base) ** 2)) / (64 * 64)
+    s_out = np.linspace(0.2, 0.4, nz) * np.sqrt(np.mean(om.c2r(base) ** 2)) / (n * n)
     params = co.init_params(2 * nz, nz, seed=3, last_scale=1.0)
     x = np.concatenate([qp, pp], axis=0).transpose(1, 2, 0)[None] * s_in      # (1, H, W, 2 nz)
     y, _ = co.forward(params, x.astype(np.float32), 2 * nz, nz, bf16_points=True)
-    F = orc.dealias(orc.rfft2(y[0].transpose(2, 0, 1) * s_out[:, None, None]), 64, 64)
+    F = orc.dealias(orc.rfft2(y[0].transpose(2, 0, 1) * s_out[:, None, None]), n, n)
     gm = QGN(case["pp"], case["H"], case["Modes"], case["EVals"], case["uBar"], case["qyBar"])
     r0 = gm.get_rhs(qh)
     net = CNN(2 * nz, nz, params)
     gm.set_closure(net, s_in, s_out)
     r1 = gm.get_rhs(qh)
     err = rel_l2(r1 - r0, F)
-    print(precision, "closure forcing rel L2", err, "forcing/RHS", np.linalg.norm(F) / np.linalg.norm(base), flush=True)
+    print(precision, nz, n, "closure forcing rel L2", err, "forcing/RHS", np.linalg.norm(F) / np.linalg.norm(base), flush=True)
     assert err < (2e-3 if precision == "fp64" else 5e-3)
     # the closure inside the steppers: a step with it differs from a step without
     gm.set_state(qh)
