@@ -62,6 +62,8 @@ typedef struct qgn_ctx qgn_ctx;
  * qyBar (nz) are the module variables of QGN_Module.f90:22-26. */
 int qgn_create(qgn_ctx** ctx, const qgn_params* p, const double* H, const double* Modes,
                const double* EVals, const double* uBar, const double* qyBar);
+/* Cleanup (QGN_Module.f90:448-458).  Returns QGN_EINVAL and destroys
+ * nothing while the context belongs to an ensemble. */
 int qgn_destroy(qgn_ctx* ctx);
 
 /* ---- Batches of members (ensembles) ------------------------------------
@@ -100,6 +102,10 @@ int qgn_batch_get_time(qgn_ctx* ctx, double* t, double* dt, long long* steps, lo
  * after the work already queued on every context and before what is
  * queued on them afterwards, and returns without waiting. */
 typedef struct qgn_ens qgn_ens;
+/* The ensemble's graph holds its contexts (qgn_destroy refuses them until
+ * qgn_ensemble_destroy) and the closure networks they had attached when it
+ * was created (which keep their buffers until qgn_ensemble_destroy, even
+ * after qgn_set_closure(ctx, NULL, ...)). */
 int qgn_ensemble_create(qgn_ens** ens, qgn_ctx* const* ctxs, int n);
 int qgn_ensemble_step(qgn_ens* ens, int attempts);
 int qgn_ensemble_destroy(qgn_ens* ens);
@@ -205,6 +211,7 @@ int qgn_dft_forward_device(qgn_dft* h, const void* grid, void* spec, void* strea
 int qgn_dft_backward_device(qgn_dft* h, const void* spec, void* grid, void* stream);
 int qgn_dft_free(qgn_dft* h);
 
+
 /* ---- LAPACK DGEEV replacement for the vertical structure ---------------
  * QGN_Module.f90:84-148: H <- Htot*H/sum(H); tridiagonal stretching matrix
  * L from S(0..nz) (only S(1..nz-1) used); eigenpairs of L (real, since L is
@@ -226,6 +233,8 @@ int qgn_vertical_modes(int nz, double Htot, const double* H_in, const double* S,
  * (dx+1) for the input offset (dy, dx); qgn_cnn_nparams gives the length. */
 typedef struct qgn_cnn qgn_cnn;
 int qgn_cnn_create(qgn_cnn** h, int cin, int cout, const float* params);
+/* Returns QGN_EINVAL and destroys nothing while a model has the network
+ * attached (qgn_set_closure) or an ensemble holds it. */
 int qgn_cnn_destroy(qgn_cnn* h);
 int qgn_cnn_nparams(qgn_cnn* h, int* n);
 /* host copies of the parameters and of the last gradients (either may be NULL) */
@@ -236,9 +245,16 @@ int qgn_cnn_set_params(qgn_cnn* h, const float* params);
  * stream: device inputs (x, target) must be complete when a call is made
  * (synchronise the stream that wrote them), and each call has finished
  * its device work when it returns. */
+/* While the network is attached to a model or held by an ensemble, its
+ * activation buffers are fixed at the attached model's pixel count: a pass
+ * on more pixels returns QGN_EINVAL instead of reallocating them. */
 int qgn_cnn_forward(qgn_cnn* h, const void* x, int n, int H, int W, float* y);
 /* gradients of the mean-squared error of the last forward's y against the
- * device array target (n, H, W, cout); *loss receives the error */
+ * device array target (n, H, W, cout); *loss receives the error.  The
+ * backward pass reads the x and y of that forward: they must stay allocated
+ * and unchanged until it returns.  After a closure evaluation (an RHS of a
+ * model that has the network attached) the last forward is the closure's,
+ * and backward returns QGN_EINVAL until qgn_cnn_forward runs again. */
 int qgn_cnn_backward(qgn_cnn* h, const float* target, double* loss);
 /* one Adam step (bias-corrected) with the last gradients */
 int qgn_cnn_adam(qgn_cnn* h, float lr, float beta1, float beta2, float eps);
@@ -253,9 +269,10 @@ const char* qgn_cnn_last_error(void);
  * psi)) is evaluated on the physical fields of each member (input channels
  * q_1..q_nz, psi_1..psi_nz; output one forcing per layer).  The network
  * needs cin = 2 nz and cout = nz; nx % 16 == 0 (and nx <= 1024 on the
- * FP64 path).  The context keeps the
- * pointer: destroy the network only after detaching it.  An ensemble built
- * before this call keeps the attempt it captured. */
+ * FP64 path).  The context keeps the pointer and holds the network until it
+ * is detached or the context destroyed (qgn_cnn_destroy refuses a held
+ * network).  An ensemble built before this call keeps the attempt it
+ * captured, and with it its hold on the network it had. */
 int qgn_set_closure(qgn_ctx* ctx, qgn_cnn* net, const double* in_scale, const double* out_scale);
 
 const char* qgn_last_error(void);

@@ -52,8 +52,27 @@ class CNN:
         check_cnn(self.lib.qgn_cnn_set_params(self._h, fptr(np.ascontiguousarray(p, dtype=np.float32))))
 
     @staticmethod
-    def _shape(x):
+    def _check(t, name, dtype, shape):
+        """A contiguous CUDA tensor of the given dtype and shape: the library
+        reads and writes through its raw pointer."""
+        import torch
+        if not isinstance(t, torch.Tensor) or not t.is_cuda:
+            raise ValueError(f"{name} must be a CUDA tensor")
+        if t.dtype != dtype:
+            raise ValueError(f"{name} must be {dtype}, got {t.dtype}")
+        if tuple(t.shape) != tuple(shape):
+            raise ValueError(f"{name} must have shape {tuple(shape)}, got {tuple(t.shape)}")
+        if not t.is_contiguous():
+            raise ValueError(f"{name} must be contiguous")
+
+    def _shape(self, x, y=None):
+        import torch
+        if x.dim() != 4:
+            raise ValueError("x must be (n, H, W, cin)")
         n, H, W, _ = x.shape
+        self._check(x, "x", torch.int16, (n, H, W, self.cin))
+        if y is not None:
+            self._check(y, "y", torch.float32, (n, H, W, self.cout))
         return n, H, W
 
     @staticmethod
@@ -66,11 +85,19 @@ class CNN:
 
     def forward(self, x, y):
         """x: int16 CUDA tensor (n, H, W, cin); y: float32 CUDA tensor (n, H, W, cout)."""
-        n, H, W = self._shape(x)
+        n, H, W = self._shape(x, y)
+        self._last = (n, H, W)
         self._ready(x)
         check_cnn(self.lib.qgn_cnn_forward(self._h, C.c_void_p(x.data_ptr()), n, H, W, C.c_void_p(y.data_ptr())))
 
     def backward(self, target):
+        """Gradients of the mean-squared error of the last forward's output
+        against target; that forward's x and y must still be alive and
+        unchanged."""
+        import torch
+        if getattr(self, "_last", None) is None:
+            raise ValueError("backward needs a forward first")
+        self._check(target, "target", torch.float32, self._last + (self.cout,))
         loss = C.c_double()
         self._ready(target)
         check_cnn(self.lib.qgn_cnn_backward(self._h, C.c_void_p(target.data_ptr()), C.byref(loss)))
@@ -80,7 +107,10 @@ class CNN:
         check_cnn(self.lib.qgn_cnn_adam(self._h, lr, b1, b2, eps))
 
     def train_step(self, x, target, y, lr):
-        n, H, W = self._shape(x)
+        import torch
+        n, H, W = self._shape(x, y)
+        self._check(target, "target", torch.float32, (n, H, W, self.cout))
+        self._last = (n, H, W)
         loss = C.c_double()
         self._ready(x)
         check_cnn(self.lib.qgn_cnn_train_step(self._h, C.c_void_p(x.data_ptr()), C.c_void_p(target.data_ptr()),
@@ -88,8 +118,10 @@ class CNN:
         return loss.value
 
     def close(self):
+        """Destroy the network; raises (and keeps it) while a model has it
+        attached or an ensemble holds it."""
         if self._h:
-            self.lib.qgn_cnn_destroy(self._h)
+            check_cnn(self.lib.qgn_cnn_destroy(self._h))
             self._h = C.c_void_p()
 
     def __del__(self):

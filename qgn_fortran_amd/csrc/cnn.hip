@@ -482,6 +482,11 @@ struct qgn_cnn {
   float* part = nullptr;       // weight-gradient workgroup slices
   long part_cap = 0;           // floats part holds
   hipStream_t stream = nullptr;
+  // Models (as their closure) and ensembles (in their captured graphs) that
+  // use the activation buffers and packed weights: while any holds the
+  // network, those buffers may neither move nor be freed.
+  int holds = 0;
+  bool closure_fwd = false;    // the last forward ran inside a model's RHS
 };
 
 namespace {
@@ -515,6 +520,10 @@ int repack(qgn_cnn* h, hipStream_t s) {
 
 int ensure(qgn_cnn* h, long npix) {
   if (npix <= h->cap) return 0;
+  if (h->holds > 0)
+    return cfail(QGN_EINVAL, "cnn: the network is attached to a model or captured by an ensemble, which use its "
+                             "activation buffers for " + std::to_string(h->cap) + " pixels; a pass on " +
+                             std::to_string(npix) + " pixels would reallocate them (detach it first)");
   for (u16* p : h->hid)
     if (p) (void)hipFree(p);
   for (u16* p : h->dbuf)
@@ -599,9 +608,10 @@ int layer_fwd(qgn_cnn* h, int l, const u16* in, void* out, int n, int H, int W, 
 
 namespace qcnn {
 
-int forward(qgn_cnn* h, const u16* x, int n, int H, int W, float* y, hipStream_t s) {
+int forward(qgn_cnn* h, const u16* x, int n, int H, int W, float* y, hipStream_t s, bool closure) {
   CRET(check_shape(h, n, H, W));
   CRET(ensure(h, (long)n * H * W));
+  h->closure_fwd = closure;
   h->act[0] = x;
   for (int l = 0; l < kLayers; ++l) {
     void* out = l < kLayers - 1 ? (void*)h->hid[l] : (void*)y;
@@ -616,6 +626,8 @@ int forward(qgn_cnn* h, const u16* x, int n, int H, int W, float* y, hipStream_t
 }
 
 int reserve(qgn_cnn* h, long npix) { return ensure(h, npix); }
+
+void hold(qgn_cnn* h, int delta) { h->holds += delta; }
 
 const char* last_error() { return g_cnn_err.c_str(); }
 
@@ -633,6 +645,9 @@ const char* qgn_cnn_last_error(void) { return g_cnn_err.c_str(); }
 
 int qgn_cnn_destroy(qgn_cnn* h) {
   if (!h) return 0;
+  if (h->holds > 0)
+    return cfail(QGN_EINVAL, "cnn: the network is attached to a model or captured by an ensemble; detach it "
+                             "(qgn_set_closure(ctx, NULL, ...)) and destroy the ensemble first");
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   void* ptrs[] = {h->w,       h->g,       h->m,       h->v,       h->dpred,  h->sq,
                   h->hid[0],  h->hid[1],  h->hid[2],  h->dbuf[0], h->dbuf[1], h->part};
@@ -724,7 +739,7 @@ int qgn_cnn_set_params(qgn_cnn* h, const float* params) {
 
 int qgn_cnn_forward(qgn_cnn* h, const void* x, int n, int H, int W, float* y) {
   if (!h || !x || !y) return cfail(QGN_EINVAL, "null argument");
-  CRET(qcnn::forward(h, reinterpret_cast<const u16*>(x), n, H, W, y, h->stream));
+  CRET(qcnn::forward(h, reinterpret_cast<const u16*>(x), n, H, W, y, h->stream, false));
   CCHK(hipStreamSynchronize(h->stream));
   return 0;
 }
@@ -733,6 +748,9 @@ int qgn_cnn_forward(qgn_cnn* h, const void* x, int n, int H, int W, float* y) {
 // and the mean-squared error.
 static int backward_impl(qgn_cnn* h, const float* target, double* loss) {
   if (!h->pred) return cfail(QGN_EINVAL, "cnn: backward needs a forward first");
+  if (h->closure_fwd)
+    return cfail(QGN_EINVAL, "cnn: the last forward pass was the closure's, inside a model's RHS; call "
+                             "qgn_cnn_forward before qgn_cnn_backward");
   hipStream_t s = h->stream;
   const int n = h->n, H = h->H, W = h->W;
   const long npix = (long)n * H * W;
@@ -800,7 +818,7 @@ int qgn_cnn_adam(qgn_cnn* h, float lr, float b1, float b2, float eps) {
 int qgn_cnn_train_step(qgn_cnn* h, const void* x, const float* target, int n, int H, int W, float* y, float lr,
                        double* loss) {
   if (!h || !x || !target || !y) return cfail(QGN_EINVAL, "null argument");
-  CRET(qcnn::forward(h, reinterpret_cast<const u16*>(x), n, H, W, y, h->stream));
+  CRET(qcnn::forward(h, reinterpret_cast<const u16*>(x), n, H, W, y, h->stream, false));
   CRET(backward_impl(h, target, loss));
   return qgn_cnn_adam(h, lr, 0.9f, 0.999f, 1e-8f);
 }

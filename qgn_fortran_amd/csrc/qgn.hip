@@ -279,7 +279,7 @@ struct qgn_ctx {
   double* mp[5] = {};
   double* spec = nullptr;  // spectra accumulator, 2 nz nbins
   long spec_n = 0;
-  // learned closure: network, scales and work arrays
+  // learned closure: network (held while attached), scales and work arrays
   qgn_cnn* net = nullptr;
   double *cl_in = nullptr, *cl_out = nullptr;   // 2 nzm and nzm scales
   void *cl_mq = nullptr, *cl_mp = nullptr;      // mixed q and psi (cplx<R>)
@@ -299,6 +299,7 @@ struct qgn_ctx {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   double2* graph_q = nullptr;   // the buffer the graph takes as the state
+  int ens_refs = 0;             // ensembles whose graphs hold this context's attempt
   // stepper state (the reference's `save` variables)
   bool first = true, second = false;
   double err0 = 0.0, last_err = 0.0;
@@ -315,6 +316,7 @@ struct qgn_ctx {
 // with each batch's attempt graph as an independent child.
 struct qgn_ens {
   std::vector<qgn_ctx*> ctx;
+  std::vector<qgn_cnn*> nets;   // closure networks its graph captured (held until destroy)
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
   std::vector<hipEvent_t> ready;
@@ -740,7 +742,8 @@ int closure_forcing(qgn_ctx* c) {
   const Geom& g = c->g;
   if (g.nzm != 8 && g.nzm != 16) return fail(QGN_EINVAL, "closure network: nz must be 8 or 16");
   CHK((launch_closure_rows<R, true>(c, nullptr)));
-  if (qcnn::forward(c->net, reinterpret_cast<const unsigned short*>(c->cl_x), c->nm, g.ny, g.nx, c->cl_y, c->stream))
+  if (qcnn::forward(c->net, reinterpret_cast<const unsigned short*>(c->cl_x), c->nm, g.ny, g.nx, c->cl_y, c->stream,
+                    true))
     return fail(QGN_EINVAL, std::string("closure network: ") + qcnn::last_error());
   return launch_closure_rows<R, false>(c, c->cl_mq);   // its input rows are consumed
 }
@@ -1231,6 +1234,7 @@ int alloc(void** p, size_t bytes) {
 void destroy_ctx(qgn_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->net) qcnn::hold(c->net, -1);
   void* ptrs[] = {c->HM, c->MD, c->evals, c->ubar, c->qybar, c->a8l, c->partial, c->q, c->qn, c->E, c->psi64,
                   c->psi, c->q32, c->Um, c->Vm, c->QXm, c->QYm, c->Jm, c->DUm, c->DVm, c->Em, c->errslot, c->errpart, c->fa, c->fb,
                   c->phys};
@@ -1590,6 +1594,8 @@ int qgn_create_batch(qgn_ctx** ctx, const qgn_params* p, int members, const doub
 }
 
 int qgn_destroy(qgn_ctx* ctx) {
+  if (ctx && ctx->ens_refs > 0)
+    return fail(QGN_EINVAL, "the context belongs to an ensemble: destroy the ensemble first");
   destroy_ctx(ctx);
   return 0;
 }
@@ -1671,6 +1677,7 @@ int qgn_set_closure(qgn_ctx* c, qgn_cnn* net, const double* in_scale, const doub
     c->graph = nullptr;
   }
   if (!net) {
+    if (c->net) qcnn::hold(c->net, -1);
     c->net = nullptr;
     return 0;
   }
@@ -1695,7 +1702,11 @@ int qgn_set_closure(qgn_ctx* c, qgn_cnn* net, const double* in_scale, const doub
   HIPCHK(hipStreamSynchronize(c->stream));
   HIPCHK(hipMemcpy(c->cl_in, in_scale, 2 * (size_t)g.nzm * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->cl_out, out_scale, (size_t)g.nzm * sizeof(double), hipMemcpyHostToDevice));
-  if (qcnn::reserve(net, pix)) return fail(QGN_ENOMEM, std::string("closure network: ") + qcnn::last_error());
+  if (qcnn::reserve(net, pix)) return fail(QGN_EINVAL, std::string("closure network: ") + qcnn::last_error());
+  if (c->net != net) {
+    qcnn::hold(net, +1);
+    if (c->net) qcnn::hold(c->net, -1);
+  }
   c->net = net;
   return 0;
 }
@@ -1989,6 +2000,8 @@ extern "C" {
 int qgn_ensemble_destroy(qgn_ens* e) {
   if (!e) return 0;
   if (e->stream) (void)hipStreamSynchronize(e->stream);
+  for (qgn_ctx* c : e->ctx) c->ens_refs--;
+  for (qgn_cnn* n : e->nets) qcnn::hold(n, -1);
   if (e->exec) (void)hipGraphExecDestroy(e->exec);
   if (e->graph) (void)hipGraphDestroy(e->graph);
   for (hipEvent_t x : e->ready) (void)hipEventDestroy(x);
@@ -2010,6 +2023,11 @@ static int ensemble_create_impl(qgn_ens* e, qgn_ctx* const* ctxs, int n) {
     }
     CHK(batch_check(ctxs[i]));
     e->ctx.push_back(ctxs[i]);
+    ctxs[i]->ens_refs++;
+    if (ctxs[i]->net) {   // the child graph keeps the network's buffers after a detach
+      qcnn::hold(ctxs[i]->net, +1);
+      e->nets.push_back(ctxs[i]->net);
+    }
     hipGraphNode_t node;
     HIPCHK(hipGraphAddChildGraphNode(&node, e->graph, nullptr, 0, ctxs[i]->graph));
     hipEvent_t ev;

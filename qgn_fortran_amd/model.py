@@ -232,10 +232,12 @@ class QGN:
         return n.value, ms.value
 
     def close(self):
-        """Cleanup (QGN_Module.f90:448-458)."""
+        """Cleanup (QGN_Module.f90:448-458); raises (and keeps the context)
+        while an ensemble holds it."""
         if self._ctx:
-            self.lib.qgn_destroy(self._ctx)
+            check(self.lib.qgn_destroy(self._ctx))
             self._ctx = C.c_void_p()
+            self._net = None
 
     def __del__(self):
         try:

@@ -214,3 +214,60 @@ def test_inputs_written_by_torch_just_before_the_call():
         net.forward(xs, y2)
         assert torch.equal(y1, y2), trial
     net.close()
+
+
+def test_closure_network_lifetime_is_enforced():
+    """A model with the network attached, and an ensemble whose captured
+    graph ran it, use the network's activation buffers: while either holds
+    the network, a pass on more pixels and qgn_cnn_destroy return an error
+    instead of freeing them; qgn_destroy refuses a context an ensemble
+    holds; backward after the closure's own forward (inside an RHS) is
+    refused.  Only the error paths run: nothing touches freed memory."""
+    import torch
+    from oracle import cnn_oracle as co
+    from qgn_fortran_amd import Batch, Ensemble, configs
+    from qgn_fortran_amd._lib import QGN_EINVAL, QgnError
+    from qgn_fortran_amd.cnn import CNN, to_bf16_bits
+    group = configs.ensemble96(sizes=(64,), per_size=1, nz=8, seed=3, scheme="ARK43")[0]
+    net = CNN(16, 8, co.init_params(16, 8, seed=4, last_scale=0.05))
+    s_in, s_out = np.full(16, 1e5), np.full(8, 1e-12)
+    b = Batch(group.params, group.members)
+    qh = b.initialize(group.q0)
+    b.set_closure(net, s_in, s_out)                  # holds 64 x 64 pixels
+    rng = np.random.default_rng(0)
+    big = _dev(to_bf16_bits(rng.standard_normal((1, 128, 128, 16))), torch.int16)
+    ybig = torch.empty((1, 128, 128, 8), device="cuda", dtype=torch.float32)
+    with pytest.raises(QgnError, match="reallocate") as e:
+        net.forward(big, ybig)
+    assert e.value.code == QGN_EINVAL
+    with pytest.raises(QgnError, match="attached"):
+        net.close()
+    # a user forward within the held size, then an RHS (the closure's forward): backward refuses
+    x = _dev(to_bf16_bits(rng.standard_normal((1, 64, 64, 16))), torch.int16)
+    y = torch.empty((1, 64, 64, 8), device="cuda", dtype=torch.float32)
+    t = torch.zeros_like(y)
+    net.forward(x, y)
+    net.backward(t)
+    b.get_rhs(qh)
+    with pytest.raises(QgnError, match="closure"):
+        net.backward(t)
+    net.forward(x, y)
+    net.backward(t)
+    # an ensemble keeps its hold after the model detaches
+    b.set_time(0.0, 1000.0)
+    ens = Ensemble([b])
+    ens.step(1)
+    b.times()
+    b.set_closure(None)
+    with pytest.raises(QgnError, match="ensemble"):
+        net.close()
+    with pytest.raises(QgnError, match="ensemble"):
+        b.close()
+    with pytest.raises(QgnError, match="reallocate"):
+        net.forward(big, ybig)
+    ens.close()
+    net.forward(big, ybig)                           # free again: the buffers may grow
+    b.close()
+    net.close()
+    with pytest.raises(ValueError, match="int16"):
+        CNN(16, 8, co.init_params(16, 8, seed=4)).forward(x.float(), y)
