@@ -211,6 +211,21 @@ int qgn_dft_forward_device(qgn_dft* h, const void* grid, void* spec, void* strea
 int qgn_dft_backward_device(qgn_dft* h, const void* spec, void* grid, void* stream);
 int qgn_dft_free(qgn_dft* h);
 
+/* The transforms of qgn_dft_forward_device / qgn_dft_backward_device
+ * (forward != 0: r2c, DftiComputeForward; else c2r, DftiComputeBackward)
+ * without a handle, allocating nothing: for callers that cannot allocate
+ * device memory (work recorded under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.  in/out are device arrays in the DFTI layout of the handle
+ * variants (float data for QGN_MIXED); stream NULL is the null stream; the
+ * call returns without waiting.  The intermediate spectrum and twiddle
+ * tables live in a fixed 64 MiB device array of the library on each GPU
+ * (held until the process ends): howmany*ny*(nx/2+1) complex values plus
+ * the tables must fit it, else QGN_EINVAL (1024^2 FP64: up to 7 transforms).
+ * Calls on one GPU share that array, so calls that overlap on one GPU
+ * race. */
+int qgn_dft_compute_noalloc(int precision, int forward, int nx, int ny, int howmany, const void* in, void* out,
+                            void* stream);
 
 /* ---- LAPACK DGEEV replacement for the vertical structure ---------------
  * QGN_Module.f90:84-148: H <- Htot*H/sum(H); tridiagonal stretching matrix

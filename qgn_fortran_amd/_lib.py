@@ -25,7 +25,7 @@ EXPORTS = [
     "qgn_batch_get_time", "qgn_ensemble_create", "qgn_ensemble_step", "qgn_ensemble_destroy",
     "qgn_cnn_create", "qgn_cnn_destroy", "qgn_cnn_nparams", "qgn_cnn_get_params", "qgn_cnn_set_params",
     "qgn_cnn_forward", "qgn_cnn_backward", "qgn_cnn_adam", "qgn_cnn_train_step", "qgn_cnn_last_error",
-    "qgn_set_closure", "qgn_get_tuning", "qgn_set_tuning",
+    "qgn_set_closure", "qgn_get_tuning", "qgn_set_tuning", "qgn_dft_compute_noalloc",
 ]
 
 
@@ -86,6 +86,7 @@ def load():
         "qgn_dft_backward": (I, [P, D, D]),
         "qgn_dft_forward_device": (I, [P, P, P, P]),
         "qgn_dft_backward_device": (I, [P, P, P, P]),
+        "qgn_dft_compute_noalloc": (I, [I, I, I, I, I, P, P, P]),
         "qgn_dft_free": (I, [P]),
         "qgn_vertical_modes": (I, [I, C.c_double, D, D, D, D, D, D, D, D]),
         "qgn_create_batch": (I, [C.POINTER(P), C.POINTER(QgnParams), I, D, D, D, D, D, D]),

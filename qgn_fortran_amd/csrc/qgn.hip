@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <utility>
@@ -42,31 +43,33 @@ int fail(int code, const std::string& msg) {
 
 constexpr int kEPT = 16;        // FFT elements per thread bound (template)
 constexpr int kColThreads = 256;
+// fixed device scratch of qgn_dft_compute_noalloc, per GPU
+constexpr size_t kDftScratchBytes = size_t(64) << 20;
 constexpr long double kPi = 3.141592653589793238462643383279502884L;
 
 // ---------------------------------------------------------------------------
 // FFT plans and twiddle tables
 // ---------------------------------------------------------------------------
 bool make_plan(int n, FftPlan* P) {
-  std::vector<int> rad;
+  // radices 3 and 5 first, then 8s with a 4, two 4s or a 2 for the
+  // remaining powers of two (fixed arrays: no allocation on this path)
+  int rad[64], k = 0;
   int m = n, e2 = 0;
   while (m % 2 == 0) { m /= 2; ++e2; }
-  int n8 = e2 / 3, rem = e2 % 3;
-  std::vector<int> twos;
-  for (int i = 0; i < n8; ++i) twos.push_back(8);
-  if (rem == 1) {
-    if (n8 > 0) { twos.pop_back(); twos.push_back(4); twos.push_back(4); }
-    else twos.push_back(2);
-  } else if (rem == 2) {
-    twos.push_back(4);
-  }
-  while (m % 3 == 0) { rad.push_back(3); m /= 3; }
-  while (m % 5 == 0) { rad.push_back(5); m /= 5; }
+  while (m % 3 == 0 && k < 32) { rad[k++] = 3; m /= 3; }
+  while (m % 5 == 0 && k < 32) { rad[k++] = 5; m /= 5; }
   if (m != 1) return false;
-  rad.insert(rad.end(), twos.begin(), twos.end());
-  if ((int)rad.size() > kMaxPasses) return false;
+  const int n8 = e2 / 3, rem = e2 % 3;
+  for (int i = 0; i < n8 - (rem == 1 && n8 > 0 ? 1 : 0); ++i) rad[k++] = 8;
+  if (rem == 1) {
+    if (n8 > 0) { rad[k++] = 4; rad[k++] = 4; }
+    else rad[k++] = 2;
+  } else if (rem == 2) {
+    rad[k++] = 4;
+  }
+  if (k > kMaxPasses) return false;
   P->n = n;
-  P->npass = (int)rad.size();
+  P->npass = k;
   int p = 1;
   for (int i = 0; i < P->npass; ++i) {
     P->radix[i] = rad[i];
@@ -1991,6 +1994,95 @@ int qgn_dft_forward_device(qgn_dft* h, const void* grid, void* spec, void* strea
 int qgn_dft_backward_device(qgn_dft* h, const void* spec, void* grid, void* stream) {
   if (!h || !grid || !spec) return fail(QGN_EINVAL, "null argument");
   return dft_run(h, spec, grid, stream ? (hipStream_t)stream : h->stream, false);
+}
+
+}  // extern "C"
+
+// ---- Transforms that allocate nothing ---------------------------------------
+// qgn_dft_compute_noalloc keeps its intermediate spectrum and its twiddle
+// tables in `scratch`, a fixed array of every GPU the library runs on (held
+// until the process ends).  Host code never passes `scratch` itself to a
+// kernel: its device address on the current GPU comes from
+// hipGetSymbolAddress, looked up once per device and kept in
+// g_scratch_dev.  Every call on one GPU shares that GPU's array, so calls
+// that overlap on one GPU race.
+static __device__ __align__(256) unsigned char scratch[kDftScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxDevices] = {};
+
+int scratch_on_device(void** out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(QGN_EINVAL, "device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(QGN_EHIP, "hipGetSymbolAddress returned no address for the scratch array");
+    g_scratch_dev[dev] = p;
+  }
+  *out = g_scratch_dev[dev];
+  return 0;
+}
+
+// exp(-2 pi i m / n) for m < count, formed in FP64 and rounded to R
+template <typename R>
+__global__ void k_twiddle_table(cplx<R>* __restrict__ tw, int n, int count) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= count) return;
+  double sv, cv;
+  sincospi(-2.0 * (double)m / (double)n, &sv, &cv);
+  tw[m] = cmk<cplx<R>>((R)cv, (R)sv);
+}
+
+size_t round256(size_t b) { return (b + 255) / 256 * 256; }
+
+template <typename R>
+int dft_noalloc(bool fwd, int nx, int ny, int howmany, const void* in, void* out, hipStream_t s) {
+  std::string why;
+  if (!size_ok(nx, ny, &why)) return fail(QGN_EINVAL, why);
+  if (howmany < 1) return fail(QGN_EINVAL, "howmany must be >= 1");
+  const Geom g = make_geom(nx, ny, howmany, 1.0, 1.0);
+  FftPlan px, py;
+  make_plan(ny, &py);
+  make_plan(nx / 2, &px);
+  const size_t cs = sizeof(cplx<R>);
+  const size_t tmp_bytes = round256((size_t)howmany * ny * g.nxh * cs);
+  const int ntw = ny + nx / 2 + (nx / 2 + 1);
+  const size_t need = tmp_bytes + (size_t)ntw * cs;
+  if (need > kDftScratchBytes)
+    return fail(QGN_EINVAL, "qgn_dft_compute_noalloc: " + std::to_string(need) + " bytes of scratch needed, more than "
+                            "its fixed " + std::to_string(kDftScratchBytes) + "; use a qgn_dft handle");
+  void* base = nullptr;
+  CHK(scratch_on_device(&base));
+  // the whole intermediate spectrum is written before it is read, and the
+  // tables are rewritten by every call: nothing has to start at zero
+  cplx<R>* tmp = reinterpret_cast<cplx<R>*>(base);
+  Tw<R> tw;
+  tw.y = reinterpret_cast<cplx<R>*>(static_cast<unsigned char*>(base) + tmp_bytes);
+  tw.x = tw.y + ny;
+  tw.r = tw.x + nx / 2;
+  hipLaunchKernelGGL((k_twiddle_table<R>), dim3((ny + 255) / 256), dim3(256), 0, s, tw.y, ny, ny);
+  hipLaunchKernelGGL((k_twiddle_table<R>), dim3((nx / 2 + 255) / 256), dim3(256), 0, s, tw.x, nx / 2, nx / 2);
+  hipLaunchKernelGGL((k_twiddle_table<R>), dim3((nx / 2 + 1 + 255) / 256), dim3(256), 0, s, tw.r, nx, nx / 2 + 1);
+  HIPCHK(hipGetLastError());
+  if (fwd) return dft_fwd<R>(s, g, px, py, tw, howmany, static_cast<const R*>(in), static_cast<cplx<R>*>(out), tmp);
+  return dft_bwd<R>(s, g, px, py, tw, howmany, static_cast<const cplx<R>*>(in), static_cast<R*>(out), tmp, (R)1);
+}
+}  // namespace
+
+extern "C" {
+
+int qgn_dft_compute_noalloc(int precision, int forward, int nx, int ny, int howmany, const void* in, void* out,
+                            void* stream) {
+  if (!in || !out) return fail(QGN_EINVAL, "null argument");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (precision == QGN_FP64) return dft_noalloc<double>(forward != 0, nx, ny, howmany, in, out, s);
+  if (precision == QGN_MIXED) return dft_noalloc<float>(forward != 0, nx, ny, howmany, in, out, s);
+  return fail(QGN_EINVAL, "unknown precision");
 }
 
 }  // extern "C"
