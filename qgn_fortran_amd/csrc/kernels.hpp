@@ -466,20 +466,23 @@ __device__ __forceinline__ qgn_float4 mfma3(const qgn_short4& ah, const qgn_shor
 }
 
 // Stage combination (or, with a.ns = 0 and a.out = nullptr, a plain copy of
-// q) followed by the MFMA projection, nz <= 16.  Workgroup: T tiles of 32
-// points x 16 layers, 2T (point, layer) elements per thread for the FP64
-// stage sums (consecutive threads on consecutive points: 512 contiguous bytes
-// of q per layer), all of a thread's loads issued before its first store;
-// then, per tile, each of the 4 waves projects 8 points (16 columns: re, im).
-// Every wave builds the split matrix operands once, before the stage loads, so
-// those small loads overlap them.  T > 1 keeps more bytes in flight where a
+// q) followed by the MFMA projection, nz <= 16.  A group is T tiles of 32
+// points x 16 layers; workgroups walk the groups (grid-stride: the grid
+// holds as many workgroups as fit on the GPU at once, so the last round
+// leaves no CU idle).  Per group: 2T (point, layer) elements per thread for
+// the FP64 stage sums (consecutive threads on consecutive points: 512
+// contiguous bytes of q per layer), all of a thread's loads issued before
+// its first store; then, per tile, each of the 4 waves projects 8 points
+// (16 columns: re, im); psi goes out through LDS, 16 bytes per thread over
+// whole 256-byte rows of a layer.  Every wave builds the split matrix
+// operands once per workgroup.  T > 1 keeps more bytes in flight where a
 // stage has few input arrays.
 template <typename S, int NS, int T>
 __global__ void __launch_bounds__(256) k_stage_proj_mfma16(StageArgs a, Geom g, const double* __restrict__ HM,
                                                          const double* __restrict__ MD,
                                                          const double* __restrict__ evals, double inv_htot,
                                                          float2* __restrict__ psi) {
-  __shared__ double2 sq[T][16][32];    // [tile][layer][point]
+  __shared__ double2 sq[T][16][32];    // [tile][layer][point]; then psi as float2 [tile][layer][point]
   const int nz = g.nzm;
   const int mb = blockIdx.y;         // member
   HM += mb * nz * nz;
@@ -489,7 +492,7 @@ __global__ void __launch_bounds__(256) k_stage_proj_mfma16(StageArgs a, Geom g, 
   const double dt = a.dtv ? a.dtv[mb] : a.dt;
   const int tid = threadIdx.x;
   const long plane = (long)g.Ky * g.Kxp;
-  const long p0 = (long)blockIdx.x * (32 * T);       // first point of the workgroup
+  const long ngroups = (plane + 32 * T - 1) / (32 * T);
   const int l = tid & 63;
   const int wv = tid >> 6;                    // wave: points 8*wv .. 8*wv+7 of each tile
   const int row = l & 15, kb = 4 * (l >> 4);  // A: i = row; B, D: j = row
@@ -506,77 +509,95 @@ __global__ void __launch_bounds__(256) k_stage_proj_mfma16(StageArgs a, Geom g, 
     a2l[r] = bf16_lo(m2, a2h[r]);
     ev[r] = kk < nz ? evals[kk] : 1.0;
   }
-  {
-    const int pp = tid & 31, k0 = tid >> 5;   // layers k0 and k0 + 8
-    double2 qt[T][2];
+  float2* spsi = reinterpret_cast<float2*>(&sq[0][0][0]);   // [T][16][32] after the operands are read
+  for (long grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+    const long p0 = grp * (32 * T);             // first point of the group
+    {
+      const int pp = tid & 31, k0 = tid >> 5;   // layers k0 and k0 + 8
+      double2 qt[T][2];
 #pragma unroll
-    for (int t = 0; t < T; ++t) {
-      const long pt = p0 + 32 * t + pp;
+      for (int t = 0; t < T; ++t) {
+        const long pt = p0 + 32 * t + pp;
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const int k = k0 + 8 * h;
-        qt[t][h] = make_double2(0.0, 0.0);
-        if (k < nz && pt < plane) qt[t][h] = stage_value<S, NS>(a, g, moff + k * plane + pt, pt, mb * nz + k, dt);
-      }
-    }
-#pragma unroll
-    for (int t = 0; t < T; ++t) {
-      const long pt = p0 + 32 * t + pp;
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const int k = k0 + 8 * h;
-        if (k < nz && pt < plane) {
-          if (a.out) a.out[moff + k * plane + pt] = qt[t][h];
-          if (a.out32) a.out32[moff + k * plane + pt] = make_float2((float)qt[t][h].x, (float)qt[t][h].y);
+        for (int h = 0; h < 2; ++h) {
+          const int k = k0 + 8 * h;
+          qt[t][h] = make_double2(0.0, 0.0);
+          if (k < nz && pt < plane) qt[t][h] = stage_value<S, NS>(a, g, moff + k * plane + pt, pt, mb * nz + k, dt);
         }
-        sq[t][k][pp] = qt[t][h];
+      }
+      __syncthreads();   // the previous group's psi rows have been read out of sq
+#pragma unroll
+      for (int t = 0; t < T; ++t) {
+        const long pt = p0 + 32 * t + pp;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int k = k0 + 8 * h;
+          if (k < nz && pt < plane) {
+            if (a.out) a.out[moff + k * plane + pt] = qt[t][h];
+            if (a.out32) a.out32[moff + k * plane + pt] = make_float2((float)qt[t][h].x, (float)qt[t][h].y);
+          }
+          sq[t][k][pp] = qt[t][h];
+        }
       }
     }
-  }
-  __syncthreads();
+    __syncthreads();
+    qgn_float4 pv[T];
 #pragma unroll
-  for (int t = 0; t < T; ++t) {
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const double2 v = sq[t][kb + r][8 * wv + (row >> 1)];
-      const float x = (float)((row & 1) ? v.y : v.x);
-      bh[r] = bf16_rne(x);
-      bl[r] = bf16_lo(x, bh[r]);
-    }
-    qgn_float4 d = {0.f, 0.f, 0.f, 0.f};
-    d = mfma3(a1h, a1l, bh, bl, d);            // modal amplitudes, D[mode][column]
-    const long pt = p0 + 32 * t + 8 * wv + (row >> 1);
-    bool zero_mode = pt >= plane;
-    double k2 = 0.0;
-    if (!zero_mode) {
-      const int i = (int)pt % g.Kxp;
-      const int j = compact_row_to_full((int)pt / g.Kxp, g);
-      const double kx = kx_of(i, g), ky = ky_of(j, g);
-      k2 = kx * kx + ky * ky;
-      zero_mode = (i == 0 && j == 0) || i >= g.Kx;
-    }
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int mode = kb + r;
-      const float pm = (zero_mode || mode >= nz) ? 0.f : d[r] / (float)(-k2 + ev[r]);
-      bh[r] = bf16_rne(pm);
-      bl[r] = bf16_lo(pm, bh[r]);
-    }
-    qgn_float4 p = {0.f, 0.f, 0.f, 0.f};
-    p = mfma3(a2h, a2l, bh, bl, p);            // D[layer][column]
-    if (pt < plane) {
-      float* out = reinterpret_cast<float*>(psi);
+    for (int t = 0; t < T; ++t) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int layer = kb + r;
-        if (layer < nz) out[2 * (moff + layer * plane + pt) + (row & 1)] = p[r];
+        const double2 v = sq[t][kb + r][8 * wv + (row >> 1)];
+        const float x = (float)((row & 1) ? v.y : v.x);
+        bh[r] = bf16_rne(x);
+        bl[r] = bf16_lo(x, bh[r]);
+      }
+      qgn_float4 d = {0.f, 0.f, 0.f, 0.f};
+      d = mfma3(a1h, a1l, bh, bl, d);            // modal amplitudes, D[mode][column]
+      const long pt = p0 + 32 * t + 8 * wv + (row >> 1);
+      bool zero_mode = pt >= plane;
+      double k2 = 0.0;
+      if (!zero_mode) {
+        const int i = (int)pt % g.Kxp;
+        const int j = compact_row_to_full((int)pt / g.Kxp, g);
+        const double kx = kx_of(i, g), ky = ky_of(j, g);
+        k2 = kx * kx + ky * ky;
+        zero_mode = (i == 0 && j == 0) || i >= g.Kx;
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int mode = kb + r;
+        const float pm = (zero_mode || mode >= nz) ? 0.f : d[r] / (float)(-k2 + ev[r]);
+        bh[r] = bf16_rne(pm);
+        bl[r] = bf16_lo(pm, bh[r]);
+      }
+      qgn_float4 pz = {0.f, 0.f, 0.f, 0.f};
+      pv[t] = mfma3(a2h, a2l, bh, bl, pz);       // D[layer][column]
+    }
+    __syncthreads();   // every wave has read its B operands out of sq
+    float* sp = reinterpret_cast<float*>(spsi);
+#pragma unroll
+    for (int t = 0; t < T; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) sp[2 * ((t * 16 + kb + r) * 32 + 8 * wv + (row >> 1)) + (row & 1)] = pv[t][r];
+    __syncthreads();
+    // tile t, layer k: 32 points = 256 bytes; 16 threads per row, 2 points each
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+      const int k = tid >> 4, pp2 = 2 * (tid & 15);
+      const long pt = p0 + 32 * t + pp2;
+      if (k < nz && pt < plane) {
+        const float4 v = *reinterpret_cast<const float4*>(&spsi[(t * 16 + k) * 32 + pp2]);
+        float2* dst = psi + moff + k * plane + pt;
+        if (pt + 1 < plane) *reinterpret_cast<float4*>(dst) = v;
+        else *dst = make_float2(v.x, v.y);
       }
     }
   }
 }
 
-// tiles per workgroup of k_stage_proj_mfma16 for a stage with NS earlier
-// stages (measured at 1024^2 x 16: 4 and 2 beat 1, 8 and 4)
+// tiles per group of k_stage_proj_mfma16 for a stage with NS earlier
+// stages (measured at 1024^2 x 16: 4 and 2; one tile per group for NS >= 3
+// takes more registers, not fewer)
 template <int NS> struct ProjTiles { static constexpr int T = NS == 0 ? 4 : 2; };
 
 // Stage combination (or, with a.ns = 0 and a.out = nullptr, a plain copy of

@@ -276,7 +276,10 @@ __global__ void __launch_bounds__(ROWS * (N2 / 8)) k_row_jac_fused2(RowJacArgs a
   const int Kx = g.Kx;
   const long Kxp = g.Kxp;
   const long rel = (long)r * Kxp;   // this thread's row within the group
-  // x-c2r of one mixed row into registers: o[j] holds x[2m], x[2m+1] at m = t + j NB
+  // x-c2r of one mixed row into registers: o[j] holds x[2m], x[2m+1] at m = t + j NB,
+  // unnormalised (every product below is of two such fields: s2 = scale^2
+  // normalises it once)
+  const R s2 = scale * scale;
   auto inverse = [&](const void* src, Cx(&o)[8]) {
     const __amdgpu_buffer_rsrc_t rs =
         row_rsrc(reinterpret_cast<const Cx*>(src) + (long)row0 * Kxp, ROWS * (int)Kxp * (int)sizeof(Cx));
@@ -284,7 +287,7 @@ __global__ void __launch_bounds__(ROWS * (N2 / 8)) k_row_jac_fused2(RowJacArgs a
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       const int k = t + j * NB;
-      v[j] = c2r_pre_rs<R>(rs, rel, Kx, k, N2, tr, scale);
+      v[j] = c2r_pre_rs<R>(rs, rel, Kx, k, N2, tr, (R)1);
     }
     Dft<Cx, 8>::run(v, +1);
     __syncthreads();   // earlier readers of buf are done
@@ -324,13 +327,15 @@ __global__ void __launch_bounds__(ROWS * (N2 / 8)) k_row_jac_fused2(RowJacArgs a
     Cx w8[8];
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
-      const R r0 = sqrt(fu[m].x * fu[m].x + fv[m].x * fv[m].x), r1 = sqrt(fu[m].y * fu[m].y + fv[m].y * fv[m].y);
+      const R r0 = s2 * sqrt(fu[m].x * fu[m].x + fv[m].x * fv[m].x);
+      const R r1 = s2 * sqrt(fu[m].y * fu[m].y + fv[m].y * fv[m].y);
       w8[m] = cmk<Cx>(r0 * fu[m].x, r1 * fu[m].y);
     }
     forward(w8, reinterpret_cast<Cx*>(a.DU) + drow);
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
-      const R r0 = sqrt(fu[m].x * fu[m].x + fv[m].x * fv[m].x), r1 = sqrt(fu[m].y * fu[m].y + fv[m].y * fv[m].y);
+      const R r0 = s2 * sqrt(fu[m].x * fu[m].x + fv[m].x * fv[m].x);
+      const R r1 = s2 * sqrt(fu[m].y * fu[m].y + fv[m].y * fv[m].y);
       w8[m] = cmk<Cx>(r0 * fv[m].x, r1 * fv[m].y);
     }
     forward(w8, reinterpret_cast<Cx*>(a.DV) + drow);
@@ -343,7 +348,8 @@ __global__ void __launch_bounds__(ROWS * (N2 / 8)) k_row_jac_fused2(RowJacArgs a
     Cx qy[8];
     inverse(a.QY, qy);
 #pragma unroll
-    for (int m = 0; m < 8; ++m) acc[m] = cmk<Cx>(acc[m].x + fv[m].x * qy[m].x, acc[m].y + fv[m].y * qy[m].y);
+    for (int m = 0; m < 8; ++m)
+      acc[m] = cmk<Cx>(s2 * (acc[m].x + fv[m].x * qy[m].x), s2 * (acc[m].y + fv[m].y * qy[m].y));
   }
   forward(acc, reinterpret_cast<Cx*>(a.J) + (long)row0 * Kxp);
 }

@@ -527,22 +527,31 @@ int with_ns(int ns, F&& f) {
 // Mixed precision with nz <= 32: the stage + split-BF16 MFMA projection.
 int launch_stage_proj_mfma(qgn_ctx* c, const StageArgs& a, int kernel_id) {
   const long plane = (long)c->g.Ky * c->g.Kxp;
-  // nz <= 16: ProjTiles<NS>::T tiles of 32 points per workgroup; nz <= 32: workgroups walk
+  // nz <= 16: groups of ProjTiles<NS>::T tiles of 32 points, walked by as
+  // many workgroups as fit on the GPU at once; nz <= 32: workgroups walk
   // tiles of 16 points, 8 per CU over all members, so the 4x larger split
   // matrices are built once per workgroup
   return with_ns(a.ns, [&](auto nsc) {
     constexpr int NS = decltype(nsc)::value;
     constexpr int T = ProjTiles<NS>::T;
-    const long tiles = c->g.nzm <= 16 ? (plane + 32 * T - 1) / (32 * T) : (plane + 15) / 16;
-    const long want = c->g.nzm <= 16 ? tiles : std::max(1L, (8L * c->ncu + c->nm - 1) / c->nm);
+    if (c->g.nzm <= 16) {
+      auto kern = k_stage_proj_mfma16<float, NS, T>;
+      const long groups = (plane + 32 * T - 1) / (32 * T);
+      int per_cu = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+      const long fit = std::max(1L, ((long)per_cu * c->ncu + c->nm - 1) / c->nm);
+      const int blocks = (int)std::min(groups, fit);
+      return timed(c, kernel_id, [&] {
+        hipLaunchKernelGGL(kern, dim3(blocks, c->nm), dim3(256), 0, c->stream, a, c->g, c->HM, c->MD, c->evals,
+                           1.0 / c->p.Htot, reinterpret_cast<float2*>(c->psi));
+      });
+    }
+    const long tiles = (plane + 15) / 16;
+    const long want = std::max(1L, (8L * c->ncu + c->nm - 1) / c->nm);
     const int blocks = (int)std::min(tiles, want);
     return timed(c, kernel_id, [&] {
-      if (c->g.nzm <= 16)
-        hipLaunchKernelGGL((k_stage_proj_mfma16<float, NS, T>), dim3(blocks, c->nm), dim3(256), 0, c->stream, a,
-                           c->g, c->HM, c->MD, c->evals, 1.0 / c->p.Htot, reinterpret_cast<float2*>(c->psi));
-      else
-        hipLaunchKernelGGL((k_stage_proj_mfma<float, 32, NS>), dim3(blocks, c->nm), dim3(256), 0, c->stream, a, c->g,
-                           c->HM, c->MD, c->evals, 1.0 / c->p.Htot, reinterpret_cast<float2*>(c->psi));
+      hipLaunchKernelGGL((k_stage_proj_mfma<float, 32, NS>), dim3(blocks, c->nm), dim3(256), 0, c->stream, a, c->g,
+                         c->HM, c->MD, c->evals, 1.0 / c->p.Htot, reinterpret_cast<float2*>(c->psi));
     });
   });
 }
