@@ -21,12 +21,16 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# Algorithmic HBM bytes per launch of the RHS passes, counting only the kept
-# (unpadded) modes: rs = bytes of one complex of the pass's arithmetic type;
-# the stage value q is read as the projection kernel's FP32 copy (rs bytes)
-# on the mixed path and as the FP64 state (16 bytes) on the FP64 path.
+# Algorithmic HBM bytes of each kernel group of one attempt, counting only the
+# kept (unpadded) modes: rs = bytes of one complex of the pass's arithmetic
+# type; the stage value is read as the projection kernel's FP32 copy (rs
+# bytes) on the mixed path and as the FP64 state (16 bytes) on the FP64 path.
 # ncmp = nz*Ky*Kx compact modes, nmix = nz*ny*Kx mixed values.
-def kernel_bytes(kernel, g, rs):
+def kernel_bytes(kernel, g, rs, qrs=None):
+    """Bytes per launch (row_jac, col_inv, col_rhs, project, update) or per
+    attempt of all launches (stage: NS = 1..5; errnorm: the FP64 column and
+    row passes of the error field)."""
+    qrs = rs if qrs is None else qrs
     nz, ny, Kx, Ky = g["nz"], g["ny"], g["Kx"], g["Ky"]
     ncmp = nz * Ky * Kx
     nmix = nz * ny * Kx
@@ -34,29 +38,47 @@ def kernel_bytes(kernel, g, rs):
     if kernel == "row_jac":     # read u, v, q_x, q_y; write J (+ |u|u, |u|v on layer nz)
         return 5 * nmix * rs + drag
     if kernel == "col_inv":     # read q, psi; write u, v, q_x, q_y
-        return 2 * ncmp * rs + 4 * nmix * rs
+        return (qrs + rs) * ncmp + 4 * nmix * rs
     if kernel == "col_rhs":     # read J, q, psi; write N (+ read |u|u, |u|v for layer nz)
-        return nmix * rs + 3 * ncmp * rs + drag
+        return nmix * rs + (qrs + 2 * rs) * ncmp + drag
+    if kernel == "project":     # read q (FP64); write psi and the stage value's FP32 copy (mixed)
+        return ncmp * (16 + rs + (rs if rs == 8 else 0))
+    if kernel == "stage":       # NS = 1..5: read q and NS stage RHS; write psi and the stage value
+        # (its FP32 copy on the mixed path, the FP64 value on the FP64 path)
+        return sum(ncmp * (16 + ns * rs + rs + (8 if rs == 8 else 16)) for ns in range(1, 6))
+    if kernel == "update":      # read q and 6 stage RHS; write the candidate and the error field (FP64)
+        return ncmp * (16 + 6 * rs + 32)
+    if kernel == "errnorm":     # FP64 y-inverse (read compact, write mixed) and row max (read mixed)
+        return ncmp * 16 + 2 * nmix * 16
     raise ValueError(kernel)
 
 
-def measured_traffic(workload, kernel):
-    """HBM bytes per launch of `kernel` from the committed PMC summary
-    (profiles/traffic_<workload>.txt), or None."""
+def projection_flops(g):
+    """BF16 MFMA flops of one stage/projection launch (nz <= 16): per tile of
+    32 points, 4 waves x 2 products x 3 MFMAs of 16x16x16."""
+    plane = g["Ky"] * g["Kxp"]
+    return -(-plane // 32) * 4 * 2 * 3 * 2 * 16 * 16 * 16
+
+
+def profile_figures(workload, kernel):
+    """(HBM bytes per launch, its commit, rocprof average us) of `kernel` from
+    the committed summaries profiles/traffic_<workload>.txt, or Nones."""
     path = os.path.join(ROOT, "profiles", f"traffic_{workload}.txt")
     if not os.path.exists(path):
-        return None
+        return None, None, None
     for line in open(path):
         f = dict(kv.split("=", 1) for kv in line.split() if "=" in kv)
         if f.get("kernel") == kernel:
-            return int(f["bytes_per_launch"])
-    return None
+            avg = float(f["avg_us"]) if "avg_us" in f else None
+            return int(f["bytes_per_launch"]), f.get("commit"), avg
+    return None, None, None
 
 
 def geom(p):
     ny = p.ny
     ya, yb = ny // 3, 2 * (ny // 3) + 1
-    return dict(nx=p.nx, ny=ny, nz=p.nz, Kx=p.nx // 3, Ky=ya + (ny - yb), drag=p.C_d != 0.0)
+    Kx = p.nx // 3
+    return dict(nx=p.nx, ny=ny, nz=p.nz, Kx=Kx, Kxp=-(-Kx // 16) * 16, Ky=ya + (ny - yb), drag=p.C_d != 0.0)
 
 
 def _model(wl, precision):
@@ -82,7 +104,7 @@ def run_gpu(wl, precision, q_spun, t0, dt0, steps, warmup, prof_kernel):
         t, dt = m.time_step(t, dt)
     q_w, t_w, dt_w = m.get_state(), t, dt
     st0 = m.stats()
-    m.prof_enable(prof_kernel)
+    m.prof_enable("all")
     torch.cuda.synchronize()
     tw0 = time.perf_counter()
     ts = t
@@ -90,7 +112,8 @@ def run_gpu(wl, precision, q_spun, t0, dt0, steps, warmup, prof_kernel):
         t, dt = m.time_step(t, dt)
     torch.cuda.synchronize()
     wall = time.perf_counter() - tw0
-    nlaunch, kms = m.prof_read()
+    per = m.prof_read_all()
+    nlaunch, kms = per[prof_kernel]
     m.prof_enable(-1)
     st1 = m.stats()
     res = dict(
@@ -99,6 +122,7 @@ def run_gpu(wl, precision, q_spun, t0, dt0, steps, warmup, prof_kernel):
         mean_dt=(t - ts) / steps, rejects=st1["rejects"] - st0["rejects"],
         rhs_evals=st1["rhs_evals"] - st0["rhs_evals"], kernel=prof_kernel,
         kernel_launches=nlaunch, kernel_ms=kms / max(nlaunch, 1), t_end=t, dt_end=dt,
+        kernels={k: dict(launches=n, ms=ms) for k, (n, ms) in per.items()},
     )
     res["ms_per_rhs"] = 1e3 * wall / max(res["rhs_evals"], 1)
     # graph path from the same post-warm-up state
@@ -148,30 +172,56 @@ def spin_up(wl, steps, precision="mixed"):
     return q, float(info["t"][0]), float(info["dt"][0])
 
 
+def _cpu_model_name():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(wl, q_spun, t0, dt0, budget_s=30.0):
-    """The FP64 oracle (scipy pocketfft, multithreaded FFTs) on the host: as
-    many accepted ARK43_QGLeith steps as fit in ~budget_s, at least one."""
+    """The reference algorithm on the host cores, from the same spun-up
+    state: the C++/OpenMP FP64 restatement with MKL DFTI transforms
+    (oracle/qgn_cpu.cpp; MKL is the reference's FFT library), as many
+    accepted ARK43_QGLeith steps as fit in ~budget_s/2, at least one.
+    Falls back to the numpy/scipy oracle where the restatement or MKL is
+    missing."""
     from oracle import qgn_oracle as orc
     from qgn_fortran_amd.model import vertical_modes
     p = wl.params
-    cores = min(os.cpu_count() or 1, 16)
+    cores = int(os.environ.get("OMP_NUM_THREADS", min(os.cpu_count() or 1, 16)))
     H, _L, Modes, EVals, qy = vertical_modes(wl.H, wl.S, wl.uBar, p.Htot)
     po = orc.Params(nx=p.nx, ny=p.ny, nz=p.nz, Lx=p.Lx, Ly=p.Ly, Htot=p.Htot, beta=p.beta, A2=p.A2, A8=p.A8,
                     r0=p.r0, C_d=p.C_d, TOL=p.TOL, is_explicit=p.is_explicit, dt_max=p.dt_max)
-    om = orc.Model(po, H, Modes, EVals, wl.uBar, qy, workers=cores)
-    stp = orc.ARK43(om, leith=True)
+    what = "C++/OpenMP FP64 restatement with MKL DFTI (oracle/qgn_cpu.cpp)"
+    try:
+        from oracle import qgn_cpu
+        cm = qgn_cpu.Model(po, H, Modes, EVals, wl.uBar, qy, threads=cores)
+        cores = cm.threads
+        step = lambda q, t, dt: cm.ark43_step(q, t, dt, leith=True)
+    except (RuntimeError, OSError, ImportError) as e:
+        om = orc.Model(po, H, Modes, EVals, wl.uBar, qy, workers=cores)
+        stp = orc.ARK43(om, leith=True)
+        step = stp.step
+        what = f"FP64 numpy/scipy oracle (the C++ restatement did not load: {e})"
     q, t, dt = q_spun, t0, dt0
+    q, t, dt = step(q, t, dt)           # first step: page faults, FFT plans, thread start-up
+    t1 = t
     tw = time.perf_counter()
     n = 0
     while True:
-        q, t, dt = stp.step(q, t, dt)
+        q, t, dt = step(q, t, dt)
         n += 1
-        if time.perf_counter() - tw > budget_s * 0.5 or n >= 20:
+        if time.perf_counter() - tw > budget_s * 0.5 or n >= 40:
             break
     wall = time.perf_counter() - tw
-    return dict(value=(t - t0) / 86400.0 / wall, unit="simulated days/s", cores=cores, kind="port",
-                sample=f"{n} accepted ARK43_QGLeith step(s) of the same 1024^2x16 state, FP64 numpy/scipy "
-                       f"oracle, {wall:.1f} s")
+    return dict(value=(t - t1) / 86400.0 / wall, unit="simulated days/s", cores=cores, kind="port",
+                sample=f"{n} accepted ARK43_QGLeith steps of the same spun-up 1024^2x16 state after one untimed step, "
+                       f"{what}, {wall:.1f} s, {1e3 * wall / n:.0f} ms/step",
+                cpu_model=_cpu_model_name(), nproc=os.cpu_count())
 
 
 def run_cnn(steps, warmup):
@@ -180,12 +230,13 @@ def run_cnn(steps, warmup):
     forward passes on 4 images (the 4-member coarse ensemble), with HIP-event
     timings and the dense-MFMA flops of each."""
     import torch
-    from oracle import cnn_oracle as co
-    from qgn_fortran_amd.cnn import CNN, to_bf16_bits
+    from qgn_fortran_amd.closure import _he_init
+    from qgn_fortran_amd.cnn import CNN, HIDDEN, to_bf16_bits
     cin, cout, H, W = 16, 8, 256, 256
     rng = np.random.default_rng(0)
-    net = CNN(cin, cout, co.init_params(cin, cout, 1))
-    macs_px = sum(o * 9 * i for i, o in co.shapes(cin, cout))        # per pixel, forward
+    net = CNN(cin, cout, _he_init(cin, cout, 1))
+    dims = [(cin, HIDDEN), (HIDDEN, HIDDEN), (HIDDEN, HIDDEN), (HIDDEN, cout)]
+    macs_px = sum(o * 9 * i for i, o in dims)        # per pixel, forward
     out = {}
     for name, n, train in (("train_step", 10, True), ("forward", 4, False)):
         x = torch.from_numpy(to_bf16_bits(rng.standard_normal((n, H, W, cin)))).cuda()
@@ -209,6 +260,43 @@ def run_cnn(steps, warmup):
     return out
 
 
+def run_ensemble(precision, attempts, spinup):
+    """The 96-member ensemble (32 members of 8 layers on each of 128^2,
+    256^2, 384^2, ARK43_QGLeith): `spinup` attempts from the seeded states,
+    then `attempts` timed attempts of every member through chained graph
+    launches.  Member-days per second sums the simulated time every member
+    advanced."""
+    import torch
+    from qgn_fortran_amd import Batch, Ensemble, configs
+    groups = configs.ensemble96(precision=precision)
+    batches = []
+    for g in groups:
+        b = Batch(g.params, g.members)
+        b.initialize(g.q0)
+        b.set_time(0.0, g.dt0)
+        batches.append(b)
+    ens = Ensemble(batches)
+    ens.step(spinup)
+    t0s = [b.times()["t"].copy() for b in batches]
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    ens.step(attempts)
+    infos = [b.times() for b in batches]
+    wall = time.perf_counter() - w0
+    sizes = {}
+    total = 0.0
+    for g, t0, i in zip(groups, t0s, infos):
+        d = float(np.sum(i["t"] - t0)) / 86400.0
+        total += d
+        sizes[f"{g.params.nx}^2x{g.params.nz} x{len(g.members)}"] = dict(
+            member_days=d, rejects=int(i["rejects"].sum()), mean_dt=float(np.mean(i["dt"])))
+    ens.close()
+    for b in batches:
+        b.close()
+    return dict(precision=precision, members=96, attempts=attempts, spinup_attempts=spinup, wall_s=wall,
+                ms_per_attempt=1e3 * wall / attempts, member_days_per_s=total / wall, sizes=sizes)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -221,6 +309,10 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=30.0)
     ap.add_argument("--cnn", action="store_true",
                     help="time the closure network's MFMA kernels instead of the stepper")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the 96-member ensemble figures")
+    ap.add_argument("--ensemble-only", action="store_true", help="only the 96-member ensemble figures")
+    ap.add_argument("--ensemble-attempts", type=int, default=40)
+    ap.add_argument("--ensemble-spinup", type=int, default=100)
     ap.add_argument("--calib-copy", action="store_true",
                     help="first run a 1 GiB read + 1 GiB write multiply three times (PMC byte-counter reference)")
     a = ap.parse_args()
@@ -246,17 +338,54 @@ def main():
     if a.cnn:
         print(json.dumps({"cnn": run_cnn(a.steps, a.warmup)}), flush=True)
         return
+    if a.ensemble_only:
+        print(json.dumps({"ensemble": {p: run_ensemble(p, a.ensemble_attempts, a.ensemble_spinup)
+                                       for p in ("mixed", "fp64")}}), flush=True)
+        return
     wl = configs.WORKLOADS[a.workload]()
     q, t0, dt0 = spin_up(wl, a.spinup)
     mixed, _ = run_gpu(configs.WORKLOADS[a.workload](), "mixed", q, t0, dt0, a.steps, a.warmup, a.kernel)
     fp64, _ = run_gpu(configs.WORKLOADS[a.workload](), "fp64", q, t0, dt0, a.steps, a.warmup, a.kernel)
     g = geom(wl.params)
+    steps_timed = a.steps
+    passes = {}
+    for prec, res, rs in (("mixed", mixed, 8), ("fp64", fp64, 16)):
+        attempts = steps_timed + res["rejects"]
+        out = {}
+        for k, v in res["kernels"].items():
+            if v["launches"] == 0:
+                continue
+            per_attempt = k in ("stage", "errnorm")      # the byte model counts all launches of an attempt
+            nb = kernel_bytes(k, g, rs)
+            sec = v["ms"] * 1e-3 / (attempts if per_attempt else v["launches"])
+            out[k] = dict(launches=v["launches"], avg_us=round(1e6 * v["ms"] * 1e-3 / v["launches"], 2),
+                          bytes=nb, gbs=round(nb / sec / 1e9, 1), frac_hbm_peak=round(nb / sec / 8e12, 4))
+        passes[prec] = out
+    # MFMA utilisation of the split-BF16 projection (mixed; nz <= 16): the
+    # stage/projection kernels' BF16 flops over their summed time
+    kp = mixed["kernels"]
+    proj_ms = kp["project"]["ms"] + kp["stage"]["ms"]
+    proj_launches = kp["project"]["launches"] + kp["stage"]["launches"]
+    mfma = None
+    if wl.params.nz <= 16 and proj_ms > 0:
+        fl = projection_flops(g) * proj_launches
+        mfma = dict(kernel="k_stage_proj_mfma16", bf16_flops_per_launch=projection_flops(g),
+                    tflops=round(fl / (proj_ms * 1e-3) / 1e12, 2),
+                    frac_bf16_dense_peak=round(fl / (proj_ms * 1e-3) / 2.5e15, 5))
     rs = 8  # mixed path: FP32 complex intermediates
     kb = kernel_bytes(a.kernel, g, rs)
     achieved = kb / (mixed["kernel_ms"] * 1e-3) / 1e9
+    traffic, traffic_commit, prof_avg = profile_figures(a.workload, a.kernel)
     roof = dict(bound="hbm", achieved=round(achieved, 1), peak=8000.0, unit="GB/s",
-                frac=round(achieved / 8000.0, 4), traffic=measured_traffic(a.workload, a.kernel), kernel=a.kernel,
-                bytes_per_launch=kb, avg_launch_ms=mixed["kernel_ms"])
+                frac=round(achieved / 8000.0, 4), traffic=traffic, kernel=a.kernel,
+                bytes_per_launch=kb, avg_launch_ms=mixed["kernel_ms"],
+                timing="HIP events around each launch on the library's stream, qgn_time_step path",
+                profile_avg_ms=None if prof_avg is None else prof_avg * 1e-3,
+                profile_frac=None if prof_avg is None else round(kb / (prof_avg * 1e-6) / 8e12, 4),
+                traffic_commit=traffic_commit)
+    ensemble = None
+    if not a.no_ensemble:
+        ensemble = {p: run_ensemble(p, a.ensemble_attempts, a.ensemble_spinup) for p in ("mixed", "fp64")}
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(configs.WORKLOADS[a.workload](), q, t0, dt0, a.cpu_budget)
@@ -273,7 +402,8 @@ def main():
                    "scheme": "ARK43_QGLeith", "spinup_steps": a.spinup, "parallelism": "single GPU"},
         "mixed": mixed, "fp64": fp64,
         "mixed_vs_fp64": round(mixed["graph"]["days_per_s"] / fp64["graph"]["days_per_s"], 3),
-        "roofline": roof, "cpu_baseline": cpu,
+        "passes": passes, "projection_mfma": mfma,
+        "roofline": roof, "ensemble": ensemble, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 

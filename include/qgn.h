@@ -95,9 +95,10 @@ int qgn_batch_step(qgn_ctx* ctx, int attempts);
  * has that status. */
 int qgn_batch_get_time(qgn_ctx* ctx, double* t, double* dt, long long* steps, long long* rejects, int* status);
 
-/* Several batches (e.g. one per grid size) advanced by one graph launch per
- * attempt, the batches' attempt graphs as independent children that may run
- * concurrently (so no two batches may share a closure network).  Each
+/* Several batches (e.g. one per grid size) advanced together by graph
+ * launches (one per 8 attempts), the batches' attempt graphs as independent
+ * children that may run concurrently (so no two batches may share a
+ * closure network).  Each
  * context needs qgn_batch_set_time first; qgn_ensemble_step is ordered
  * after the work already queued on every context and before what is
  * queued on them afterwards, and returns without waiting. */
@@ -107,6 +108,8 @@ typedef struct qgn_ens qgn_ens;
  * was created (which keep their buffers until qgn_ensemble_destroy, even
  * after qgn_set_closure(ctx, NULL, ...)). */
 int qgn_ensemble_create(qgn_ens** ens, qgn_ctx* const* ctxs, int n);
+/* `attempts` attempts of every member: one launch of a graph of 8 chained
+ * attempts per 8, one single-attempt launch per remaining attempt. */
 int qgn_ensemble_step(qgn_ens* ens, int attempts);
 int qgn_ensemble_destroy(qgn_ens* ens);
 
@@ -184,13 +187,16 @@ int qgn_get_tuning(qgn_ctx* ctx, int* col_inv, int* col_rhs, int* row, float* ti
 int qgn_set_tuning(qgn_ctx* ctx, int col_inv, int col_rhs, int row);
 
 /* Kernel timing with HIP events on the context stream: after
- * qgn_prof_enable(ctx, id), every launch of kernel `id` is bracketed by
- * events; qgn_prof_read returns the launch count and summed milliseconds
- * since enable (it synchronises the stream).  id: QGN_K_* below. */
-enum { QGN_K_ROW_JAC = 0, QGN_K_COL_INV = 1, QGN_K_COL_RHS = 2, QGN_K_PROJECT = 3,
+ * qgn_prof_enable(ctx, id), every launch of kernel `id` (QGN_K_ALL: of every
+ * kernel below; -1: none) outside a captured graph is bracketed by events;
+ * qgn_prof_read returns the launch count and summed milliseconds since
+ * enable, qgn_prof_read_all the same per kernel id (arrays of QGN_K_COUNT);
+ * both synchronise the stream.  No counterpart in the reference. */
+enum { QGN_K_ALL = -2, QGN_K_ROW_JAC = 0, QGN_K_COL_INV = 1, QGN_K_COL_RHS = 2, QGN_K_PROJECT = 3,
        QGN_K_STAGE = 4, QGN_K_UPDATE = 5, QGN_K_ERRNORM = 6, QGN_K_COUNT = 7 };
 int qgn_prof_enable(qgn_ctx* ctx, int kernel_id);
 int qgn_prof_read(qgn_ctx* ctx, int* count, double* total_ms);
+int qgn_prof_read_all(qgn_ctx* ctx, int* counts, double* ms);
 
 /* ---- MKL DFTI replacement (MKL_DFTI_Setup.f90:1-52) --------------------
  * Batched 2-D real transforms with the reference's descriptor:

@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(_PKG, "libqgn.so")
 QGN_OK, QGN_EINVAL, QGN_EHIP, QGN_ENOMEM, QGN_ENONFINITE, QGN_EREJECT = 0, -1, -2, -3, -4, -5
 SCHEMES = {"AB3": 0, "RK4": 1, "BSRK32": 2, "ARK43": 3, "ARK43_QGLeith": 4}
 PRECISIONS = {"fp64": 0, "mixed": 1}
-KERNELS = {"row_jac": 0, "col_inv": 1, "col_rhs": 2, "project": 3, "stage": 4, "update": 5, "errnorm": 6}
+KERNELS = {"all": -2, "row_jac": 0, "col_inv": 1, "col_rhs": 2, "project": 3, "stage": 4, "update": 5, "errnorm": 6}
 
 # Every symbol include/qgn.h declares.
 EXPORTS = [
@@ -26,6 +26,7 @@ EXPORTS = [
     "qgn_cnn_create", "qgn_cnn_destroy", "qgn_cnn_nparams", "qgn_cnn_get_params", "qgn_cnn_set_params",
     "qgn_cnn_forward", "qgn_cnn_backward", "qgn_cnn_adam", "qgn_cnn_train_step", "qgn_cnn_last_error",
     "qgn_set_closure", "qgn_get_tuning", "qgn_set_tuning", "qgn_dft_compute_noalloc",
+    "qgn_prof_read_all",
 ]
 
 
@@ -87,6 +88,7 @@ def load():
         "qgn_dft_forward_device": (I, [P, P, P, P]),
         "qgn_dft_backward_device": (I, [P, P, P, P]),
         "qgn_dft_compute_noalloc": (I, [I, I, I, I, I, P, P, P]),
+        "qgn_prof_read_all": (I, [P, C.POINTER(C.c_int), D]),
         "qgn_dft_free": (I, [P]),
         "qgn_vertical_modes": (I, [I, C.c_double, D, D, D, D, D, D, D, D]),
         "qgn_create_batch": (I, [C.POINTER(P), C.POINTER(QgnParams), I, D, D, D, D, D, D]),

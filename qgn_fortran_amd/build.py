@@ -78,9 +78,26 @@ def build_host(verbose=False) -> str | None:
     return HOST_BIN
 
 
+def build_oracle(verbose=False) -> str | None:
+    """Compile the oracle's C++/OpenMP restatement (oracle/qgn_cpu.cpp, test
+    infrastructure: the CPU check and baseline, never loaded by the product)
+    into oracle/libqgn_cpu.so.  It loads MKL at run time."""
+    src = os.path.join(ROOT, "oracle", "qgn_cpu.cpp")
+    if not os.path.exists(src):
+        return None
+    out = os.path.join(ROOT, "oracle", "libqgn_cpu.so")
+    _run(["g++", "-O3", "-march=x86-64-v3", "-fopenmp", "-fPIC", "-shared", "-std=c++17", src, "-o", out + ".tmp",
+          "-ldl"])
+    os.replace(out + ".tmp", out)
+    if verbose:
+        print("built", out)
+    return out
+
+
 def build_all(verbose=False):
     build_lib(verbose)
     build_host(verbose)
+    build_oracle(verbose)
 
 
 if __name__ == "__main__":

@@ -308,11 +308,12 @@ struct qgn_ctx {
   double err0 = 0.0, last_err = 0.0;
   qgn_stats st = {};
   // profiling
-  int prof_id = -1;
+  int prof_id = -1;       // a QGN_K_* id, QGN_K_ALL, or -1 (off)
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+  std::vector<int> ev_kid;  // kernel id of each recorded pair
   int ev_used = 0;
-  double prof_ms = 0.0;
-  int prof_count = 0;
+  double prof_ms[QGN_K_COUNT] = {};
+  int prof_count[QGN_K_COUNT] = {};
 };
 
 // Several batches (e.g. one per grid size) advanced together: one graph
@@ -323,37 +324,48 @@ struct qgn_ens {
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
   std::vector<hipEvent_t> ready;
-  hipGraph_t graph = nullptr;
+  hipGraph_t graph = nullptr;         // one attempt of every batch
   hipGraphExec_t exec = nullptr;
+  hipGraph_t chain = nullptr;         // kChain attempts of every batch, one launch
+  hipGraphExec_t chain_exec = nullptr;
 };
+// attempts per launch of an ensemble's chained graph
+constexpr int kChain = 8;
 
 namespace {
 
+// fold the recorded event pairs into the per-kernel sums (synchronises)
+int prof_fold(qgn_ctx* c) {
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int i = 0; i < c->ev_used; ++i) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev[i].first, c->ev[i].second));
+    c->prof_ms[c->ev_kid[i]] += ms;
+    c->prof_count[c->ev_kid[i]]++;
+  }
+  c->ev_used = 0;
+  return 0;
+}
+
 template <typename F>
 int timed(qgn_ctx* c, int id, F&& launch) {
-  bool on = (c->prof_id == id);
+  bool on = (c->prof_id == id) || (c->prof_id == QGN_K_ALL && id >= 0);
   hipEvent_t a = nullptr, b = nullptr;
   if (on) {
     if (c->ev_used == (int)c->ev.size()) {
       if (c->ev.size() >= 4096) {
-        // fold what has been recorded so far
-        HIPCHK(hipStreamSynchronize(c->stream));
-        for (int i = 0; i < c->ev_used; ++i) {
-          float ms = 0.f;
-          HIPCHK(hipEventElapsedTime(&ms, c->ev[i].first, c->ev[i].second));
-          c->prof_ms += ms;
-          c->prof_count++;
-        }
-        c->ev_used = 0;
+        CHK(prof_fold(c));   // fold what has been recorded so far
       } else {
         hipEvent_t x, y;
         HIPCHK(hipEventCreate(&x));
         HIPCHK(hipEventCreate(&y));
         c->ev.push_back({x, y});
+        c->ev_kid.push_back(0);
       }
     }
     a = c->ev[c->ev_used].first;
     b = c->ev[c->ev_used].second;
+    c->ev_kid[c->ev_used] = id;
     HIPCHK(hipEventRecord(a, c->stream));
   }
   launch();
@@ -476,7 +488,7 @@ bool rhs_size_ok(int nx, int ny) {
 
 // P1: y-inverse FFT of up to 4 compact fields into mixed arrays
 template <typename R>
-int launch_col_inv(qgn_ctx* c, const ColInvArgs& a) {
+int launch_col_inv(qgn_ctx* c, const ColInvArgs& a, int kernel_id = QGN_K_COL_INV) {
   const Geom& g = c->g;
   const Tw<R>& tw = tw_of<R>(c);
   int rc = 0;
@@ -492,7 +504,7 @@ int launch_col_inv(qgn_ctx* c, const ColInvArgs& a) {
         rc = allow_lds(kern, T::LDS);
         if (rc) return;
         dim3 grid(g.Kxp / C * g.nz * a.nf);
-        rc = timed(c, QGN_K_COL_INV, [&] {
+        rc = timed(c, kernel_id, [&] {
           hipLaunchKernelGGL(kern, grid, dim3(T::NTHR), T::LDS, c->stream, a, g, tw.sy);
         });
       }
@@ -772,7 +784,7 @@ int errnorm_launch(qgn_ctx* c, const double2* Ecmp) {
   ColInvArgs ca{};
   ca.nf = 1;
   ca.src[0] = Ecmp; ca.src_is_double[0] = 1; ca.mult[0] = 0; ca.dst[0] = c->Em;
-  CHK(launch_col_inv<double>(c, ca));
+  CHK(launch_col_inv<double>(c, ca, QGN_K_ERRNORM));
   {
     int rc = 0;
     bool hit = with_size(g.N2, [&](auto nc) {
@@ -1854,27 +1866,36 @@ int qgn_get_stream(qgn_ctx* c, void** stream) {
 }
 
 int qgn_prof_enable(qgn_ctx* c, int id) {
-  if (!c || id < -1 || id >= QGN_K_COUNT) return fail(QGN_EINVAL, "bad kernel id");
+  if (!c || id < QGN_K_ALL || id >= QGN_K_COUNT) return fail(QGN_EINVAL, "bad kernel id");
   HIPCHK(hipStreamSynchronize(c->stream));
   c->prof_id = id;
   c->ev_used = 0;
-  c->prof_ms = 0.0;
-  c->prof_count = 0;
+  for (int k = 0; k < QGN_K_COUNT; ++k) {
+    c->prof_ms[k] = 0.0;
+    c->prof_count[k] = 0;
+  }
   return 0;
 }
 
 int qgn_prof_read(qgn_ctx* c, int* count, double* total_ms) {
   if (!c || !count || !total_ms) return fail(QGN_EINVAL, "null argument");
-  HIPCHK(hipStreamSynchronize(c->stream));
-  for (int i = 0; i < c->ev_used; ++i) {
-    float ms = 0.f;
-    HIPCHK(hipEventElapsedTime(&ms, c->ev[i].first, c->ev[i].second));
-    c->prof_ms += ms;
-    c->prof_count++;
+  CHK(prof_fold(c));
+  *count = 0;
+  *total_ms = 0.0;
+  for (int k = 0; k < QGN_K_COUNT; ++k) {
+    *count += c->prof_count[k];
+    *total_ms += c->prof_ms[k];
   }
-  c->ev_used = 0;
-  *count = c->prof_count;
-  *total_ms = c->prof_ms;
+  return 0;
+}
+
+int qgn_prof_read_all(qgn_ctx* c, int* counts, double* ms) {
+  if (!c || !counts || !ms) return fail(QGN_EINVAL, "null argument");
+  CHK(prof_fold(c));
+  for (int k = 0; k < QGN_K_COUNT; ++k) {
+    counts[k] = c->prof_count[k];
+    ms[k] = c->prof_ms[k];
+  }
   return 0;
 }
 
@@ -2105,6 +2126,8 @@ int qgn_ensemble_destroy(qgn_ens* e) {
   for (qgn_cnn* n : e->nets) qcnn::hold(n, -1);
   if (e->exec) (void)hipGraphExecDestroy(e->exec);
   if (e->graph) (void)hipGraphDestroy(e->graph);
+  if (e->chain_exec) (void)hipGraphExecDestroy(e->chain_exec);
+  if (e->chain) (void)hipGraphDestroy(e->chain);
   for (hipEvent_t x : e->ready) (void)hipEventDestroy(x);
   if (e->done) (void)hipEventDestroy(e->done);
   if (e->stream) (void)hipStreamDestroy(e->stream);
@@ -2112,8 +2135,12 @@ int qgn_ensemble_destroy(qgn_ens* e) {
   return 0;
 }
 
+// Two graphs: one attempt of every batch (its attempt graph as a child),
+// and kChain attempts of every batch, each batch a chain of kChain children
+// that depend on one another and on nothing of the other batches.
 static int ensemble_create_impl(qgn_ens* e, qgn_ctx* const* ctxs, int n) {
   HIPCHK(hipGraphCreate(&e->graph, 0));
+  HIPCHK(hipGraphCreate(&e->chain, 0));
   for (int i = 0; i < n; ++i) {
     if (!ctxs[i]) return fail(QGN_EINVAL, "null context");
     for (int k = 0; k < i; ++k) {
@@ -2131,11 +2158,18 @@ static int ensemble_create_impl(qgn_ens* e, qgn_ctx* const* ctxs, int n) {
     }
     hipGraphNode_t node;
     HIPCHK(hipGraphAddChildGraphNode(&node, e->graph, nullptr, 0, ctxs[i]->graph));
+    hipGraphNode_t prev = nullptr;
+    for (int k = 0; k < kChain; ++k) {
+      hipGraphNode_t nk;
+      HIPCHK(hipGraphAddChildGraphNode(&nk, e->chain, prev ? &prev : nullptr, prev ? 1 : 0, ctxs[i]->graph));
+      prev = nk;
+    }
     hipEvent_t ev;
     HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
     e->ready.push_back(ev);
   }
   HIPCHK(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
+  HIPCHK(hipGraphInstantiate(&e->chain_exec, e->chain, nullptr, nullptr, 0));
   HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
   HIPCHK(hipEventCreateWithFlags(&e->done, hipEventDisableTiming));
   return 0;
@@ -2164,7 +2198,8 @@ int qgn_ensemble_step(qgn_ens* e, int attempts) {
     HIPCHK(hipEventRecord(e->ready[i], e->ctx[i]->stream));
     HIPCHK(hipStreamWaitEvent(e->stream, e->ready[i], 0));
   }
-  for (int a = 0; a < attempts; ++a) HIPCHK(hipGraphLaunch(e->exec, e->stream));
+  for (int a = 0; a < attempts / kChain; ++a) HIPCHK(hipGraphLaunch(e->chain_exec, e->stream));
+  for (int a = 0; a < attempts % kChain; ++a) HIPCHK(hipGraphLaunch(e->exec, e->stream));
   HIPCHK(hipEventRecord(e->done, e->stream));
   for (qgn_ctx* c : e->ctx) {
     HIPCHK(hipStreamWaitEvent(c->stream, e->done, 0));

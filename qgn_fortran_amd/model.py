@@ -231,6 +231,14 @@ class QGN:
         check(self.lib.qgn_prof_read(self._ctx, C.byref(n), C.byref(ms)))
         return n.value, ms.value
 
+    def prof_read_all(self):
+        """{kernel name: (launches, total ms)} since prof_enable("all")."""
+        k = len(_lib.KERNELS) - 1
+        n = (C.c_int * k)()
+        ms = np.zeros(k)
+        check(self.lib.qgn_prof_read_all(self._ctx, n, dptr(ms)))
+        return {name: (n[i], float(ms[i])) for name, i in _lib.KERNELS.items() if i >= 0}
+
     def close(self):
         """Cleanup (QGN_Module.f90:448-458); raises (and keeps the context)
         while an ensemble holds it."""

@@ -19,8 +19,10 @@ def _single(p, m):
     return QGN(dataclasses.replace(p, r0=m["r0"]), m["H"], m["Modes"], m["EVals"], m["uBar"], m["qyBar"])
 
 
-def _check_members(batch, group, q0_hat, which, tol_state, tol_t):
-    """Run members `which` alone for as many accepted steps as the batch took."""
+def _check_members(batch, group, q0_hat, which, tol_state, tol_t, same_tiles=False):
+    """Run members `which` alone for as many accepted steps as the batch took
+    (same_tiles: with the batch's FFT tile choices, which on the mixed path
+    decide the FP32 rounding)."""
     info = batch.times()
     state = batch.get_state()
     nz = group.params.nz
@@ -28,6 +30,9 @@ def _check_members(batch, group, q0_hat, which, tol_state, tol_t):
         steps = int(info["steps"][m])
         assert info["status"][m] == 0
         single = _single(group.params, group.members[m])
+        if same_tiles:
+            tb = batch.tuning()
+            single.set_tuning(tb["col_inv"], tb["col_rhs"], tb["row"])
         single.set_state(q0_hat[m * nz:(m + 1) * nz])
         t, dt = 0.0, group.dt0
         for _ in range(steps):
@@ -123,12 +128,36 @@ def test_time_step_then_batch_step_on_one_member():
     fresh.close()
 
 
-def test_ensemble96():
+def _oracle_replay(group, m, q0_hat, dts):
+    """Member m of a group through the oracle's ARK43_QGLeith
+    (ARK43_QGLeith.f90) with the accepted step sizes `dts` (an unreachable
+    tolerance, so every step is accepted)."""
+    from oracle import qgn_oracle as orc
+    p, mm = group.params, group.members[m]
+    po = orc.Params(nx=p.nx, ny=p.ny, nz=p.nz, Lx=p.Lx, Ly=p.Ly, Htot=p.Htot, beta=p.beta, r0=mm["r0"],
+                    C_d=p.C_d, TOL=1e30)
+    om = orc.Model(po, mm["H"], mm["Modes"], mm["EVals"], mm["uBar"], mm["qyBar"])
+    st = orc.ARK43(om, leith=True)
+    nz = p.nz
+    q, t = q0_hat[m * nz:(m + 1) * nz].copy(), 0.0
+    for dt in dts:
+        q, t, _ = st.step(q, t, dt)
+    return q, t
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_ensemble96(precision):
     """The 96-member ensemble (32 members of 8 layers on each of 128^2,
-    256^2, 384^2): all members advance by one graph launch per attempt; one
-    member of each size is checked against its single run."""
+    256^2, 384^2; mixed batches pick their FFT tiles with the autotuner at
+    creation).  The first 3 attempts run one at a time: member 7 of every
+    size is then replayed through the oracle with the step sizes it took
+    (state <= 1e-10 FP64, <= 1e-5 mixed).  Then 20 attempts run as chained
+    launches (8 attempts per graph launch, the rest single) and member 7
+    of each size is checked against its own single run (mixed: with the
+    batch's FFT tiles, so with the same FP32 rounding)."""
     from qgn_fortran_amd import Batch, Ensemble, configs
-    groups = configs.ensemble96()
+    print("ensemble96", precision, flush=True)
+    groups = configs.ensemble96(precision=precision)
     batches, q0s = [], []
     for g in groups:
         b = Batch(g.params, g.members)
@@ -136,9 +165,22 @@ def test_ensemble96():
         b.set_time(0.0, g.dt0)
         batches.append(b)
     ens = Ensemble(batches)
-    ens.step(2)                                  # warm-up
-    for b in batches:
-        b.times()
+    m = 7
+    dts = [[] for _ in groups]
+    t_prev = [0.0 for _ in groups]
+    for _ in range(3):
+        ens.step(1)
+        for k, b in enumerate(batches):
+            t = float(b.times()["t"][m])
+            if t != t_prev[k]:
+                dts[k].append(t - t_prev[k])
+                t_prev[k] = t
+    tol = 1e-10 if precision == "fp64" else 1e-5
+    for k, (g, b) in enumerate(zip(groups, batches)):
+        q, t = _oracle_replay(g, m, q0s[k], dts[k])
+        err = rel_l2(b.get_state()[m * g.params.nz:(m + 1) * g.params.nz], q)
+        print(f"  {g.params.nx}^2 member {m}: {len(dts[k])} steps vs oracle, state rel L2 {err:.2e}", flush=True)
+        assert len(dts[k]) >= 1 and err < tol and abs(t - t_prev[k]) <= 1e-12 * t
     t0 = time.perf_counter()
     attempts = 20
     ens.step(attempts)
@@ -146,11 +188,14 @@ def test_ensemble96():
     wall = time.perf_counter() - t0
     steps = sum(int(i["steps"].sum()) for i in infos)
     days = sum(float(i["t"].sum()) for i in infos) / 86400.0
-    print(f"ensemble96: {attempts + 2} attempts, {steps} accepted member-steps, "
+    print(f"ensemble96 {precision}: {attempts + 3} attempts, {steps} accepted member-steps, "
           f"{attempts} attempts in {wall * 1e3:.1f} ms ({attempts / wall:.1f} attempts/s)", flush=True)
     for g, b, q0, i in zip(groups, batches, q0s, infos):
         assert np.all(i["status"] == 0) and np.all(i["steps"] >= 1)
-        _check_members(b, g, q0, [7], 1e-10, 1e-12)
+        if precision == "fp64":
+            _check_members(b, g, q0, [m], 1e-10, 1e-12)
+        else:
+            _check_members(b, g, q0, [m], 1e-12, 1e-12, same_tiles=True)
     assert days > 0
     ens.close()
     for b in batches:
