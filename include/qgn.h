@@ -57,7 +57,12 @@ typedef struct qgn_params {
 typedef struct qgn_ctx qgn_ctx;
 
 /* Module set-up.  Replaces the vertical-structure and plan parts of
- * Initialize (QGN_Module.f90:53-175): H (nz, already rescaled to Htot),
+ * Initialize (QGN_Module.f90:53-175).  Sizes: nx even, nx/2 and ny
+ * products of 2, 3 and 5, nx <= 2048, ny <= 4096.  nx/2 and ny in {32, 48,
+ * 64, 96, 128, 192, 256, 384, 512, 768, 1024, 1536, 2048} with nx, ny <=
+ * 2048 run the compiled column and row passes; every other size runs a
+ * generic path of run-time-plan transforms (the same results, more HBM
+ * traffic, no tuned tiles and no closure network).  H (nz, already rescaled to Htot),
  * Modes (nz x nz, column-major, Modes(:,k) = mode k), EVals (nz), uBar (nz),
  * qyBar (nz) are the module variables of QGN_Module.f90:22-26. */
 int qgn_create(qgn_ctx** ctx, const qgn_params* p, const double* H, const double* Modes,

@@ -15,6 +15,7 @@
 
 #include "cnn.hpp"
 #include "kernels_ct.hpp"
+#include "kernels_gen.hpp"
 #include "qgn.h"
 
 using namespace qgn;
@@ -292,6 +293,14 @@ struct qgn_ctx {
   // autotune() at creation or set by qgn_set_tuning; the measured times (us)
   int tune_col_inv = 0, tune_col_rhs = 0, tune_row = 0;
   float tune_ms[3][4] = {};
+  // sizes outside the compiled transform lengths: the generic RHS path's
+  // full spectra (2 work arrays, J), physical u, v, q_x, q_y and the bottom
+  // layers' drag products with their spectra (all cplx<R> / R)
+  bool ct = true;
+  void* gfull[3] = {};
+  void* gphys[4] = {};
+  void* gdrag[2] = {};
+  void* gdragh[2] = {};
   // device-controlled ARK43 attempts of a batch: per-member controller state
   double *bt = nullptr, *bdt = nullptr, *berr0 = nullptr, *blast = nullptr;
   long long* bcount = nullptr;  // steps, attempts, rejects per member (3 nm)
@@ -664,6 +673,68 @@ int launch_col_rhs(qgn_ctx* c, const ColRhsArgs& ka0, bool drag) {
 template <typename R>
 int closure_forcing(qgn_ctx* c);
 
+template <typename R>
+int dft_fwd(hipStream_t s, const Geom& g, const FftPlan& px, const FftPlan& py, const Tw<R>& tw, int howmany,
+            const R* grid, cplx<R>* spec, cplx<R>* tmp);
+template <typename R>
+int dft_bwd(hipStream_t s, const Geom& g, const FftPlan& px, const FftPlan& py, const Tw<R>& tw, int howmany,
+            const cplx<R>* spec, R* grid, cplx<R>* tmp, R scale);
+
+// GetRHS of the generic path (kernels_gen.hpp): psi (and, on the mixed
+// path, the FP32 copy of the input) as rhs_impl forms them, then each of
+// u, v, q_x, q_y expanded to the full spectrum and transformed by the
+// run-time-plan c2r, the products in physical space, their r2c, and the
+// spectral terms on the kept modes.
+template <typename R>
+int rhs_generic(qgn_ctx* c, const void* psi, const void* qsrc, bool qsrc_double, void* out) {
+  const Geom& g = c->g;
+  const Tw<R>& tw = tw_of<R>(c);
+  hipStream_t s = c->stream;
+  const R nn = (R)(1.0 / ((double)g.nx * (double)g.ny));
+  const long nf = (long)g.nz * g.ny * g.nxh;
+  struct { const void* src; bool isd; int mult; } f[4] = {
+      {psi, std::is_same<R, double>::value, 1},   // u = -i ky psi
+      {psi, std::is_same<R, double>::value, 2},   // v = i kx psi
+      {qsrc, qsrc_double, 2},                      // q_x
+      {qsrc, qsrc_double, 3}};                     // q_y
+  cplx<R>* w0 = static_cast<cplx<R>*>(c->gfull[0]);
+  cplx<R>* w1 = static_cast<cplx<R>*>(c->gfull[1]);
+  R* ph[4];
+  for (int k = 0; k < 4; ++k) ph[k] = static_cast<R*>(c->gphys[k]);
+  for (int k = 0; k < 4; ++k) {
+    hipLaunchKernelGGL((k_gen_expand<R>), dim3(grid1d(nf)), dim3(256), 0, s, f[k].src, (int)f[k].isd, f[k].mult, w0, g);
+    HIPCHK(hipGetLastError());
+    CHK(dft_bwd<R>(s, g, c->px, c->py, tw, g.nz, w0, ph[k], w1, nn));
+  }
+  const bool drag = c->p.C_d != 0.0;
+  hipLaunchKernelGGL((k_gen_products<R>), dim3(grid1d((long)g.nz * g.ny * g.nx)), dim3(256), 0, s, ph[0], ph[1],
+                     ph[2], ph[3], static_cast<R*>(c->gdrag[0]), static_cast<R*>(c->gdrag[1]), (int)drag, g);
+  HIPCHK(hipGetLastError());
+  cplx<R>* jh = static_cast<cplx<R>*>(c->gfull[2]);
+  CHK(dft_fwd<R>(s, g, c->px, c->py, tw, g.nz, ph[2], jh, w1));
+  if (drag) {
+    CHK(dft_fwd<R>(s, g, c->px, c->py, tw, c->nm, static_cast<R*>(c->gdrag[0]), static_cast<cplx<R>*>(c->gdragh[0]),
+                   w1));
+    CHK(dft_fwd<R>(s, g, c->px, c->py, tw, c->nm, static_cast<R*>(c->gdrag[1]), static_cast<cplx<R>*>(c->gdragh[1]),
+                   w1));
+  }
+  ColRhsArgs ka{};
+  ka.q = qsrc_double ? static_cast<const double2*>(qsrc) : nullptr;
+  ka.q32 = qsrc_double ? nullptr : qsrc;
+  ka.psi = psi; ka.out = out;
+  ka.ubar = c->ubar; ka.qybar = c->qybar;
+  ka.beta = c->p.beta;
+  ka.ekman = c->ekm;
+  ka.A2 = c->p.A2; ka.A8 = c->p.A8;
+  ka.explicit_visc = c->p.is_explicit;
+  ka.cdrag = c->cdm;
+  ka.drag = drag;
+  hipLaunchKernelGGL((k_gen_combine<R>), dim3(grid1d(c->ncmp)), dim3(256), 0, s, ka, jh,
+                     static_cast<const cplx<R>*>(c->gdragh[0]), static_cast<const cplx<R>*>(c->gdragh[1]), g);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
 // GetRHS on compact arrays: out = RHS(qin), with the closure network's
 // forcing when one is attached
 template <typename R>
@@ -686,6 +757,11 @@ int rhs_impl(qgn_ctx* c, const double2* qin, void* out, bool psi_ready) {
     } else {
       CHK(launch_project<R>(c, qin, psi));
     }
+  }
+  if (!c->ct) {
+    CHK(rhs_generic<R>(c, psi, q32 ? q32 : (const void*)qin, q32 == nullptr, out));
+    c->st.rhs_evals++;
+    return 0;
   }
   ColInvArgs ca{};
   ca.nf = 4;
@@ -781,6 +857,20 @@ int rhs(qgn_ctx* c, const double2* qin, void* out, bool psi_ready = false) {
 // into c->errslot[m] as ordered bit patterns (device only)
 int errnorm_launch(qgn_ctx* c, const double2* Ecmp) {
   const Geom& g = c->g;
+  if (!c->ct) {   // the generic path: full spectrum, run-time-plan c2r, per-member max
+    const long nf = (long)g.nz * g.ny * g.nxh;
+    hipLaunchKernelGGL((k_gen_expand<double>), dim3(grid1d(nf)), dim3(256), 0, c->stream, (const void*)Ecmp, 1, 0,
+                       c->fa, g);
+    HIPCHK(hipGetLastError());
+    CHK(dft_bwd<double>(c->stream, g, c->px, c->py, c->twd, g.nz, c->fa, c->phys, c->fb, 1.0));
+    const long per = (long)g.nzm * g.ny * g.nx;
+    const int nblk = (int)std::min<long>(std::min<long>((per + 255) / 256, 256), (long)g.ny * g.nzm);
+    hipLaunchKernelGGL(k_gen_maxabs, dim3(nblk, c->nm), dim3(256), 0, c->stream, c->phys, per, c->errpart);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_max_final, dim3(c->nm), dim3(1024), 0, c->stream, c->errpart, nblk, c->errslot);
+    HIPCHK(hipGetLastError());
+    return 0;
+  }
   ColInvArgs ca{};
   ca.nf = 1;
   ca.src[0] = Ecmp; ca.src_is_double[0] = 1; ca.mult[0] = 0; ca.dst[0] = c->Em;
@@ -1278,6 +1368,14 @@ void destroy_ctx(qgn_ctx* c) {
   void* cl[] = {c->cl_in, c->cl_out, c->cl_mq, c->cl_mp, c->cl_x, c->cl_y};
   for (void* x : cl)
     if (x) (void)hipFree(x);
+  for (void* x : c->gfull)
+    if (x) (void)hipFree(x);
+  for (void* x : c->gphys)
+    if (x) (void)hipFree(x);
+  for (int k = 0; k < 2; ++k) {
+    if (c->gdrag[k]) (void)hipFree(c->gdrag[k]);
+    if (c->gdragh[k]) (void)hipFree(c->gdragh[k]);
+  }
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   if (c->graph) (void)hipGraphDestroy(c->graph);
   void* bp[] = {c->ekm, c->cdm, c->bt, c->bdt, c->berr0, c->blast, c->bcount, c->bint, c->brep};
@@ -1308,7 +1406,7 @@ int upload(double** dst, const double* src, size_t n) {
 // on the data.  QGN_AUTOTUNE=0 in the environment keeps the fixed rules.
 int autotune(qgn_ctx* c) {
   c->tune_col_inv = c->tune_col_rhs = c->tune_row = 0;
-  if (c->p.precision != QGN_MIXED) return 0;
+  if (c->p.precision != QGN_MIXED || !c->ct) return 0;
   const char* env = std::getenv("QGN_AUTOTUNE");
   if (env && env[0] == '0') return 0;
   const Geom& g = c->g;
@@ -1394,9 +1492,9 @@ int create_impl(qgn_ctx* c, const qgn_params* p, int members, const double* H, c
   if (members < 1 || members > 4096) return fail(QGN_EINVAL, "members must be in [1, 4096]");
   std::string why;
   if (!size_ok(p->nx, p->ny, &why)) return fail(QGN_EINVAL, why);
-  if (!rhs_size_ok(p->nx, p->ny))
-    return fail(QGN_EINVAL, "the RHS supports nx/2 and ny in {32,48,64,96,128,192,256,384,512,768,1024,1536,2048} "
-                            "with nx, ny <= 2048");
+  // compiled column and row passes for nx/2 and ny in {32, 48, ..., 2048};
+  // every other size takes the generic path (kernels_gen.hpp)
+  c->ct = rhs_size_ok(p->nx, p->ny);
   if (p->nz < 2 || p->nz > 64) return fail(QGN_EINVAL, "nz must be in [2, 64]");
   if (p->scheme < QGN_AB3 || p->scheme > QGN_ARK43_QGLEITH) return fail(QGN_EINVAL, "unknown scheme");
   if (p->precision != QGN_FP64 && p->precision != QGN_MIXED) return fail(QGN_EINVAL, "unknown precision");
@@ -1488,6 +1586,15 @@ int create_impl(qgn_ctx* c, const qgn_params* p, int members, const double* H, c
   HIPCHK(hipHostMalloc((void**)&c->h_rep, 8 * sizeof(double), hipHostMallocDefault));
   CHK(alloc((void**)&c->errpart, (size_t)g.ny * nz * sizeof(unsigned long long)));
   HIPCHK(hipHostMalloc((void**)&c->h_errslot, sizeof(unsigned long long), hipHostMallocDefault));
+  if (!c->ct) {
+    const size_t re = rs / 2;   // one real of the RHS type
+    for (int k = 0; k < 3; ++k) CHK(alloc(&c->gfull[k], (size_t)nz * g.ny * g.nxh * rs));
+    for (int k = 0; k < 4; ++k) CHK(alloc(&c->gphys[k], (size_t)nz * g.ny * g.nx * re));
+    for (int k = 0; k < 2; ++k) {
+      CHK(alloc(&c->gdrag[k], (size_t)members * g.ny * g.nx * re));
+      CHK(alloc(&c->gdragh[k], (size_t)members * g.ny * g.nxh * rs));
+    }
+  }
   CHK(autotune(c));
   const size_t fbytes = (size_t)nz * g.ny * g.nxh * sizeof(double2);
   CHK(alloc((void**)&c->fa, fbytes));
@@ -1712,6 +1819,9 @@ int qgn_set_closure(qgn_ctx* c, qgn_cnn* net, const double* in_scale, const doub
   if (cin != 2 * g.nzm || cout != g.nzm)
     return fail(QGN_EINVAL, "closure network needs 2*nz input and nz output channels");
   const long pix = (long)c->nm * g.ny * g.nx;
+  if (!c->ct)
+    return fail(QGN_EINVAL, "closure network: the row passes of the closure exist for nx/2 in {32, 48, 64, 96, "
+                            "128, 192, 256, 384, 512, 768, 1024} only");
   if (g.nx % 16 || pix % 64) return fail(QGN_EINVAL, "closure network needs nx % 16 == 0");
   if (c->p.precision != QGN_MIXED && g.nx > 1024)
     return fail(QGN_EINVAL, "closure network: the FP64 path supports nx <= 1024 (the mixed path nx <= 2048)");
@@ -1798,6 +1908,7 @@ int qgn_set_tuning(qgn_ctx* c, int col_inv, int col_rhs, int row) {
   if (!c) return fail(QGN_EINVAL, "null argument");
   const Geom& g = c->g;
   if (c->p.precision != QGN_MIXED) return fail(QGN_EINVAL, "the tuned passes are those of the mixed path");
+  if (!c->ct) return fail(QGN_EINVAL, "this grid size takes the generic path, which has no tuned passes");
   if ((col_inv && !col_tile_ok(col_inv, g.ny, sizeof(float2))) || (col_rhs && !col_tile_ok(col_rhs, g.ny, sizeof(float2))))
     return fail(QGN_EINVAL, "column tile must be 2, 4, 8 or 16 and fit 1024 threads and 160 KB of LDS");
   if (row && (row != 2 && row != 4 && row != 8)) return fail(QGN_EINVAL, "row group must be 2, 4 or 8");

@@ -47,6 +47,10 @@ RHS_CASES = [
     dict(nx=192, ny=128, nz=3, C_d=1e-5, Ly=0.7e6),
     dict(nx=384, ny=384, nz=8, C_d=1e-5, k_peak=16),
     dict(nx=256, ny=256, nz=16, C_d=0.0, k_peak=16),
+    # sizes outside the compiled transform lengths: the generic path
+    dict(nx=576, ny=576, nz=4, C_d=1e-5, k_peak=12),
+    dict(nx=320, ny=320, nz=3, C_d=1e-5),
+    dict(nx=160, ny=90, nz=2, profile="two", A8=1e21, is_explicit=True, C_d=1e-5, Ly=0.6e6),
 ]
 
 
@@ -155,6 +159,9 @@ STEP_CASES = [
     ("ARK43", dict(nx=96, ny=96, nz=3, C_d=1e-5, A8=1e22, TOL=1e-11, rms=1e-5), 8000.0, 6),
     ("ARK43_QGLeith", dict(nx=96, ny=96, nz=4, C_d=1e-5, TOL=1e-11, rms=1e-5), 8000.0, 6),
     ("ARK43_QGLeith", dict(nx=128, ny=128, nz=4, C_d=1e-5, TOL=1e-12, rms=1e-5), 20000.0, 6),
+    # generic-path sizes (576 = 3 x 192, as parameters.f90:11 recommends; 320 = 2^6 x 5)
+    ("ARK43_QGLeith", dict(nx=576, ny=576, nz=4, C_d=1e-5, TOL=1e-11, rms=1e-5), 8000.0, 4),
+    ("BSRK32", dict(nx=320, ny=320, nz=3, C_d=1e-5, TOL=1e-11, rms=1e-5), 3000.0, 4),
 ]
 
 
@@ -201,6 +208,22 @@ def test_mixed_precision_rhs(n, nz):
     out = gm.get_rhs(q_hat)
     err = rel_l2(out, ref, axis=(1, 2))
     print("mixed per-layer rel L2", err)
+    assert np.all(err < 1e-5)
+    gm.close()
+
+
+@pytest.mark.parametrize("nx,ny,nz", [(576, 576, 4), (320, 320, 3), (1152, 1152, 4)])
+def test_generic_size_mixed_rhs(nx, ny, nz):
+    """The generic path in FP32 (run-time-plan transforms, MFMA projection):
+    RHS relative L2 per layer vs the FP64 oracle <= 1e-5."""
+    case = make_case(nx, ny, nz, k_peak=16, C_d=1e-5)
+    om = _oracle_model(case)
+    q_hat = om.initialize(case["q0"])
+    ref = om.get_rhs(q_hat)
+    case["pp"].precision = "mixed"
+    gm = _gpu_model(case)
+    err = rel_l2(gm.get_rhs(q_hat), ref, axis=(1, 2))
+    print(f"generic {nx}x{ny}x{nz} mixed per-layer rel L2 max {err.max():.2e}", flush=True)
     assert np.all(err < 1e-5)
     gm.close()
 
