@@ -222,6 +222,32 @@ int qgn_dft_forward_device(qgn_dft* h, const void* grid, void* spec, void* strea
 int qgn_dft_backward_device(qgn_dft* h, const void* spec, void* grid, void* stream);
 int qgn_dft_free(qgn_dft* h);
 
+/* The same transforms behind DFTI's descriptor calls, in the order
+ * MKL_DFTI_Setup.f90:11-50 makes them (create, set values, commit) and as
+ * QGN_Module.f90 computes and frees them (DftiComputeForward at :180,
+ * :238-241, :287-290; DftiComputeBackward at :228-234; DftiFreeDescriptor
+ * at :452-455).  Lengths are in Fortran order {nx, ny}; strides are
+ * {offset, x stride, y stride} as the reference sets them.  Commit accepts
+ * the descriptions of real(nx,ny,howmany) <-> complex(nx/2+1,ny,howmany)
+ * arrays, x fastest, out of place, unnormalised, and returns QGN_EINVAL
+ * with a message for anything else; a compute call checks that the strides
+ * fit its direction.  Host arrays, DFTI_DOUBLE data. */
+typedef struct qgn_dfti qgn_dfti;
+enum { QGN_DFTI_DOUBLE = 1, QGN_DFTI_REAL = 2, QGN_DFTI_COMPLEX_COMPLEX = 3, QGN_DFTI_CCE_FORMAT = 4,
+       QGN_DFTI_NOT_INPLACE = 5, QGN_DFTI_INPLACE = 6 };
+enum { QGN_DFTI_NUMBER_OF_TRANSFORMS = 1, QGN_DFTI_INPUT_DISTANCE = 2, QGN_DFTI_OUTPUT_DISTANCE = 3,
+       QGN_DFTI_CONJUGATE_EVEN_STORAGE = 4, QGN_DFTI_PACKED_FORMAT = 5, QGN_DFTI_PLACEMENT = 6,
+       QGN_DFTI_INPUT_STRIDES = 7, QGN_DFTI_OUTPUT_STRIDES = 8, QGN_DFTI_FORWARD_SCALE = 9,
+       QGN_DFTI_BACKWARD_SCALE = 10 };
+int qgn_dfti_create_descriptor(qgn_dfti** h, int precision, int domain, int dimension, const long long* lengths);
+int qgn_dfti_set_value_int(qgn_dfti* h, int param, long long value);
+int qgn_dfti_set_value_ints(qgn_dfti* h, int param, const long long* values, int n);
+int qgn_dfti_set_value_double(qgn_dfti* h, int param, double value);
+int qgn_dfti_commit_descriptor(qgn_dfti* h);
+int qgn_dfti_compute_forward(qgn_dfti* h, const double* grid, double* spec);
+int qgn_dfti_compute_backward(qgn_dfti* h, const double* spec, double* grid);
+int qgn_dfti_free_descriptor(qgn_dfti** h);
+
 /* The transforms of qgn_dft_forward_device / qgn_dft_backward_device
  * (forward != 0: r2c, DftiComputeForward; else c2r, DftiComputeBackward)
  * without a handle, allocating nothing: for callers that cannot allocate

@@ -26,7 +26,9 @@ EXPORTS = [
     "qgn_cnn_create", "qgn_cnn_destroy", "qgn_cnn_nparams", "qgn_cnn_get_params", "qgn_cnn_set_params",
     "qgn_cnn_forward", "qgn_cnn_backward", "qgn_cnn_adam", "qgn_cnn_train_step", "qgn_cnn_last_error",
     "qgn_set_closure", "qgn_get_tuning", "qgn_set_tuning", "qgn_dft_compute_noalloc",
-    "qgn_prof_read_all",
+    "qgn_prof_read_all", "qgn_dfti_create_descriptor", "qgn_dfti_set_value_int", "qgn_dfti_set_value_ints",
+    "qgn_dfti_set_value_double", "qgn_dfti_commit_descriptor", "qgn_dfti_compute_forward",
+    "qgn_dfti_compute_backward", "qgn_dfti_free_descriptor",
 ]
 
 
@@ -89,6 +91,14 @@ def load():
         "qgn_dft_backward_device": (I, [P, P, P, P]),
         "qgn_dft_compute_noalloc": (I, [I, I, I, I, I, P, P, P]),
         "qgn_prof_read_all": (I, [P, C.POINTER(C.c_int), D]),
+        "qgn_dfti_create_descriptor": (I, [C.POINTER(P), I, I, I, C.POINTER(C.c_longlong)]),
+        "qgn_dfti_set_value_int": (I, [P, I, C.c_longlong]),
+        "qgn_dfti_set_value_ints": (I, [P, I, C.POINTER(C.c_longlong), I]),
+        "qgn_dfti_set_value_double": (I, [P, I, C.c_double]),
+        "qgn_dfti_commit_descriptor": (I, [P]),
+        "qgn_dfti_compute_forward": (I, [P, D, D]),
+        "qgn_dfti_compute_backward": (I, [P, D, D]),
+        "qgn_dfti_free_descriptor": (I, [C.POINTER(P)]),
         "qgn_dft_free": (I, [P]),
         "qgn_vertical_modes": (I, [I, C.c_double, D, D, D, D, D, D, D, D]),
         "qgn_create_batch": (I, [C.POINTER(P), C.POINTER(QgnParams), I, D, D, D, D, D, D]),

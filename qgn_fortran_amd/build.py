@@ -75,6 +75,15 @@ def build_host(verbose=False) -> str | None:
           "-Wl,-rpath,$ORIGIN/..", "-o", HOST_BIN], cwd=objdir)
     if verbose:
         print("built", HOST_BIN)
+    # the DFTI module and the test program that drives it (tests/fortran)
+    check_src = os.path.join(ROOT, "tests", "fortran", "dfti_check.f90")
+    if os.path.exists(check_src):
+        cdir = os.path.join(ROOT, "tests", "fortran", "_build")
+        os.makedirs(cdir, exist_ok=True)
+        _run([FLANG, "-O2", "-J", cdir, os.path.join(HOST, "qgn_dfti.f90"), check_src, "-L", PKG, "-lqgn",
+              "-Wl,-rpath,$ORIGIN/../../../qgn_fortran_amd", "-o", os.path.join(cdir, "dfti_check")], cwd=cdir)
+        if verbose:
+            print("built", os.path.join(cdir, "dfti_check"))
     return HOST_BIN
 
 

@@ -2139,6 +2139,134 @@ int qgn_dft_backward_device(qgn_dft* h, const void* spec, void* grid, void* stre
 
 }  // extern "C"
 
+// ---- DFTI descriptors (create / set / commit / compute / free) -------------
+// The descriptor records the settings MKL_DFTI_Setup.f90:11-50 makes; commit
+// checks that they describe batched 2-D real transforms in the reference's
+// layout (CCE half spectrum along x, out of place, unit scales) and binds a
+// qgn_dft; a compute call checks that the input and output strides and
+// distances fit its direction.
+struct qgn_dfti {
+  int precision = 0, domain = 0, dim = 0;
+  long long len[2] = {};
+  long long howmany = 1, in_dist = -1, out_dist = -1;
+  long long in_str[3] = {-1, -1, -1}, out_str[3] = {-1, -1, -1};
+  int ce_storage = QGN_DFTI_COMPLEX_COMPLEX, packed = QGN_DFTI_CCE_FORMAT, placement = QGN_DFTI_INPLACE;
+  double fscale = 1.0, bscale = 1.0;
+  qgn_dft* dft = nullptr;   // bound at commit
+};
+
+namespace {
+bool dfti_layout(const qgn_dfti* d, bool fwd) {
+  const long long nx = d->len[0], ny = d->len[1], nxh = nx / 2 + 1;
+  const long long rin[3] = {0, 1, nx}, cin[3] = {0, 1, nxh};
+  const long long* want_in = fwd ? rin : cin;
+  const long long* want_out = fwd ? cin : rin;
+  for (int k = 0; k < 3; ++k) {
+    if (d->in_str[k] >= 0 && d->in_str[k] != want_in[k]) return false;
+    if (d->out_str[k] >= 0 && d->out_str[k] != want_out[k]) return false;
+  }
+  const long long rd = nx * ny, cd = nxh * ny;
+  if (d->howmany > 1) {
+    if (d->in_dist >= 0 && d->in_dist != (fwd ? rd : cd)) return false;
+    if (d->out_dist >= 0 && d->out_dist != (fwd ? cd : rd)) return false;
+  }
+  return true;
+}
+}  // namespace
+
+extern "C" {
+
+int qgn_dfti_create_descriptor(qgn_dfti** h, int precision, int domain, int dimension, const long long* lengths) {
+  if (!h || !lengths) return fail(QGN_EINVAL, "null argument");
+  *h = nullptr;
+  if (precision != QGN_DFTI_DOUBLE) return fail(QGN_EINVAL, "DFTI: host transforms take DFTI_DOUBLE data");
+  if (domain != QGN_DFTI_REAL) return fail(QGN_EINVAL, "DFTI: only the DFTI_REAL forward domain is provided");
+  if (dimension != 2) return fail(QGN_EINVAL, "DFTI: only 2-D transforms are provided");
+  qgn_dfti* d = new qgn_dfti();
+  d->precision = precision;
+  d->domain = domain;
+  d->dim = dimension;
+  d->len[0] = lengths[0];
+  d->len[1] = lengths[1];
+  *h = d;
+  return 0;
+}
+
+int qgn_dfti_set_value_int(qgn_dfti* d, int param, long long v) {
+  if (!d) return fail(QGN_EINVAL, "null argument");
+  if (d->dft) return fail(QGN_EINVAL, "DFTI: the descriptor is committed; set values before the commit");
+  switch (param) {
+    case QGN_DFTI_NUMBER_OF_TRANSFORMS:
+      if (v < 1) return fail(QGN_EINVAL, "DFTI: NUMBER_OF_TRANSFORMS must be >= 1");
+      d->howmany = v;
+      return 0;
+    case QGN_DFTI_INPUT_DISTANCE: d->in_dist = v; return 0;
+    case QGN_DFTI_OUTPUT_DISTANCE: d->out_dist = v; return 0;
+    case QGN_DFTI_CONJUGATE_EVEN_STORAGE: d->ce_storage = (int)v; return 0;
+    case QGN_DFTI_PACKED_FORMAT: d->packed = (int)v; return 0;
+    case QGN_DFTI_PLACEMENT: d->placement = (int)v; return 0;
+    default: return fail(QGN_EINVAL, "DFTI: unknown integer parameter " + std::to_string(param));
+  }
+}
+
+int qgn_dfti_set_value_ints(qgn_dfti* d, int param, const long long* v, int n) {
+  if (!d || !v) return fail(QGN_EINVAL, "null argument");
+  if (d->dft) return fail(QGN_EINVAL, "DFTI: the descriptor is committed; set values before the commit");
+  if (n != 3) return fail(QGN_EINVAL, "DFTI: strides of a 2-D transform are 3 values (offset, x, y)");
+  if (param == QGN_DFTI_INPUT_STRIDES) std::memcpy(d->in_str, v, sizeof(d->in_str));
+  else if (param == QGN_DFTI_OUTPUT_STRIDES) std::memcpy(d->out_str, v, sizeof(d->out_str));
+  else return fail(QGN_EINVAL, "DFTI: unknown array parameter " + std::to_string(param));
+  return 0;
+}
+
+int qgn_dfti_set_value_double(qgn_dfti* d, int param, double v) {
+  if (!d) return fail(QGN_EINVAL, "null argument");
+  if (d->dft) return fail(QGN_EINVAL, "DFTI: the descriptor is committed; set values before the commit");
+  if (param == QGN_DFTI_FORWARD_SCALE) d->fscale = v;
+  else if (param == QGN_DFTI_BACKWARD_SCALE) d->bscale = v;
+  else return fail(QGN_EINVAL, "DFTI: unknown real parameter " + std::to_string(param));
+  return 0;
+}
+
+int qgn_dfti_commit_descriptor(qgn_dfti* d) {
+  if (!d) return fail(QGN_EINVAL, "null argument");
+  if (d->dft) return 0;
+  if (d->ce_storage != QGN_DFTI_COMPLEX_COMPLEX || d->packed != QGN_DFTI_CCE_FORMAT)
+    return fail(QGN_EINVAL, "DFTI: the conjugate-even spectrum is stored as DFTI_COMPLEX_COMPLEX in DFTI_CCE_FORMAT");
+  if (d->placement != QGN_DFTI_NOT_INPLACE) return fail(QGN_EINVAL, "DFTI: transforms are DFTI_NOT_INPLACE");
+  if (d->fscale != 1.0 || d->bscale != 1.0) return fail(QGN_EINVAL, "DFTI: transforms are unnormalised (scale 1)");
+  if (!dfti_layout(d, true) && !dfti_layout(d, false))
+    return fail(QGN_EINVAL, "DFTI: strides and distances must describe x-fastest arrays real(nx,ny,howmany) and "
+                            "complex(nx/2+1,ny,howmany)");
+  if (d->len[0] > 1 << 20 || d->len[1] > 1 << 20 || d->howmany > 1 << 20)
+    return fail(QGN_EINVAL, "DFTI: lengths out of range");
+  return qgn_dft_create(&d->dft, QGN_FP64, (int)d->len[0], (int)d->len[1], (int)d->howmany);
+}
+
+int qgn_dfti_compute_forward(qgn_dfti* d, const double* in, double* out) {
+  if (!d || !in || !out) return fail(QGN_EINVAL, "null argument");
+  if (!d->dft) return fail(QGN_EINVAL, "DFTI: commit the descriptor before computing");
+  if (!dfti_layout(d, true)) return fail(QGN_EINVAL, "DFTI: the descriptor's strides describe a backward transform");
+  return qgn_dft_forward(d->dft, in, out);
+}
+
+int qgn_dfti_compute_backward(qgn_dfti* d, const double* in, double* out) {
+  if (!d || !in || !out) return fail(QGN_EINVAL, "null argument");
+  if (!d->dft) return fail(QGN_EINVAL, "DFTI: commit the descriptor before computing");
+  if (!dfti_layout(d, false)) return fail(QGN_EINVAL, "DFTI: the descriptor's strides describe a forward transform");
+  return qgn_dft_backward(d->dft, in, out);
+}
+
+int qgn_dfti_free_descriptor(qgn_dfti** h) {
+  if (!h || !*h) return 0;
+  if ((*h)->dft) qgn_dft_free((*h)->dft);
+  delete *h;
+  *h = nullptr;
+  return 0;
+}
+
+}  // extern "C"
+
 // ---- Transforms that allocate nothing ---------------------------------------
 // qgn_dft_compute_noalloc keeps its intermediate spectrum and its twiddle
 // tables in `scratch`, a fixed array of every GPU the library runs on (held

@@ -107,8 +107,18 @@ end subroutine Initialize
 subroutine TimeStep(q_hat, t, dt)
     complex(dp), intent(inout) :: q_hat(nx/2+1,ny,nz)
     real(dp),    intent(inout) :: t, dt
+    type(qgn_stats) :: s0, s1
+    integer(c_long_long) :: k
+    call qgn_check(qgn_get_stats(ctx, s0), 'qgn_get_stats')
     call qgn_check(qgn_time_step(ctx, t, dt), 'qgn_time_step')
     if (sync_host) call qgn_check(qgn_get_state(ctx, q_hat), 'qgn_get_state')
+    ! ARK43_QGLeith reports each rejected attempt (ARK43_QGLeith.f90:126)
+    if (time_scheme == QGN_ARK43_QGLEITH) then
+        call qgn_check(qgn_get_stats(ctx, s1), 'qgn_get_stats')
+        do k = 1, s1%rejects - s0%rejects
+            print *, 'Rejected step'
+        end do
+    end if
 end subroutine TimeStep
 
 subroutine GetState(q_hat)
