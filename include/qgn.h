@@ -90,7 +90,8 @@ int qgn_batch_set_time(qgn_ctx* ctx, const double* t, const double* dt);
  * the accept/reject and PI.3.4 control of ARK43*.f90 on the device, each
  * member with its own dt: an accepted attempt advances that member, a
  * rejected one shrinks its dt.  The attempt is captured once into a HIP
- * graph; this call only launches it and returns without waiting. */
+ * graph (and a graph of 8 chained attempts, launched once per 8); this call
+ * only launches them and returns without waiting. */
 int qgn_batch_step(qgn_ctx* ctx, int attempts);
 
 /* Wait for the context's work and read per-member t, dt, accepted steps,

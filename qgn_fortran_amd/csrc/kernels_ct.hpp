@@ -124,9 +124,11 @@ __global__ void __launch_bounds__(NTHR) k_col_inv_ct(ColInvArgs a, Geom g, const
   }
 }
 
-// Padding of the contiguous-row LDS layout (one slot per 32 FP32 / 16 FP64
-// complex values; chosen by measured bank conflicts).
-template <typename R> constexpr int row_pad_shift() { return sizeof(R) == 4 ? 5 : 4; }
+// Padding of the contiguous-row LDS layout: one slot per 16 complex values,
+// FP32 and FP64 (the first radix-8 pass stores 8 consecutive values per
+// lane; a slot per 32 left lanes t and t + 32 on one bank: the fused row
+// pass took 76.6 us at 1024^2 x 16 with it, 74.1 with this spacing)
+template <typename R> constexpr int row_pad_shift() { return 4; }
 template <typename R> constexpr int row_lds_slots(int n) { return n + (n >> row_pad_shift<R>()) + 1; }
 
 // x-r2c post-processing on a padded LDS sequence starting at slot `base`.

@@ -44,6 +44,8 @@ int fail(int code, const std::string& msg) {
 
 constexpr int kEPT = 16;        // FFT elements per thread bound (template)
 constexpr int kColThreads = 256;
+// attempts per launch of the chained attempt graphs of batches and ensembles
+constexpr int kChain = 8;
 // fixed device scratch of qgn_dft_compute_noalloc, per GPU
 constexpr size_t kDftScratchBytes = size_t(64) << 20;
 constexpr long double kPi = 3.141592653589793238462643383279502884L;
@@ -310,6 +312,8 @@ struct qgn_ctx {
   bool batch_ready = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
+  hipGraph_t chain = nullptr;          // kChain attempts in sequence, one launch
+  hipGraphExec_t chain_exec = nullptr;
   double2* graph_q = nullptr;   // the buffer the graph takes as the state
   int ens_refs = 0;             // ensembles whose graphs hold this context's attempt
   // stepper state (the reference's `save` variables)
@@ -338,8 +342,7 @@ struct qgn_ens {
   hipGraph_t chain = nullptr;         // kChain attempts of every batch, one launch
   hipGraphExec_t chain_exec = nullptr;
 };
-// attempts per launch of an ensemble's chained graph
-constexpr int kChain = 8;
+
 
 namespace {
 
@@ -1149,7 +1152,28 @@ int build_graph(qgn_ctx* c) {
   c->graph = gr;
   c->graph_q = c->q;
   HIPCHK(hipGraphInstantiate(&c->gexec, gr, nullptr, nullptr, 0));
+  // kChain attempts as one launch: the attempt graph as kChain chained children
+  HIPCHK(hipGraphCreate(&c->chain, 0));
+  hipGraphNode_t prev = nullptr;
+  for (int k = 0; k < kChain; ++k) {
+    hipGraphNode_t nk;
+    HIPCHK(hipGraphAddChildGraphNode(&nk, c->chain, prev ? &prev : nullptr, prev ? 1 : 0, c->graph));
+    prev = nk;
+  }
+  HIPCHK(hipGraphInstantiate(&c->chain_exec, c->chain, nullptr, nullptr, 0));
   return 0;
+}
+
+// drop the captured attempt graphs (their launches changed)
+void drop_graphs(qgn_ctx* c) {
+  if (c->chain_exec) (void)hipGraphExecDestroy(c->chain_exec);
+  if (c->chain) (void)hipGraphDestroy(c->chain);
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->chain_exec = nullptr;
+  c->chain = nullptr;
+  c->gexec = nullptr;
+  c->graph = nullptr;
 }
 
 // ARK43 / ARK43_QGLeith TimeStep.  The accept/reject decision and the
@@ -1376,8 +1400,7 @@ void destroy_ctx(qgn_ctx* c) {
     if (c->gdrag[k]) (void)hipFree(c->gdrag[k]);
     if (c->gdragh[k]) (void)hipFree(c->gdragh[k]);
   }
-  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
-  if (c->graph) (void)hipGraphDestroy(c->graph);
+  drop_graphs(c);
   void* bp[] = {c->ekm, c->cdm, c->bt, c->bdt, c->berr0, c->blast, c->bcount, c->bint, c->brep};
   for (void* x : bp)
     if (x) (void)hipFree(x);
@@ -1801,12 +1824,7 @@ int qgn_modal_fields(qgn_ctx* c, double* q_mode, double* psi_mode, double* j_mod
 
 int qgn_set_closure(qgn_ctx* c, qgn_cnn* net, const double* in_scale, const double* out_scale) {
   if (!c) return fail(QGN_EINVAL, "null argument");
-  if (c->gexec) {   // the captured attempt changes
-    (void)hipGraphExecDestroy(c->gexec);
-    (void)hipGraphDestroy(c->graph);
-    c->gexec = nullptr;
-    c->graph = nullptr;
-  }
+  drop_graphs(c);   // the captured attempt changes
   if (!net) {
     if (c->net) qcnn::hold(c->net, -1);
     c->net = nullptr;
@@ -1872,7 +1890,8 @@ static int batch_check(qgn_ctx* c) {
 int qgn_batch_step(qgn_ctx* c, int attempts) {
   if (!c || attempts < 0) return fail(QGN_EINVAL, "bad argument");
   CHK(batch_check(c));
-  for (int a = 0; a < attempts; ++a) HIPCHK(hipGraphLaunch(c->gexec, c->stream));
+  for (int a = 0; a < attempts / kChain; ++a) HIPCHK(hipGraphLaunch(c->chain_exec, c->stream));
+  for (int a = 0; a < attempts % kChain; ++a) HIPCHK(hipGraphLaunch(c->gexec, c->stream));
   c->st.rhs_evals += 6LL * attempts * c->nm;
   return 0;
 }
@@ -1917,12 +1936,7 @@ int qgn_set_tuning(qgn_ctx* c, int col_inv, int col_rhs, int row) {
   c->tune_col_inv = col_inv;
   c->tune_col_rhs = col_rhs;
   c->tune_row = row;
-  if (c->gexec) {   // the captured attempt has the old launches
-    (void)hipGraphExecDestroy(c->gexec);
-    (void)hipGraphDestroy(c->graph);
-    c->gexec = nullptr;
-    c->graph = nullptr;
-  }
+  drop_graphs(c);   // the captured attempt has the old launches
   return 0;
 }
 

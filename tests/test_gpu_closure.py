@@ -1,28 +1,40 @@
-"""The learned closure end to end at a reduced size (256^2 -> 64^2, 8 layers,
-3 seeds, Gaussian spectral filter of 2 coarse cells): training pairs from
-the GPU model's own high-resolution runs, GPU training, then the trained
-network against no closure and against the QG-Leith forcing on the held-out
-seed (offline R^2), and coarse runs with each (online KE spectra against the
-filtered high-resolution run)."""
+"""The learned closure end to end at the full grid sizes with a shorter data
+set (1024^2 -> 256^2, 8 layers, 3 seeds of 24 snapshots, the last seed held
+out, Gaussian spectral filter of 2 coarse cells, 600 epochs): training pairs
+from the GPU model's own high-resolution runs, GPU training, then the
+trained network against no closure and against the QG-Leith forcing on the
+held-out seed, offline (R^2) and online (coarse runs scored by their
+time-mean KE spectrum against the filtered high-resolution one).  About 20 s
+on MI355X; the full-size pipeline is `python -m qgn_fortran_amd.closure`."""
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
 
 
-def test_closure_pipeline_reduced():
+def test_closure_pipeline_beats_qg_leith():
     from qgn_fortran_amd import closure
-    st = closure.Setup(n_hi=256, n_lo=64, seeds=(21, 22, 23), snapshots=16, every=20, spinup=1500,
-                       epochs=80, batch=4, lr=2e-3, filter_cells=2.0)
+    st = closure.Setup(n_hi=1024, n_lo=256, seeds=(21, 22, 23), holdout=1, snapshots=24, every=25, spinup=2000,
+                       epochs=600, batch=8, lr=1e-3, filter_cells=2.0)
     res = closure.run(st, log=lambda s: print(s, flush=True))
-    print({k: v for k, v in res.items() if k != "setup"}, flush=True)
-    off = res["offline"]
-    assert res["images"] == 48 and off["test_images"] == 16
+    off, on = res["offline"], res["online"]
+    print({k: v for k, v in res.items() if k not in ("setup", "online")}, flush=True)
+    for m in ("network", "network_leith", "qg_leith", "none"):
+        r = on[m]
+        if not r.get("failed"):
+            print(f"{m}: rms {r['ke_spectrum_rms_rel_err']:.3f}, energy-weighted {r['ke_spectrum_l2_rel_err']:.3f}, "
+                  f"max band {r['ke_spectrum_max_rel_err']:.3f}", flush=True)
+    assert res["images"] == 72 and off["test_images"] == 24
     assert res["train"]["loss_last"] < res["train"]["loss_first"]
-    # on the held-out seed the trained network explains part of the subgrid
-    # forcing, where no closure explains none, and more than QG-Leith does
-    assert off["r2_network"] > 0.02
-    assert off["r2_network"] > off["r2_qg_leith"]
-    on = res["online"]
-    assert not on["qg_leith"]["failed"]
-    assert "network" in on
+    # offline: the network explains part of the held-out subgrid forcing
+    # (measured 0.14), far more than QG-Leith does (0.017)
+    assert off["r2_network"] > 0.08 and off["r2_network"] > 3 * off["r2_qg_leith"]
+    # online: the network's KE spectrum error per band is below QG-Leith's
+    # (measured 6.5% against 38.6% rms) and below no closure's; with QG-Leith
+    # beneath it, too (7.4%)
+    for m in ("network", "network_leith", "qg_leith", "none"):
+        assert not on[m]["failed"], m
+    net, leith, none = (on[m]["ke_spectrum_rms_rel_err"] for m in ("network", "qg_leith", "none"))
+    assert net < 0.5 * leith and net < 0.5 * none
+    assert on["network_leith"]["ke_spectrum_rms_rel_err"] < 0.5 * leith
+    assert on["network"]["ke_spectrum_max_rel_err"] < on["qg_leith"]["ke_spectrum_max_rel_err"]
