@@ -743,6 +743,7 @@ struct UpdateArgs {
   double A8, A2;
   int implicit_visc;
   double2* E;     // may be null
+  int e32;        // E as float2 (the mixed path's error norm runs in FP32)
   double2* qn;
   int recompute;  // as in StageArgs: Y_2..Y_ns from q and N
   double ye[6][6], yi[6][6];
@@ -795,7 +796,10 @@ __global__ void k_update(UpdateArgs a, Geom g) {
       ux += a.b[jj] * tx;
       uy += a.b[jj] * ty;
     }
-    if (a.E) a.E[e] = make_double2(ex, ey);
+    if (a.E) {
+      if (a.e32) reinterpret_cast<float2*>(a.E)[e] = make_float2((float)ex, (float)ey);
+      else a.E[e] = make_double2(ex, ey);
+    }
     double2 r = make_double2(q.x + dt * ux, q.y + dt * uy);
     if (pt == 0) r = make_double2(0.0, 0.0);
     a.qn[e] = r;
@@ -1025,7 +1029,9 @@ struct BatchCtl {
   double *t, *dt, *err0, *last;
   long long *steps, *attempts, *rejects;
   int *nrej, *status, *accept;
-  const unsigned long long* maxbits;   // per member, from k_max_final
+  unsigned long long* maxbits;         // per member, from k_max_final, or formed here from partial
+  const unsigned long long* partial;   // npart per-block maxima per member (null: maxbits is set)
+  int npart;
   double* report;   // single model: accept, t, dt, err1, status, err0 of member 0 (or nullptr)
   double TOL, safety, reject_factor, dt_max, nn;
   int max_rejects, nm;
@@ -1054,9 +1060,27 @@ __device__ void batch_control_member(const BatchCtl& c, int m) {
   c.steps[m] += 1;
 }
 
-__global__ void k_batch_control(BatchCtl c) {
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= c.nm) return;
+// one workgroup per member: the max over the error norm's per-block
+// partials (when given), then the member's step control on thread 0
+__global__ void __launch_bounds__(256) k_batch_control(BatchCtl c) {
+  const int m = blockIdx.x;
+  if (c.partial) {
+    __shared__ unsigned long long red[4];
+    const unsigned long long* pm = c.partial + (long)m * c.npart;
+    unsigned long long x = 0;
+    for (int i = threadIdx.x; i < c.npart; i += blockDim.x) x = pm[i] > x ? pm[i] : x;
+    for (int off = 32; off > 0; off >>= 1) {
+      const unsigned long long o = __shfl_down(x, off, 64);
+      x = o > x ? o : x;
+    }
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int w = 1; w < 4; ++w) x = red[w] > x ? red[w] : x;
+      c.maxbits[m] = x;
+    }
+  }
+  if (threadIdx.x != 0) return;
   batch_control_member(c, m);
   if (c.report && m == 0) {
     c.report[0] = c.accept[0];

@@ -273,9 +273,10 @@ struct qgn_ctx {
   void* hist[3] = {};    // AB3 / BSRK3 saved RHS, cplx<R>
   // mixed arrays, cplx<R>
   void *Um = nullptr, *Vm = nullptr, *QXm = nullptr, *QYm = nullptr, *Jm = nullptr, *DUm = nullptr, *DVm = nullptr;
-  double2* Em = nullptr; // FP64 mixed for the error norm
+  double2* Em = nullptr; // mixed rows of the error norm (FP32 on the mixed path)
   unsigned long long* errslot = nullptr;
   unsigned long long* errpart = nullptr;  // per-block partial maxima (ny*nz entries at most)
+  int err_npart = 0;                      // of them per member, set by errnorm_launch
   unsigned long long* h_errslot = nullptr;  // pinned
   // full-layout scratch for the boundary
   double2 *fa = nullptr, *fb = nullptr;
@@ -856,59 +857,76 @@ int rhs(qgn_ctx* c, const double2* qin, void* out, bool psi_ready = false) {
   return rhs_impl<double>(c, qin, out, psi_ready);
 }
 
-// max over the physical grid of |c2r(E)| (unnormalised), FP64, per member
-// into c->errslot[m] as ordered bit patterns (device only)
-int errnorm_launch(qgn_ctx* c, const double2* Ecmp) {
+// the error field's c2r runs in FP32 on the mixed path of the compiled
+// sizes, whose update kernel then writes E as float2: its RHS values carry
+// FP32 rounding already, and the norm only sets the next step size
+bool errnorm_fp32(const qgn_ctx* c) { return c->ct && c->p.precision == QGN_MIXED; }
+
+template <typename R>
+int errnorm_rows(qgn_ctx* c, const void* Ecmp, bool e_double, bool final_max) {
+  const Geom& g = c->g;
+  const Tw<R>& tw = tw_of<R>(c);
+  ColInvArgs ca{};
+  ca.nf = 1;
+  ca.src[0] = Ecmp; ca.src_is_double[0] = e_double; ca.mult[0] = 0; ca.dst[0] = c->Em;
+  CHK(launch_col_inv<R>(c, ca, QGN_K_ERRNORM));
+  int rc = 0;
+  bool hit = with_size(g.N2, [&](auto nc) {
+    constexpr int N2 = decltype(nc)::value;
+    using Cfg = RowCfg<float2, N2, 4>;   // 4 rows of one sequence where the Jacobian has 1 row of 4
+    constexpr int RW = Cfg::ROWS * 4 > 16 ? 16 : Cfg::ROWS * 4;
+    constexpr int NT0 = RW * N2 / 8;
+    constexpr int NT = NT0 < 64 ? 64 : (NT0 > 1024 ? 1024 : ((NT0 + 63) / 64) * 64);
+    const size_t lds = (size_t)(row_lds_slots<R>(RW * N2) + ct_tw_size(N2)) * sizeof(cplx<R>);
+    auto kern = k_row_maxabs_ct<R, N2, RW, NT>;
+    rc = allow_lds(kern, lds);
+    if (rc) return;
+    rc = timed(c, QGN_K_ERRNORM, [&] {
+      hipLaunchKernelGGL(kern, dim3(g.ny * g.nz / RW), dim3(NT), lds, c->stream,
+                         reinterpret_cast<const cplx<R>*>(c->Em), g.Kx, g.Kxp, g, tw.sx, tw.r, c->errpart);
+    });
+    if (rc) return;
+    // RW divides ny, so every block's rows belong to one member
+    c->err_npart = g.ny * g.nzm / RW;
+    if (!final_max) return;
+    hipLaunchKernelGGL(k_max_final, dim3(c->nm), dim3(1024), 0, c->stream, c->errpart, c->err_npart, c->errslot);
+    rc = hipGetLastError() == hipSuccess ? 0 : fail(QGN_EHIP, "max reduction launch failed");
+  }, CtSizes{});
+  if (!hit) return fail(QGN_EINVAL, "nx/2 not supported by the error norm");
+  return rc;
+}
+
+// max over the physical grid of |c2r(E)| (unnormalised) per member into
+// c->errslot[m] as ordered bit patterns (device only); E is FP64 unless
+// e_double is false (float2, errnorm_fp32 only).  final_max = false leaves
+// c->err_npart per-block maxima per member in c->errpart for the step
+// control kernel to reduce.
+int errnorm_launch(qgn_ctx* c, const void* Ecmp, bool e_double = true, bool final_max = true) {
   const Geom& g = c->g;
   if (!c->ct) {   // the generic path: full spectrum, run-time-plan c2r, per-member max
+    if (!e_double) return fail(QGN_EINVAL, "error norm: the generic path takes an FP64 error field");
     const long nf = (long)g.nz * g.ny * g.nxh;
-    hipLaunchKernelGGL((k_gen_expand<double>), dim3(grid1d(nf)), dim3(256), 0, c->stream, (const void*)Ecmp, 1, 0,
-                       c->fa, g);
+    hipLaunchKernelGGL((k_gen_expand<double>), dim3(grid1d(nf)), dim3(256), 0, c->stream, Ecmp, 1, 0, c->fa, g);
     HIPCHK(hipGetLastError());
     CHK(dft_bwd<double>(c->stream, g, c->px, c->py, c->twd, g.nz, c->fa, c->phys, c->fb, 1.0));
     const long per = (long)g.nzm * g.ny * g.nx;
     const int nblk = (int)std::min<long>(std::min<long>((per + 255) / 256, 256), (long)g.ny * g.nzm);
     hipLaunchKernelGGL(k_gen_maxabs, dim3(nblk, c->nm), dim3(256), 0, c->stream, c->phys, per, c->errpart);
     HIPCHK(hipGetLastError());
+    c->err_npart = nblk;
+    if (!final_max) return 0;
     hipLaunchKernelGGL(k_max_final, dim3(c->nm), dim3(1024), 0, c->stream, c->errpart, nblk, c->errslot);
     HIPCHK(hipGetLastError());
     return 0;
   }
-  ColInvArgs ca{};
-  ca.nf = 1;
-  ca.src[0] = Ecmp; ca.src_is_double[0] = 1; ca.mult[0] = 0; ca.dst[0] = c->Em;
-  CHK(launch_col_inv<double>(c, ca, QGN_K_ERRNORM));
-  {
-    int rc = 0;
-    bool hit = with_size(g.N2, [&](auto nc) {
-      constexpr int N2 = decltype(nc)::value;
-      using Cfg = RowCfg<float2, N2, 4>;   // 4 rows of one sequence where the Jacobian has 1 row of 4
-      constexpr int RW = Cfg::ROWS * 4 > 16 ? 16 : Cfg::ROWS * 4;
-      constexpr int NT0 = RW * N2 / 8;
-      constexpr int NT = NT0 < 64 ? 64 : (NT0 > 1024 ? 1024 : ((NT0 + 63) / 64) * 64);
-      const size_t lds = (size_t)(row_lds_slots<double>(RW * N2) + ct_tw_size(N2)) * sizeof(double2);
-      auto kern = k_row_maxabs_ct<double, N2, RW, NT>;
-      rc = allow_lds(kern, lds);
-      if (rc) return;
-      rc = timed(c, QGN_K_ERRNORM, [&] {
-        hipLaunchKernelGGL(kern, dim3(g.ny * g.nz / RW), dim3(NT), lds, c->stream, c->Em, g.Kx, g.Kxp, g,
-                           c->twd.sx, c->twd.r, c->errpart);
-      });
-      if (rc) return;
-      // RW divides ny, so every block's rows belong to one member
-      hipLaunchKernelGGL(k_max_final, dim3(c->nm), dim3(1024), 0, c->stream, c->errpart, g.ny * g.nzm / RW,
-                         c->errslot);
-      rc = hipGetLastError() == hipSuccess ? 0 : fail(QGN_EHIP, "max reduction launch failed");
-    }, CtSizes{});
-    if (!hit) return fail(QGN_EINVAL, "nx/2 not supported by the error norm");
-    CHK(rc);
-  }
-  return 0;
+  if (errnorm_fp32(c)) return errnorm_rows<float>(c, Ecmp, e_double, final_max);
+  if (!e_double) return fail(QGN_EINVAL, "error norm: the FP64 path takes an FP64 error field");
+  return errnorm_rows<double>(c, Ecmp, true, final_max);
 }
 
 // the same for a single model, read back to the host
 int errnorm(qgn_ctx* c, const double2* Ecmp, double* maxv) {
-  CHK(errnorm_launch(c, Ecmp));
+  CHK(errnorm_launch(c, Ecmp, true));
   HIPCHK(hipMemcpyAsync(c->h_errslot, c->errslot, sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   unsigned long long bits = *c->h_errslot;
@@ -1071,6 +1089,7 @@ int ark43_stages(qgn_ctx* c, double dt, const double* dtv, const double* a8l) {
   std::memcpy(u.ye, ye, sizeof(ye));
   std::memcpy(u.yi, yi, sizeof(yi));
   u.E = c->E;
+  u.e32 = errnorm_fp32(c);
   u.qn = c->qn;
   return update(c, u);
 }
@@ -1090,7 +1109,7 @@ int batch_attempt(qgn_ctx* c) {
   }
   CHK(rhs(c, c->q, c->N[0]));
   CHK(ark43_stages(c, 0.0, c->bdt, a8l));
-  CHK(errnorm_launch(c, c->E));
+  CHK(errnorm_launch(c, c->E, !errnorm_fp32(c), false));
   CHK(launch_control(c, nullptr));
   const long per = (long)g.nzm * g.Ky * g.Kxp;
   hipLaunchKernelGGL(k_batch_commit, dim3(grid1d(per) > 2048 ? 2048 : grid1d(per), c->nm), dim3(256), 0, c->stream,
@@ -1100,7 +1119,9 @@ int batch_attempt(qgn_ctx* c) {
 }
 
 // The ARK43 step control of every member (k_batch_control) after the error
-// norm of an attempt; report: the single model's outcome, or nullptr.
+// norm of an attempt, launched with final_max = false (the control kernel
+// reduces its per-block maxima); report: the single model's outcome, or
+// nullptr.
 int launch_control(qgn_ctx* c, double* report) {
   const qgn_params& p = c->p;
   const Geom& g = c->g;
@@ -1111,6 +1132,8 @@ int launch_control(qgn_ctx* c, double* report) {
   k.steps = c->bcount; k.attempts = c->bcount + nm; k.rejects = c->bcount + 2 * nm;
   k.nrej = c->bint; k.status = c->bint + nm; k.accept = c->bint + 2 * nm;
   k.maxbits = c->errslot;
+  k.partial = c->errpart;
+  k.npart = c->err_npart;
   k.TOL = p.TOL;
   k.safety = qgleith ? 0.2 : 0.75;
   k.reject_factor = qgleith ? 0.5 : 0.75;
@@ -1119,7 +1142,7 @@ int launch_control(qgn_ctx* c, double* report) {
   k.max_rejects = p.max_rejects > 0 ? p.max_rejects : 1000;
   k.nm = nm;
   k.report = report;
-  hipLaunchKernelGGL(k_batch_control, dim3((nm + 63) / 64), dim3(64), 0, c->stream, k);
+  hipLaunchKernelGGL(k_batch_control, dim3(nm), dim3(256), 0, c->stream, k);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -1194,7 +1217,7 @@ int step_ark43(qgn_ctx* c, double* t, double* dt, bool qgleith) {
   CHK(rhs(c, c->q, c->N[0]));
   while (true) {
     CHK(ark43_stages(c, 0.0, c->bdt, a8l));
-    CHK(errnorm_launch(c, c->E));
+    CHK(errnorm_launch(c, c->E, !errnorm_fp32(c), false));
     CHK(launch_control(c, c->brep));
     HIPCHK(hipMemcpyAsync(c->h_rep, c->brep, 6 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
