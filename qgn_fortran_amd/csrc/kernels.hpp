@@ -1062,10 +1062,10 @@ __device__ void batch_control_member(const BatchCtl& c, int m) {
 
 // one workgroup per member: the max over the error norm's per-block
 // partials (when given), then the member's step control on thread 0
-__global__ void __launch_bounds__(256) k_batch_control(BatchCtl c) {
+__global__ void __launch_bounds__(1024) k_batch_control(BatchCtl c) {
   const int m = blockIdx.x;
   if (c.partial) {
-    __shared__ unsigned long long red[4];
+    __shared__ unsigned long long red[16];
     const unsigned long long* pm = c.partial + (long)m * c.npart;
     unsigned long long x = 0;
     for (int i = threadIdx.x; i < c.npart; i += blockDim.x) x = pm[i] > x ? pm[i] : x;
@@ -1076,7 +1076,7 @@ __global__ void __launch_bounds__(256) k_batch_control(BatchCtl c) {
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = x;
     __syncthreads();
     if (threadIdx.x == 0) {
-      for (int w = 1; w < 4; ++w) x = red[w] > x ? red[w] : x;
+      for (int w = 1; w < (int)(blockDim.x + 63) / 64; ++w) x = red[w] > x ? red[w] : x;
       c.maxbits[m] = x;
     }
   }

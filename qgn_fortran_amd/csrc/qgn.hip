@@ -1142,7 +1142,7 @@ int launch_control(qgn_ctx* c, double* report) {
   k.max_rejects = p.max_rejects > 0 ? p.max_rejects : 1000;
   k.nm = nm;
   k.report = report;
-  hipLaunchKernelGGL(k_batch_control, dim3(nm), dim3(256), 0, c->stream, k);
+  hipLaunchKernelGGL(k_batch_control, dim3(nm), dim3(1024), 0, c->stream, k);
   HIPCHK(hipGetLastError());
   return 0;
 }

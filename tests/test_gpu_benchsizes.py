@@ -7,7 +7,7 @@ instantiations get their own checks against the FP64 oracle
                     3 FP64 ARK43_QGLeith steps (state <= 1e-10, t and dt
                     <= 1e-8); 3 mixed steps, the oracle taking the dt
                     sequence the mixed controller chose (state <= 1e-5,
-                    the FP32 error norm within 1e-2 of the oracle's).
+                    the FP32 error norm within 3e-2 of the oracle's).
   expstrat_2048x32  mixed RHS per layer <= 1e-5 and FP64 RHS <= 1e-12.
 
 The state is the workload's seeded initial PV with the spectrum broadened
@@ -101,7 +101,7 @@ def test_expstrat_1024x16_rhs_and_steps():
         print(f"  step {n + 1}: dt {tg - t0:.6g}, state rel L2 vs oracle {err:.2e}, err1 {e_gpu:.6e} / {e_orc:.6e} "
               f"(rel {e_rel:.1e})", flush=True)
         assert err < 1e-5
-        assert e_rel < 1e-2
+        assert e_rel < 3e-2   # measured 7.7e-3, 5.8e-5, 3.2e-4
     assert abs(t - tg) <= 1e-12 * tg
     g.close()
 
