@@ -731,6 +731,30 @@ __global__ void __launch_bounds__(256) k_stage_proj_mfma(StageArgs a, Geom g, co
 // Embedded error field and candidate update of ARK43 (ARK43*.f90 error and
 // update blocks), or of any scheme written as q + dt * sum_j b_j (N_j + L_j):
 //   E  = sum_j be_j (N_j + L_j),   qn = q + dt sum_j b_j (N_j + L_j), qn(0,0)=0
+// QG-Leith partial sums (k_leith_partial, and k_update for the candidate of
+// a graph attempt): one term 2 k^8 |v|^2 of a kx > 0 mode, with every
+// product and sum rounded on its own (no contraction), so both kernels form
+// bit-identical sums over the same decomposition (grid (nblk, nz) of 256
+// threads, thread t of block b taking points b*256 + t + k*nblk*256).
+__device__ __forceinline__ double leith_add(double s, const double2 v, long pt, const Geom& g) {
+  const int i = (int)pt % g.Kxp;
+  if (i == 0 || i >= g.Kx) return s;
+  const int j = compact_row_to_full((int)pt / g.Kxp, g);
+  const double kx = kx_of(i, g), ky = ky_of(j, g);
+  const double k2 = __dadd_rn(__dmul_rn(kx, kx), __dmul_rn(ky, ky));
+  const double k4 = __dmul_rn(k2, k2);
+  const double m = __dadd_rn(__dmul_rn(v.x, v.x), __dmul_rn(v.y, v.y));
+  return __dadd_rn(s, __dmul_rn(__dmul_rn(2.0, __dmul_rn(k4, k4)), m));
+}
+// the workgroup's sum of s (256 threads) into *out, in a fixed order
+__device__ __forceinline__ void leith_block_store(double s, double* out) {
+  __shared__ double red[4];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) *out = ((red[0] + red[1]) + red[2]) + red[3];
+}
+
 struct UpdateArgs {
   const double2* q;
   const void* N[6];
@@ -745,6 +769,8 @@ struct UpdateArgs {
   double2* E;     // may be null
   int e32;        // E as float2 (the mixed path's error norm runs in FP32)
   double2* qn;
+  double* lpart;  // QG-Leith partial sums of qn, lnblk per layer (null: none; grid (lnblk, nz) of 256)
+  int lnblk;
   int recompute;  // as in StageArgs: Y_2..Y_ns from q and N
   double ye[6][6], yi[6][6];
 };
@@ -756,6 +782,7 @@ __global__ void k_update(UpdateArgs a, Geom g) {
   const int z = blockIdx.y;
   const bool rec = a.recompute != 0 && a.implicit_visc;
   const double dt = a.dtv ? a.dtv[z / g.nzm] : a.dt;
+  double ls = 0.0;   // QG-Leith partial sum of the candidate
   for (int pt = blockIdx.x * blockDim.x + threadIdx.x; pt < plane; pt += gridDim.x * blockDim.x) {
     const long e = (long)z * plane + pt;
     const double2 q = a.q[e];
@@ -803,7 +830,9 @@ __global__ void k_update(UpdateArgs a, Geom g) {
     double2 r = make_double2(q.x + dt * ux, q.y + dt * uy);
     if (pt == 0) r = make_double2(0.0, 0.0);
     a.qn[e] = r;
+    if (a.lpart) ls = leith_add(ls, r, pt, g);
   }
+  if (a.lpart) leith_block_store(ls, a.lpart + (long)z * a.lnblk + blockIdx.x);
 }
 
 // Plain axpy-type combination for the explicit steppers:
@@ -858,39 +887,32 @@ __global__ void k_lincomb(const void* X0, const void* X1, const void* X2, const 
 // Stage 1: per-block partial sums over kx > 0; stage 2: per layer, adds the
 // kx = 0 column and the partials in a fixed order (deterministic).
 // ---------------------------------------------------------------------------
-__global__ void k_leith_partial(const double2* __restrict__ q, Geom g, double* __restrict__ partial, int nblk) {
-  __shared__ double red[16];
+__global__ void __launch_bounds__(256) k_leith_partial(const double2* __restrict__ q, Geom g,
+                                                     double* __restrict__ partial, int nblk) {
   const int z = blockIdx.y;
   const long plane = (long)g.Ky * g.Kxp;
   double s = 0.0;
-  for (long pt = blockIdx.x * (long)blockDim.x + threadIdx.x; pt < plane; pt += (long)nblk * blockDim.x) {
-    const int i = (int)pt % g.Kxp;
-    if (i == 0 || i >= g.Kx) continue;
-    const int j = compact_row_to_full((int)pt / g.Kxp, g);
-    const double kx = kx_of(i, g), ky = ky_of(j, g);
-    const double k2 = kx * kx + ky * ky;
-    const double k4 = k2 * k2;
-    const double2 v = q[z * plane + pt];
-    s += 2.0 * (k4 * k4) * (v.x * v.x + v.y * v.y);
-  }
-  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  if (lane == 0) red[wid] = s;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double t = 0.0;
-    for (int w = 0; w < (int)(blockDim.x + 63) / 64; ++w) t += red[w];
-    partial[z * nblk + blockIdx.x] = t;
-  }
+  for (long pt = blockIdx.x * (long)blockDim.x + threadIdx.x; pt < plane; pt += (long)nblk * blockDim.x)
+    s = leith_add(s, q[z * plane + pt], pt, g);
+  leith_block_store(s, partial + z * nblk + blockIdx.x);
 }
 
-__global__ void k_leith_final(const double2* __restrict__ q, Geom g, const double* __restrict__ partial, int nblk,
-                              double c12, double* __restrict__ a8l) {
+// Per layer: the kx = 0 column and the nblk partials (one per thread,
+// nblk <= 256) in a fixed order, then A8L.  With cand (a graph attempt):
+// the members that accepted take the candidate's partials in cand, which
+// then replace theirs in `partial`; q is the state after the attempt.
+__global__ void __launch_bounds__(256) k_leith_final(const double2* __restrict__ q, Geom g, double* partial,
+                                                   const double* __restrict__ cand, const int* __restrict__ accept,
+                                                   int nblk, double c12, double* __restrict__ a8l) {
   __shared__ double red[16];
   const int z = blockIdx.x;
   const long plane = (long)g.Ky * g.Kxp;
-  // the partial sums enter the block reduction one per thread (nblk <= blockDim.x)
-  double s = (int)threadIdx.x < nblk ? partial[z * nblk + threadIdx.x] : 0.0;
+  const bool take = cand && accept[z / g.nzm];
+  double s = 0.0;
+  if ((int)threadIdx.x < nblk) {
+    s = (take ? cand : partial)[z * nblk + threadIdx.x];
+    if (take) partial[z * nblk + threadIdx.x] = s;
+  }
   for (int j = threadIdx.x; j < g.ny; j += blockDim.x) {
     const int jc = full_row_to_compact(j, g);
     const int jm = (g.ny - j) % g.ny;
@@ -1104,15 +1126,17 @@ __global__ void k_control_set(double* t, double* dt, double* err0, int* nrej, in
   status[0] = 0;
 }
 
-// q <- qn for the members that accepted; grid (blocks, members)
-__global__ void k_batch_commit(double2* __restrict__ q, const double2* __restrict__ qn, const int* __restrict__ accept,
-                               Geom g) {
+// End of a graph attempt: the members that did not accept copy their state
+// q into qn, so that qn holds every member's state (the next attempt's
+// input); grid (blocks, members)
+__global__ void k_batch_settle(const double2* __restrict__ q, double2* __restrict__ qn,
+                               const int* __restrict__ accept, Geom g) {
   const int m = blockIdx.y;
-  if (!accept[m]) return;
+  if (accept[m]) return;
   const long n = (long)g.nzm * g.Ky * g.Kxp;
   const long off = (long)m * n;
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x)
-    q[off + e] = qn[off + e];
+    qn[off + e] = q[off + e];
 }
 
 // ---------------------------------------------------------------------------

@@ -256,7 +256,8 @@ struct qgn_ctx {
   // constants
   double *HM = nullptr, *MD = nullptr, *evals = nullptr, *ubar = nullptr, *qybar = nullptr;
   double *a8l = nullptr, *partial = nullptr;
-  int leith_nblk = 64;
+  double* lcand = nullptr;   // QG-Leith partial sums of a graph attempt's candidate (k_update)
+  int leith_nblk = 64;       // QG-Leith partial sums per layer (<= 256), set at creation
   int ncu = 256;         // compute units of the device
   double c12 = 0.0;
   int nm = 1;            // members stacked along z (g.nz = nm * g.nzm)
@@ -311,11 +312,13 @@ struct qgn_ctx {
   double* h_rep = nullptr;      // pinned copy of it
   int* bint = nullptr;          // consecutive rejects, status, accept per member (3 nm)
   bool batch_ready = false;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t gexec = nullptr;
-  hipGraph_t chain = nullptr;          // kChain attempts in sequence, one launch
-  hipGraphExec_t chain_exec = nullptr;
-  double2* graph_q = nullptr;   // the buffer the graph takes as the state
+  // captured attempts: variant v reads the state from gbuf[v] and leaves it
+  // in gbuf[1 - v] (gbuf: the two state buffers q and qn)
+  double2* gbuf[2] = {nullptr, nullptr};
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  hipGraph_t chain[2] = {nullptr, nullptr};          // kChain attempts from variant v on, one launch
+  hipGraphExec_t chain_exec[2] = {nullptr, nullptr};
   int ens_refs = 0;             // ensembles whose graphs hold this context's attempt
   // stepper state (the reference's `save` variables)
   bool first = true, second = false;
@@ -338,10 +341,13 @@ struct qgn_ens {
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
   std::vector<hipEvent_t> ready;
-  hipGraph_t graph = nullptr;         // one attempt of every batch
-  hipGraphExec_t exec = nullptr;
-  hipGraph_t chain = nullptr;         // kChain attempts of every batch, one launch
-  hipGraphExec_t chain_exec = nullptr;
+  // per variant v of the batches' attempt graphs (each batch's state in its
+  // gbuf[v]): one attempt of every batch, and kChain attempts of every batch
+  // in one launch
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  hipGraph_t chain[2] = {nullptr, nullptr};
+  hipGraphExec_t chain_exec[2] = {nullptr, nullptr};
 };
 
 
@@ -582,9 +588,16 @@ int launch_stage_proj_mfma(qgn_ctx* c, const StageArgs& a, int kernel_id) {
 }
 bool use_mfma_projection(const qgn_ctx* c) { return c->p.precision == QGN_MIXED && c->g.nzm <= 32; }
 
+// Rows per workgroup of the fused row pass without the autotuner: 2, or
+// more while a workgroup would hold less than one wave (measured inside
+// graph attempts at 1024^2 x 16: 73.0 us with 2 rows, 77.4 with 4; the
+// autotuner, timing whole GetRHS evaluations, keeps the rule's pick unless a
+// candidate wins by 2% of the whole)
+constexpr int row_group_rule(int N2) { return N2 >= 256 ? 2 : 512 / N2; }
+
 // P2: the Jacobian row pass.  FP32 with 64 | nx/2 uses the register-fused
-// kernel with c->tune_row rows per workgroup (default 4); otherwise the
-// persistent-grid kernel.
+// kernel with c->tune_row rows per workgroup (default row_group_rule);
+// otherwise the persistent-grid kernel.
 template <typename R>
 int launch_row_jac(qgn_ctx* c, const RowJacArgs& ra) {
   const Geom& g = c->g;
@@ -608,7 +621,7 @@ int launch_row_jac(qgn_ctx* c, const RowJacArgs& ra) {
           });
         }
       };
-      switch (c->tune_row ? c->tune_row : 4) {
+      switch (c->tune_row ? c->tune_row : row_group_rule(N2)) {
         case 2: go(std::integral_constant<int, 2>{}); break;
         case 4: go(std::integral_constant<int, 4>{}); break;
         case 8: go(std::integral_constant<int, 8>{}); break;
@@ -941,8 +954,8 @@ int leith(qgn_ctx* c, const double2* q) {
   hipLaunchKernelGGL(k_leith_partial, dim3(c->leith_nblk, g.nz), dim3(256), 0, c->stream, q, g, c->partial,
                      c->leith_nblk);
   HIPCHK(hipGetLastError());
-  hipLaunchKernelGGL(k_leith_final, dim3(g.nz), dim3(256), 0, c->stream, q, g, c->partial, c->leith_nblk, c->c12,
-                     c->a8l);
+  hipLaunchKernelGGL(k_leith_final, dim3(g.nz), dim3(256), 0, c->stream, q, g, c->partial, (const double*)nullptr,
+                     (const int*)nullptr, c->leith_nblk, c->c12, c->a8l);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -993,7 +1006,8 @@ int stage_proj(qgn_ctx* c, const StageArgs& a, bool* done) {
 
 int update(qgn_ctx* c, const UpdateArgs& a) {
   const long plane = (long)c->g.Ky * c->g.Kxp;
-  const dim3 grid(grid1d(plane), c->g.nz);
+  // with the QG-Leith partial sums: the decomposition of k_leith_partial
+  const dim3 grid(a.lpart ? a.lnblk : grid1d(plane), c->g.nz);
   return timed(c, QGN_K_UPDATE, [&] {
     if (c->p.precision == QGN_MIXED)
       hipLaunchKernelGGL((k_update<float>), grid, dim3(256), 0, c->stream, a, c->g);
@@ -1030,8 +1044,10 @@ int lincomb(qgn_ctx* c, const void* X0, const void* X1, const void* X2, const vo
 // ---------------------------------------------------------------------------
 // Stages 2-6 and the candidate update of one ARK43 attempt from N[0] =
 // RHS(q) (ARK43*.f90 stage and update blocks): Y_i, N_i, then E and qn.
-// dtv: per-member dt on the device (a batch), else dt.
-int ark43_stages(qgn_ctx* c, double dt, const double* dtv, const double* a8l) {
+// dtv: per-member dt on the device (a batch), else dt.  lpart: where the
+// update kernel leaves the candidate's QG-Leith partial sums (a graph
+// attempt), or null.
+int ark43_stages(qgn_ctx* c, double dt, const double* dtv, const double* a8l, double* lpart = nullptr) {
   const qgn_params& p = c->p;
   // The stage and update kernels recompute the earlier stage values from q
   // and N (per mode, the implicit term is diagonal) instead of reading them.
@@ -1091,30 +1107,41 @@ int ark43_stages(qgn_ctx* c, double dt, const double* dtv, const double* a8l) {
   u.E = c->E;
   u.e32 = errnorm_fp32(c);
   u.qn = c->qn;
+  u.lpart = lpart;
+  u.lnblk = c->leith_nblk;
   return update(c, u);
 }
 
 int launch_control(qgn_ctx* c, double* report);
 
 // One device-controlled ARK43 attempt of every member of a batch; only
-// kernel launches, so it can be captured into a graph.
+// kernel launches, so it can be captured into a graph.  The state is read
+// from c->q and left in c->qn: the update writes the candidate there, and
+// the members that reject copy their state over it (k_batch_settle), so
+// successive attempts alternate between the two buffers and no accepted
+// step is copied.  With QG-Leith, A8L of the state the attempt leaves is
+// formed at its end from the partial sums the update kernel took of the
+// candidate (or, for a member that rejected, those of its unchanged state),
+// so the next attempt starts with its viscosity; the caller forms A8L of
+// the first attempt's state.
 int batch_attempt(qgn_ctx* c) {
   const qgn_params& p = c->p;
   const Geom& g = c->g;
   const bool qgleith = p.scheme == QGN_ARK43_QGLEITH;
-  const double* a8l = nullptr;
-  if (qgleith) {
-    CHK(leith(c, c->q));
-    a8l = c->a8l;
-  }
   CHK(rhs(c, c->q, c->N[0]));
-  CHK(ark43_stages(c, 0.0, c->bdt, a8l));
+  CHK(ark43_stages(c, 0.0, c->bdt, qgleith ? c->a8l : nullptr, qgleith ? c->lcand : nullptr));
   CHK(errnorm_launch(c, c->E, !errnorm_fp32(c), false));
   CHK(launch_control(c, nullptr));
   const long per = (long)g.nzm * g.Ky * g.Kxp;
-  hipLaunchKernelGGL(k_batch_commit, dim3(grid1d(per) > 2048 ? 2048 : grid1d(per), c->nm), dim3(256), 0, c->stream,
-                     c->q, c->qn, c->bint + 2 * c->nm, g);
+  const int* accept = c->bint + 2 * c->nm;
+  hipLaunchKernelGGL(k_batch_settle, dim3(grid1d(per) > 2048 ? 2048 : grid1d(per), c->nm), dim3(256), 0, c->stream,
+                     c->q, c->qn, accept, g);
   HIPCHK(hipGetLastError());
+  if (qgleith) {
+    hipLaunchKernelGGL(k_leith_final, dim3(g.nz), dim3(256), 0, c->stream, c->qn, g, c->partial,
+                       (const double*)c->lcand, accept, c->leith_nblk, c->c12, c->a8l);
+    HIPCHK(hipGetLastError());
+  }
   return 0;
 }
 
@@ -1147,56 +1174,85 @@ int launch_control(qgn_ctx* c, double* report) {
   return 0;
 }
 
-// qgn_time_step swaps the state buffers; put the state back where the
-// captured graph reads it.
-int graph_state_sync(qgn_ctx* c) {
-  if (c->gexec && c->q != c->graph_q) {
-    HIPCHK(hipMemcpyAsync(c->graph_q, c->q, (size_t)c->ncmp * sizeof(double2), hipMemcpyDeviceToDevice, c->stream));
-    std::swap(c->q, c->qn);
-  }
+void drop_graphs(qgn_ctx* c);
+
+// The variant of the captured attempt whose input buffer holds the state
+// (qgn_time_step and the attempts move it between the two buffers).
+int graph_variant(qgn_ctx* c) { return c->q == c->gbuf[0] ? 0 : 1; }
+
+// Make gbuf[v] the state buffer (a copy only when it is not).
+int graph_set_variant(qgn_ctx* c, int v) {
+  if (graph_variant(c) == v) return 0;
+  HIPCHK(hipMemcpyAsync(c->gbuf[v], c->q, (size_t)c->ncmp * sizeof(double2), hipMemcpyDeviceToDevice, c->stream));
+  std::swap(c->q, c->qn);
   return 0;
 }
 
-// Capture batch_attempt into c->graph / c->gexec.
+// Capture batch_attempt reading gbuf[v] into c->graph[v] / c->gexec[v], for
+// v = 0, 1; then c->chain[v]: kChain attempts from variant v on, the
+// variants alternating (kChain is even, so a chain leaves the state where it
+// found it).
 int build_graph(qgn_ctx* c) {
-  if (c->gexec) return 0;
+  if (c->gexec[0]) return 0;
+  c->gbuf[0] = c->q;
+  c->gbuf[1] = c->qn;
   const int prof = c->prof_id;
   c->prof_id = -1;   // no event records inside the graph
-  HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
-  const int rc = batch_attempt(c);
-  hipGraph_t gr = nullptr;
-  const hipError_t e = hipStreamEndCapture(c->stream, &gr);
+  int rc = 0;
+  for (int v = 0; v < 2 && !rc; ++v) {
+    c->q = c->gbuf[v];
+    c->qn = c->gbuf[1 - v];
+    hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed);
+    if (e != hipSuccess) {
+      rc = fail(QGN_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
+      break;
+    }
+    rc = batch_attempt(c);
+    hipGraph_t gr = nullptr;
+    e = hipStreamEndCapture(c->stream, &gr);
+    if (rc) {
+      if (gr) (void)hipGraphDestroy(gr);
+    } else if (e != hipSuccess) {
+      rc = fail(QGN_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
+    } else {
+      c->graph[v] = gr;
+    }
+  }
+  c->q = c->gbuf[0];
+  c->qn = c->gbuf[1];
   c->prof_id = prof;
-  if (rc) {
-    if (gr) (void)hipGraphDestroy(gr);
-    return rc;
-  }
-  if (e != hipSuccess) return fail(QGN_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
-  c->graph = gr;
-  c->graph_q = c->q;
-  HIPCHK(hipGraphInstantiate(&c->gexec, gr, nullptr, nullptr, 0));
-  // kChain attempts as one launch: the attempt graph as kChain chained children
-  HIPCHK(hipGraphCreate(&c->chain, 0));
-  hipGraphNode_t prev = nullptr;
-  for (int k = 0; k < kChain; ++k) {
-    hipGraphNode_t nk;
-    HIPCHK(hipGraphAddChildGraphNode(&nk, c->chain, prev ? &prev : nullptr, prev ? 1 : 0, c->graph));
-    prev = nk;
-  }
-  HIPCHK(hipGraphInstantiate(&c->chain_exec, c->chain, nullptr, nullptr, 0));
-  return 0;
+  auto instantiate = [&]() -> int {
+    for (int v = 0; v < 2; ++v) {
+      HIPCHK(hipGraphInstantiate(&c->gexec[v], c->graph[v], nullptr, nullptr, 0));
+      HIPCHK(hipGraphCreate(&c->chain[v], 0));
+      hipGraphNode_t prev = nullptr;
+      for (int k = 0; k < kChain; ++k) {
+        hipGraphNode_t nk;
+        HIPCHK(hipGraphAddChildGraphNode(&nk, c->chain[v], prev ? &prev : nullptr, prev ? 1 : 0,
+                                         c->graph[(v + k) & 1]));
+        prev = nk;
+      }
+      HIPCHK(hipGraphInstantiate(&c->chain_exec[v], c->chain[v], nullptr, nullptr, 0));
+    }
+    return 0;
+  };
+  if (!rc) rc = instantiate();
+  if (rc) drop_graphs(c);
+  return rc;
 }
 
 // drop the captured attempt graphs (their launches changed)
 void drop_graphs(qgn_ctx* c) {
-  if (c->chain_exec) (void)hipGraphExecDestroy(c->chain_exec);
-  if (c->chain) (void)hipGraphDestroy(c->chain);
-  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
-  if (c->graph) (void)hipGraphDestroy(c->graph);
-  c->chain_exec = nullptr;
-  c->chain = nullptr;
-  c->gexec = nullptr;
-  c->graph = nullptr;
+  for (int v = 0; v < 2; ++v) {
+    if (c->chain_exec[v]) (void)hipGraphExecDestroy(c->chain_exec[v]);
+    if (c->chain[v]) (void)hipGraphDestroy(c->chain[v]);
+    if (c->gexec[v]) (void)hipGraphExecDestroy(c->gexec[v]);
+    if (c->graph[v]) (void)hipGraphDestroy(c->graph[v]);
+    c->chain_exec[v] = nullptr;
+    c->chain[v] = nullptr;
+    c->gexec[v] = nullptr;
+    c->graph[v] = nullptr;
+  }
 }
 
 // ARK43 / ARK43_QGLeith TimeStep.  The accept/reject decision and the
@@ -1396,7 +1452,7 @@ void destroy_ctx(qgn_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->net) qcnn::hold(c->net, -1);
-  void* ptrs[] = {c->HM, c->MD, c->evals, c->ubar, c->qybar, c->a8l, c->partial, c->q, c->qn, c->E, c->psi64,
+  void* ptrs[] = {c->HM, c->MD, c->evals, c->ubar, c->qybar, c->a8l, c->partial, c->lcand, c->q, c->qn, c->E, c->psi64,
                   c->psi, c->q32, c->Um, c->Vm, c->QXm, c->QYm, c->Jm, c->DUm, c->DVm, c->Em, c->errslot, c->errpart, c->fa, c->fb,
                   c->phys};
   for (void* p : ptrs)
@@ -1505,7 +1561,7 @@ int autotune(qgn_ctx* c) {
         if (!rc) tbest[w][k] = std::min(tbest[w][k], ms);
       }
       // the current pick (0: the fixed rule's value) against the others
-      const int dflt = w == 2 ? 4 : w == 1 ? ColRhsF(g.ny) : ColCfgF(g.ny);
+      const int dflt = w == 2 ? row_group_rule(g.N2) : w == 1 ? ColRhsF(g.ny) : ColCfgF(g.ny);
       int ci = -1;
       for (int k = 0; k < n; ++k)
         if ((w == 2 ? rows[k] : tiles[k]) == (cur ? cur : dflt)) ci = k;
@@ -1589,7 +1645,10 @@ int create_impl(qgn_ctx* c, const qgn_params* p, int members, const double* H, c
     std::vector<double> a8(nz, p->A8);
     HIPCHK(hipMemcpy(c->a8l, a8.data(), nz * sizeof(double), hipMemcpyHostToDevice));
   }
+  // as many partial sums per layer as 256-thread blocks cover the plane, at most 256
+  c->leith_nblk = std::min(256, grid1d((long)c->g.Ky * c->g.Kxp));
   CHK(alloc((void**)&c->partial, (size_t)nz * c->leith_nblk * sizeof(double)));
+  CHK(alloc((void**)&c->lcand, (size_t)nz * c->leith_nblk * sizeof(double)));
   {
     // (1.3*Lx/(nx*pi))**12 with 1.3 a default-real literal (ARK43_QGLeith.f90:72)
     const double c13 = (double)1.3f;
@@ -1906,15 +1965,27 @@ static int batch_check(qgn_ctx* c) {
   if (c->p.scheme != QGN_ARK43 && c->p.scheme != QGN_ARK43_QGLEITH)
     return fail(QGN_EINVAL, "batch stepping supports ARK43 and ARK43_QGLeith");
   if (!c->batch_ready) return fail(QGN_EINVAL, "call qgn_batch_set_time first");
-  CHK(build_graph(c));
-  return graph_state_sync(c);
+  return build_graph(c);
+}
+
+// before the first captured attempt of a launch sequence: A8L of the
+// current state (each attempt forms it for the state it leaves)
+int batch_prologue(qgn_ctx* c) {
+  if (c->p.scheme == QGN_ARK43_QGLEITH) CHK(leith(c, c->q));
+  return 0;
 }
 
 int qgn_batch_step(qgn_ctx* c, int attempts) {
   if (!c || attempts < 0) return fail(QGN_EINVAL, "bad argument");
   CHK(batch_check(c));
-  for (int a = 0; a < attempts / kChain; ++a) HIPCHK(hipGraphLaunch(c->chain_exec, c->stream));
-  for (int a = 0; a < attempts % kChain; ++a) HIPCHK(hipGraphLaunch(c->gexec, c->stream));
+  if (attempts == 0) return 0;
+  CHK(batch_prologue(c));
+  const int v = graph_variant(c);
+  for (int a = 0; a < attempts / kChain; ++a) HIPCHK(hipGraphLaunch(c->chain_exec[v], c->stream));
+  for (int a = 0; a < attempts % kChain; ++a) {
+    HIPCHK(hipGraphLaunch(c->gexec[graph_variant(c)], c->stream));
+    std::swap(c->q, c->qn);   // the attempt left the state in the other buffer
+  }
   c->st.rhs_evals += 6LL * attempts * c->nm;
   return 0;
 }
@@ -2400,10 +2471,12 @@ int qgn_ensemble_destroy(qgn_ens* e) {
   if (e->stream) (void)hipStreamSynchronize(e->stream);
   for (qgn_ctx* c : e->ctx) c->ens_refs--;
   for (qgn_cnn* n : e->nets) qcnn::hold(n, -1);
-  if (e->exec) (void)hipGraphExecDestroy(e->exec);
-  if (e->graph) (void)hipGraphDestroy(e->graph);
-  if (e->chain_exec) (void)hipGraphExecDestroy(e->chain_exec);
-  if (e->chain) (void)hipGraphDestroy(e->chain);
+  for (int v = 0; v < 2; ++v) {
+    if (e->exec[v]) (void)hipGraphExecDestroy(e->exec[v]);
+    if (e->graph[v]) (void)hipGraphDestroy(e->graph[v]);
+    if (e->chain_exec[v]) (void)hipGraphExecDestroy(e->chain_exec[v]);
+    if (e->chain[v]) (void)hipGraphDestroy(e->chain[v]);
+  }
   for (hipEvent_t x : e->ready) (void)hipEventDestroy(x);
   if (e->done) (void)hipEventDestroy(e->done);
   if (e->stream) (void)hipStreamDestroy(e->stream);
@@ -2411,12 +2484,15 @@ int qgn_ensemble_destroy(qgn_ens* e) {
   return 0;
 }
 
-// Two graphs: one attempt of every batch (its attempt graph as a child),
-// and kChain attempts of every batch, each batch a chain of kChain children
+// Per variant v of the batches' attempt graphs, two graphs: one attempt of
+// every batch (its attempt graph as a child), and kChain attempts of every
+// batch, each batch a chain of kChain children (the variants alternating)
 // that depend on one another and on nothing of the other batches.
 static int ensemble_create_impl(qgn_ens* e, qgn_ctx* const* ctxs, int n) {
-  HIPCHK(hipGraphCreate(&e->graph, 0));
-  HIPCHK(hipGraphCreate(&e->chain, 0));
+  for (int v = 0; v < 2; ++v) {
+    HIPCHK(hipGraphCreate(&e->graph[v], 0));
+    HIPCHK(hipGraphCreate(&e->chain[v], 0));
+  }
   for (int i = 0; i < n; ++i) {
     if (!ctxs[i]) return fail(QGN_EINVAL, "null context");
     for (int k = 0; k < i; ++k) {
@@ -2432,20 +2508,25 @@ static int ensemble_create_impl(qgn_ens* e, qgn_ctx* const* ctxs, int n) {
       qcnn::hold(ctxs[i]->net, +1);
       e->nets.push_back(ctxs[i]->net);
     }
-    hipGraphNode_t node;
-    HIPCHK(hipGraphAddChildGraphNode(&node, e->graph, nullptr, 0, ctxs[i]->graph));
-    hipGraphNode_t prev = nullptr;
-    for (int k = 0; k < kChain; ++k) {
-      hipGraphNode_t nk;
-      HIPCHK(hipGraphAddChildGraphNode(&nk, e->chain, prev ? &prev : nullptr, prev ? 1 : 0, ctxs[i]->graph));
-      prev = nk;
+    for (int v = 0; v < 2; ++v) {
+      hipGraphNode_t node;
+      HIPCHK(hipGraphAddChildGraphNode(&node, e->graph[v], nullptr, 0, ctxs[i]->graph[v]));
+      hipGraphNode_t prev = nullptr;
+      for (int k = 0; k < kChain; ++k) {
+        hipGraphNode_t nk;
+        HIPCHK(hipGraphAddChildGraphNode(&nk, e->chain[v], prev ? &prev : nullptr, prev ? 1 : 0,
+                                         ctxs[i]->graph[(v + k) & 1]));
+        prev = nk;
+      }
     }
     hipEvent_t ev;
     HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
     e->ready.push_back(ev);
   }
-  HIPCHK(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
-  HIPCHK(hipGraphInstantiate(&e->chain_exec, e->chain, nullptr, nullptr, 0));
+  for (int v = 0; v < 2; ++v) {
+    HIPCHK(hipGraphInstantiate(&e->exec[v], e->graph[v], nullptr, nullptr, 0));
+    HIPCHK(hipGraphInstantiate(&e->chain_exec[v], e->chain[v], nullptr, nullptr, 0));
+  }
   HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
   HIPCHK(hipEventCreateWithFlags(&e->done, hipEventDisableTiming));
   return 0;
@@ -2468,14 +2549,22 @@ int qgn_ensemble_create(qgn_ens** out, qgn_ctx* const* ctxs, int n) {
 
 int qgn_ensemble_step(qgn_ens* e, int attempts) {
   if (!e || attempts < 0) return fail(QGN_EINVAL, "bad argument");
+  if (attempts == 0) return 0;
+  // every batch's state into the input buffer of one variant, A8L of it,
   // after whatever each context has queued, and before what it queues next
+  int v = graph_variant(e->ctx[0]);
   for (size_t i = 0; i < e->ctx.size(); ++i) {
-    CHK(graph_state_sync(e->ctx[i]));
+    CHK(graph_set_variant(e->ctx[i], v));
+    CHK(batch_prologue(e->ctx[i]));
     HIPCHK(hipEventRecord(e->ready[i], e->ctx[i]->stream));
     HIPCHK(hipStreamWaitEvent(e->stream, e->ready[i], 0));
   }
-  for (int a = 0; a < attempts / kChain; ++a) HIPCHK(hipGraphLaunch(e->chain_exec, e->stream));
-  for (int a = 0; a < attempts % kChain; ++a) HIPCHK(hipGraphLaunch(e->exec, e->stream));
+  for (int a = 0; a < attempts / kChain; ++a) HIPCHK(hipGraphLaunch(e->chain_exec[v], e->stream));
+  for (int a = 0; a < attempts % kChain; ++a) {
+    HIPCHK(hipGraphLaunch(e->exec[v], e->stream));
+    v ^= 1;
+    for (qgn_ctx* c : e->ctx) std::swap(c->q, c->qn);   // the attempt left each state in the other buffer
+  }
   HIPCHK(hipEventRecord(e->done, e->stream));
   for (qgn_ctx* c : e->ctx) {
     HIPCHK(hipStreamWaitEvent(c->stream, e->done, 0));
