@@ -588,12 +588,12 @@ int launch_stage_proj_mfma(qgn_ctx* c, const StageArgs& a, int kernel_id) {
 }
 bool use_mfma_projection(const qgn_ctx* c) { return c->p.precision == QGN_MIXED && c->g.nzm <= 32; }
 
-// Rows per workgroup of the fused row pass without the autotuner: 2, or
-// more while a workgroup would hold less than one wave (measured inside
-// graph attempts at 1024^2 x 16: 73.0 us with 2 rows, 77.4 with 4; the
-// autotuner, timing whole GetRHS evaluations, keeps the rule's pick unless a
-// candidate wins by 2% of the whole)
-constexpr int row_group_rule(int N2) { return N2 >= 256 ? 2 : 512 / N2; }
+// Rows per workgroup of the fused row pass without the autotuner: the
+// fewest of 2, 4, 8 that give a workgroup (N2/8 threads per row) a full
+// wave (measured inside graph attempts at 1024^2 x 16: 73.0 us with 2 rows,
+// 77.4 with 4; the autotuner, timing whole GetRHS evaluations, keeps the
+// rule's pick unless a candidate wins by 2% of the whole)
+constexpr int row_group_rule(int N2) { return 2 * N2 >= 512 ? 2 : (4 * N2 >= 512 ? 4 : 8); }
 
 // P2: the Jacobian row pass.  FP32 with 64 | nx/2 uses the register-fused
 // kernel with c->tune_row rows per workgroup (default row_group_rule);

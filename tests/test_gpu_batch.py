@@ -19,10 +19,11 @@ def _single(p, m):
     return QGN(dataclasses.replace(p, r0=m["r0"]), m["H"], m["Modes"], m["EVals"], m["uBar"], m["qyBar"])
 
 
-def _check_members(batch, group, q0_hat, which, tol_state, tol_t, same_tiles=False):
+def _check_members(batch, group, q0_hat, which, tol_state, tol_t, same_tiles=False, exact=False):
     """Run members `which` alone for as many accepted steps as the batch took
     (same_tiles: with the batch's FFT tile choices, which on the mixed path
-    decide the FP32 rounding)."""
+    decide the FP32 rounding; exact: the states and times must agree to the
+    bit)."""
     info = batch.times()
     state = batch.get_state()
     nz = group.params.nz
@@ -43,6 +44,8 @@ def _check_members(batch, group, q0_hat, which, tol_state, tol_t, same_tiles=Fal
         assert steps >= 1
         assert abs(info["t"][m] - t) <= tol_t * t
         assert err < tol_state
+        if exact:
+            assert np.array_equal(state[m * nz:(m + 1) * nz], single.get_state()) and info["t"][m] == t
         single.close()
     return info
 
@@ -50,7 +53,11 @@ def _check_members(batch, group, q0_hat, which, tol_state, tol_t, same_tiles=Fal
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_batch_members_match_single_runs(precision):
     """4 members at 96^2 x 4 with different deformation radii, Ekman drag
-    and initial states, 10 attempts from dt = 60000 s (rejections occur)."""
+    and initial states, 10 attempts from dt = 60000 s (rejections occur).
+    The batch's graph attempts (candidate and QG-Leith partial sums from the
+    update kernel, state alternating between two buffers) and the single
+    runs' qgn_time_step (a Leith pass over q per step) agree to the bit;
+    the mixed single runs take the batch's FFT tiles."""
     from qgn_fortran_amd import Batch, configs
     print("batch", precision, flush=True)
     group = configs.ensemble96(sizes=(96,), per_size=4, nz=4, precision=precision, seed=11)[0]
@@ -62,7 +69,8 @@ def test_batch_members_match_single_runs(precision):
     b.set_time(0.0, group.dt0)
     b.step(10)
     tol = 1e-10 if precision == "fp64" else 1e-6
-    info = _check_members(b, group, q0_hat, range(4), tol, 1e-12 if precision == "fp64" else 1e-6)
+    info = _check_members(b, group, q0_hat, range(4), tol, 1e-12 if precision == "fp64" else 1e-6,
+                          same_tiles=precision == "mixed", exact=True)
     assert info["rejects"].sum() > 0       # the reject branch ran
     if precision == "fp64":
         # member 2 against the oracle directly
