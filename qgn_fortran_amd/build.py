@@ -41,17 +41,27 @@ def build_lib(verbose=False) -> str:
     """Compile csrc/*.hip and csrc/*.cpp into libqgn.so (gfx950)."""
     objdir = os.path.join(PKG, "_build")
     os.makedirs(objdir, exist_ok=True)
-    objs = []
+    objs, cmds = [], []
     for name in sorted(os.listdir(CSRC)):
         src = os.path.join(CSRC, name)
         if name.endswith(".hip"):
             obj = os.path.join(objdir, name + ".o")
-            _run([HIPCC, *DEVICE_FLAGS, "-I", INCLUDE, "-I", CSRC, "-c", src, "-o", obj])
+            cmds.append([HIPCC, *DEVICE_FLAGS, "-I", INCLUDE, "-I", CSRC, "-c", src, "-o", obj])
             objs.append(obj)
         elif name.endswith(".cpp"):
             obj = os.path.join(objdir, name + ".o")
-            _run(["g++", "-O2", "-std=c++17", "-fPIC", "-I", INCLUDE, "-c", src, "-o", obj])
+            cmds.append(["g++", "-O2", "-std=c++17", "-fPIC", "-I", INCLUDE, "-c", src, "-o", obj])
             objs.append(obj)
+    # the translation units compile in parallel
+    procs = [subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True) for cmd in cmds]
+    failed = None
+    for cmd, pr in zip(cmds, procs):
+        out, _ = pr.communicate()
+        if pr.returncode != 0 and failed is None:
+            sys.stderr.write(out)
+            failed = cmd
+    if failed:
+        raise RuntimeError("build step failed: " + " ".join(failed))
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, LIB)

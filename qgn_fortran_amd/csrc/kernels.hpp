@@ -49,6 +49,24 @@ template <typename R> __device__ __forceinline__ R ky_r(int j, const Geom& g) {
   return (R)g.dky * (R)(j <= g.ny / 2 ? j : j - g.ny);
 }
 
+// Mixed arrays (y physical, kx spectral; nz planes of ny rows of Kxp bins).
+// FP32 planes store their rows in interleaved pairs: bin k of row y sits at
+// ((y/2) Kxp + k) 2 + y%2, so a column tile of 8 FP32 columns moves whole
+// 128-byte lines (two rows of 64 bytes) and a row pass over two rows reads
+// one contiguous block.  Against the plain row-major layout this takes the
+// column passes from 64-byte (8 FP32 columns) or 32-byte (4) row segments,
+// which the HBM serves at about 4.8 TB/s on MI355X in a copy of P1's
+// shape, to whole lines (6.7 TB/s).  FP64 planes stay row-major.  Offsets
+// are relative to the start of an even row (every plane, row group and tile
+// starts on one).
+template <typename R> constexpr bool mix_pairs() { return sizeof(R) == 4; }
+template <typename R> __device__ __forceinline__ int mix_at(int y, int k, int Kxp) {
+  if constexpr (mix_pairs<R>()) return ((y >> 1) * Kxp + k) * 2 + (y & 1);
+  else return y * Kxp + k;
+}
+// distance between bins k and k+1 of one row
+template <typename R> constexpr int mix_kstep() { return mix_pairs<R>() ? 2 : 1; }
+
 template <typename T> __device__ __forceinline__ cplx<T> to_c(double2 a) { return cmk<cplx<T>>((T)a.x, (T)a.y); }
 template <typename T> __device__ __forceinline__ cplx<T> to_c(float2 a) { return cmk<cplx<T>>((T)a.x, (T)a.y); }
 
@@ -231,16 +249,16 @@ __device__ __forceinline__ cplx<R> c2r_pre(const cplx<R>* __restrict__ row, int 
   return c2r_pre_ab<R>(a, b, k, twr, scale);
 }
 
-// The same, the row at element offset rel (a lane-dependent row of a
-// workgroup's block) of the buffer rs: out-of-range bins load as zero.
+// The same, mixed row r (a lane-dependent row of a workgroup's block of
+// rows, mix_at layout) of the buffer rs: out-of-range bins load as zero.
 template <typename R>
-__device__ __forceinline__ cplx<R> c2r_pre_rs(__amdgpu_buffer_rsrc_t rs, long rel, int Kin, int k, int N2,
+__device__ __forceinline__ cplx<R> c2r_pre_rs(__amdgpu_buffer_rsrc_t rs, int r, int Kxp, int Kin, int k, int N2,
                                               const cplx<R>* __restrict__ twr, R scale) {
   using Cx = cplx<R>;
   constexpr int SZ = (int)sizeof(Cx);
   const int kk = N2 - k;
-  const Cx a = rsrc_ld<Cx>(rs, k < Kin ? (int)((rel + k) * SZ) : kRowOob);
-  const Cx b = rsrc_ld<Cx>(rs, kk < Kin ? (int)((rel + kk) * SZ) : kRowOob);
+  const Cx a = rsrc_ld<Cx>(rs, k < Kin ? mix_at<R>(r, k, Kxp) * SZ : kRowOob);
+  const Cx b = rsrc_ld<Cx>(rs, kk < Kin ? mix_at<R>(r, kk, Kxp) * SZ : kRowOob);
   return c2r_pre_ab<R>(a, b, k, twr, scale);
 }
 
@@ -311,15 +329,17 @@ __device__ __forceinline__ double2 stage_combine(int n, const double* ae, const 
 }
 
 // lam = A2 k2 + A8 k2^4 of compact point pt, layer kl of the batch
-__device__ __forceinline__ double visc_lambda(const Geom& g, long pt, int kl, const double* a8l, double A8,
-                                              double A2) {
-  const int i = (int)pt % g.Kxp;
-  const int j = compact_row_to_full((int)pt / g.Kxp, g);
+__device__ __forceinline__ double visc_lambda_ij(const Geom& g, int i, int j, int kl, const double* a8l, double A8,
+                                                 double A2) {
   const double kx = kx_of(i, g), ky = ky_of(j, g);
   const double k2 = kx * kx + ky * ky;
   const double k4 = k2 * k2;
   const double a8 = a8l ? a8l[kl] : A8;
   return A2 * k2 + a8 * (k4 * k4);
+}
+__device__ __forceinline__ double visc_lambda(const Geom& g, long pt, int kl, const double* a8l, double A8,
+                                              double A2) {
+  return visc_lambda_ij(g, (int)pt % g.Kxp, compact_row_to_full((int)pt / g.Kxp, g), kl, a8l, A8, A2);
 }
 
 // ARK stage value of element e (layer kl of the batch, point pt of its
@@ -329,9 +349,18 @@ __device__ __forceinline__ double visc_lambda(const Geom& g, long pt, int kl, co
 // with Mq = 1 and no L_j without implicit viscosity.  NS, the number of
 // previous stages, is a compile-time count, so all of the element's loads
 // are issued before the first sum.
+// The same with the element's kx index i and full ky row j given.
+template <typename S, int NS>
+__device__ __forceinline__ double2 stage_value_ij(const StageArgs& a, const Geom& g, long e, int i, int j, int kl,
+                                                  double dt);
 template <typename S, int NS>
 __device__ __forceinline__ double2 stage_value(const StageArgs& a, const Geom& g, long e, long pt, int kl,
                                                double dt) {
+  return stage_value_ij<S, NS>(a, g, e, (int)pt % g.Kxp, compact_row_to_full((int)pt / g.Kxp, g), kl, dt);
+}
+template <typename S, int NS>
+__device__ __forceinline__ double2 stage_value_ij(const StageArgs& a, const Geom& g, long e, int i, int j, int kl,
+                                                  double dt) {
   constexpr int M = NS > 0 ? NS : 1;
   const double2 q = a.q[e];
   cplx<S> nv[M];
@@ -348,7 +377,7 @@ __device__ __forceinline__ double2 stage_value(const StageArgs& a, const Geom& g
   for (int jj = 0; jj < NS; ++jj) nd[jj] = make_double2((double)nv[jj].x, (double)nv[jj].y);
   double lam = 0.0, mq = 1.0;
   if (a.implicit_visc) {
-    lam = visc_lambda(g, pt, kl, a.a8l, a.A8, a.A2);
+    lam = visc_lambda_ij(g, i, j, kl, a.a8l, a.A8, a.A2);
     mq = 1.0 / (1.0 + 0.25 * dt * lam);
     if (rec) {
       yv[0] = q;
@@ -465,94 +494,105 @@ __device__ __forceinline__ qgn_float4 mfma3(const qgn_short4& ah, const qgn_shor
   return c;
 }
 
-// Stage combination (or, with a.ns = 0 and a.out = nullptr, a plain copy of
-// q) followed by the MFMA projection, nz <= 16.  A group is T tiles of 32
-// points x 16 layers; workgroups walk the groups (grid-stride: the grid
-// holds as many workgroups as fit on the GPU at once, so the last round
-// leaves no CU idle).  Per group: 2T (point, layer) elements per thread for
-// the FP64 stage sums (consecutive threads on consecutive points: 512
-// contiguous bytes of q per layer), all of a thread's loads issued before
-// its first store; then, per tile, each of the 4 waves projects 8 points
-// (16 columns: re, im); psi goes out through LDS, 16 bytes per thread over
-// whole 256-byte rows of a layer.  Every wave builds the split matrix
-// operands once per workgroup.  T > 1 keeps more bytes in flight where a
-// stage has few input arrays.
-template <typename S, int NS, int T>
-__global__ void __launch_bounds__(256) k_stage_proj_mfma16(StageArgs a, Geom g, const double* __restrict__ HM,
-                                                         const double* __restrict__ MD,
-                                                         const double* __restrict__ evals, double inv_htot,
-                                                         float2* __restrict__ psi) {
-  __shared__ double2 sq[T][16][32];    // [tile][layer][point]; then psi as float2 [tile][layer][point]
+// The mixed path's stage values alone, as a streaming pass (the projection
+// follows in k_proj32_mfma16): block (Kxp/2, SR), thread (x, y) takes
+// columns 2x, 2x+1 of compact row blockIdx.x SR + y of layer blockIdx.y of
+// the batch, so the wavenumbers need no index division.  Writes the stage
+// value rounded to FP32 (and the FP64 value where a.out is set); every
+// stage kernel forms the sums with stage_value_ij, so the values agree to
+// the bit.
+template <int NS>
+__global__ void __launch_bounds__(1024) k_stage_ew(StageArgs a, Geom g) {
+  const int z = blockIdx.y;
+  const int jc = blockIdx.x * blockDim.y + threadIdx.y;
+  if (jc >= g.Ky) return;
+  const int i = 2 * threadIdx.x;
+  const int j = compact_row_to_full(jc, g);
+  const long e = ((long)z * g.Ky + jc) * g.Kxp + i;
+  const double dt = a.dtv ? a.dtv[z / g.nzm] : a.dt;
+  const double2 q0 = stage_value_ij<float, NS>(a, g, e, i, j, z, dt);
+  const double2 q1 = stage_value_ij<float, NS>(a, g, e + 1, i + 1, j, z, dt);
+  if (a.out) {
+    a.out[e] = q0;
+    a.out[e + 1] = q1;
+  }
+  *reinterpret_cast<float4*>(a.out32 + e) = make_float4((float)q0.x, (float)q0.y, (float)q1.x, (float)q1.y);
+}
+
+// GetPsi of the FP32 stage value q32 (nz <= 16) as a split-BF16 MFMA
+// projection (v_mfma_f32_16x16x16bf16_1k): each FP32 operand x is split
+// into hi = bf16(x) and lo = bf16(x - hi), and a product accumulates
+// hi*hi + hi*lo + lo*hi in FP32.  Per tile of 32 points x 16 layers each
+// of the 4 waves projects 8 points (16 columns: re, im): modal amplitudes
+// D1[mode][col] = (HM/Htot)[mode][layer] Q[layer][col], the modal Helmholtz
+// inversion D1 / (lambda_mode - k^2), then psi[layer][col] = Modes D.
+// Groups of T tiles are walked grid-stride, the next group's loads in
+// flight while this one is projected; psi goes out through LDS in whole
+// 256-byte rows of a layer.
+template <int T>
+__global__ void __launch_bounds__(256) k_proj32_mfma16(const float2* __restrict__ q32, Geom g,
+                                                       const double* __restrict__ HM, const double* __restrict__ MD,
+                                                       const double* __restrict__ evals, double inv_htot,
+                                                       float2* __restrict__ psi) {
+  __shared__ float2 sq[T][16][32];    // [tile][layer][point]: q32, then psi
   const int nz = g.nzm;
   const int mb = blockIdx.y;         // member
   HM += mb * nz * nz;
   MD += mb * nz * nz;
   evals += mb * nz;
-  const long moff = (long)mb * nz * g.Ky * g.Kxp;   // first element of the member
-  const double dt = a.dtv ? a.dtv[mb] : a.dt;
+  const long moff = (long)mb * nz * g.Ky * g.Kxp;
   const int tid = threadIdx.x;
   const long plane = (long)g.Ky * g.Kxp;
   const long ngroups = (plane + 32 * T - 1) / (32 * T);
   const int l = tid & 63;
-  const int wv = tid >> 6;                    // wave: points 8*wv .. 8*wv+7 of each tile
-  const int row = l & 15, kb = 4 * (l >> 4);  // A: i = row; B, D: j = row
+  const int wv = tid >> 6;
+  const int row = l & 15, kb = 4 * (l >> 4);
   qgn_short4 a1h, a1l, a2h, a2l, bh, bl;
   double ev[4];
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const int kk = kb + r;
-    const float m1 = (row < nz && kk < nz) ? (float)(HM[row * nz + kk] * inv_htot) : 0.f;  // (HM/Htot)[mode][layer]
-    const float m2 = (row < nz && kk < nz) ? (float)MD[row * nz + kk] : 0.f;               // Modes[layer][mode]
+    const float m1 = (row < nz && kk < nz) ? (float)(HM[row * nz + kk] * inv_htot) : 0.f;
+    const float m2 = (row < nz && kk < nz) ? (float)MD[row * nz + kk] : 0.f;
     a1h[r] = bf16_rne(m1);
     a1l[r] = bf16_lo(m1, a1h[r]);
     a2h[r] = bf16_rne(m2);
     a2l[r] = bf16_lo(m2, a2h[r]);
     ev[r] = kk < nz ? evals[kk] : 1.0;
   }
-  float2* spsi = reinterpret_cast<float2*>(&sq[0][0][0]);   // [T][16][32] after the operands are read
-  for (long grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
-    const long p0 = grp * (32 * T);             // first point of the group
-    {
-      const int pp = tid & 31, k0 = tid >> 5;   // layers k0 and k0 + 8
-      double2 qt[T][2];
+  // tile t, layer k of a group: 32 points = 256 bytes; 16 threads per row,
+  // 2 points each (plane is even, so pairs never straddle its end).  The
+  // next group's loads are in flight while this group is projected.
+  const int lk = tid >> 4, lp2 = 2 * (tid & 15);
+  float4 vn[T];
+  auto fetch = [&](long grp) {
 #pragma unroll
-      for (int t = 0; t < T; ++t) {
-        const long pt = p0 + 32 * t + pp;
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          const int k = k0 + 8 * h;
-          qt[t][h] = make_double2(0.0, 0.0);
-          if (k < nz && pt < plane) qt[t][h] = stage_value<S, NS>(a, g, moff + k * plane + pt, pt, mb * nz + k, dt);
-        }
-      }
-      __syncthreads();   // the previous group's psi rows have been read out of sq
-#pragma unroll
-      for (int t = 0; t < T; ++t) {
-        const long pt = p0 + 32 * t + pp;
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          const int k = k0 + 8 * h;
-          if (k < nz && pt < plane) {
-            if (a.out) a.out[moff + k * plane + pt] = qt[t][h];
-            if (a.out32) a.out32[moff + k * plane + pt] = make_float2((float)qt[t][h].x, (float)qt[t][h].y);
-          }
-          sq[t][k][pp] = qt[t][h];
-        }
-      }
+    for (int t = 0; t < T; ++t) {
+      const long pt = grp * (32 * T) + 32 * t + lp2;
+      vn[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (lk < nz && pt < plane) vn[t] = *reinterpret_cast<const float4*>(q32 + moff + lk * plane + pt);
     }
+  };
+  if (blockIdx.x < ngroups) fetch(blockIdx.x);
+  for (long grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+    const long p0 = grp * (32 * T);
+    __syncthreads();   // the previous group's psi rows have been read out of sq
+#pragma unroll
+    for (int t = 0; t < T; ++t) *reinterpret_cast<float4*>(&sq[t][lk][lp2]) = vn[t];
+    if (grp + gridDim.x < ngroups) fetch(grp + gridDim.x);
     __syncthreads();
     qgn_float4 pv[T];
 #pragma unroll
     for (int t = 0; t < T; ++t) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const double2 v = sq[t][kb + r][8 * wv + (row >> 1)];
-        const float x = (float)((row & 1) ? v.y : v.x);
+        const float2 v = sq[t][kb + r][8 * wv + (row >> 1)];
+        const float x = (row & 1) ? v.y : v.x;
         bh[r] = bf16_rne(x);
         bl[r] = bf16_lo(x, bh[r]);
       }
       qgn_float4 d = {0.f, 0.f, 0.f, 0.f};
-      d = mfma3(a1h, a1l, bh, bl, d);            // modal amplitudes, D[mode][column]
+      d = mfma3(a1h, a1l, bh, bl, d);
       const long pt = p0 + 32 * t + 8 * wv + (row >> 1);
       bool zero_mode = pt >= plane;
       double k2 = 0.0;
@@ -571,22 +611,21 @@ __global__ void __launch_bounds__(256) k_stage_proj_mfma16(StageArgs a, Geom g, 
         bl[r] = bf16_lo(pm, bh[r]);
       }
       qgn_float4 pz = {0.f, 0.f, 0.f, 0.f};
-      pv[t] = mfma3(a2h, a2l, bh, bl, pz);       // D[layer][column]
+      pv[t] = mfma3(a2h, a2l, bh, bl, pz);
     }
     __syncthreads();   // every wave has read its B operands out of sq
-    float* sp = reinterpret_cast<float*>(spsi);
+    float* sp = reinterpret_cast<float*>(&sq[0][0][0]);
 #pragma unroll
     for (int t = 0; t < T; ++t)
 #pragma unroll
       for (int r = 0; r < 4; ++r) sp[2 * ((t * 16 + kb + r) * 32 + 8 * wv + (row >> 1)) + (row & 1)] = pv[t][r];
     __syncthreads();
-    // tile t, layer k: 32 points = 256 bytes; 16 threads per row, 2 points each
 #pragma unroll
     for (int t = 0; t < T; ++t) {
       const int k = tid >> 4, pp2 = 2 * (tid & 15);
       const long pt = p0 + 32 * t + pp2;
       if (k < nz && pt < plane) {
-        const float4 v = *reinterpret_cast<const float4*>(&spsi[(t * 16 + k) * 32 + pp2]);
+        const float4 v = *reinterpret_cast<const float4*>(&sq[t][k][pp2]);
         float2* dst = psi + moff + k * plane + pt;
         if (pt + 1 < plane) *reinterpret_cast<float4*>(dst) = v;
         else *dst = make_float2(v.x, v.y);
@@ -595,10 +634,6 @@ __global__ void __launch_bounds__(256) k_stage_proj_mfma16(StageArgs a, Geom g, 
   }
 }
 
-// tiles per group of k_stage_proj_mfma16 for a stage with NS earlier
-// stages (measured at 1024^2 x 16: 4 and 2; one tile per group for NS >= 3
-// takes more registers, not fewer)
-template <int NS> struct ProjTiles { static constexpr int T = NS == 0 ? 4 : 2; };
 
 // Stage combination (or, with a.ns = 0 and a.out = nullptr, a plain copy of
 // q) followed by the MFMA projection, for nz <= NZT layers (NZT = 16 or 32:

@@ -107,17 +107,19 @@ __global__ void __launch_bounds__(NTHR) k_col_inv_ct(ColInvArgs a, Geom g, const
     }
     __syncthreads();
     ct_fft<R, NY, L, NTHR>(buf, tw, +1);
-    Cx* dst = reinterpret_cast<Cx*>(a.dst[f]);
-    if constexpr (PAIR) {
+    Cx* dst = reinterpret_cast<Cx*>(a.dst[f]) + (long)z * NY * g.Kxp;
+    if constexpr (mix_pairs<R>()) {
+      // one 16-byte store per column and row pair (rows 2p, 2p+1 of column
+      // c are adjacent in the mixed layout): 8 columns fill a 128-byte line
       for (int e2 = threadIdx.x; 2 * e2 < TOT; e2 += NTHR) {
-        const int e = 2 * e2, c = e % C, j = e / C;
-        *reinterpret_cast<float4*>(dst + ((long)z * NY + j) * g.Kxp + i0 + c) =
-            reinterpret_cast<const float4*>(buf)[e2];
+        const int c = e2 % C, p = e2 / C;
+        const Cx lo = buf[2 * p * C + c], hi = buf[(2 * p + 1) * C + c];
+        *reinterpret_cast<float4*>(dst + mix_at<R>(2 * p, i0 + c, g.Kxp)) = make_float4(lo.x, lo.y, hi.x, hi.y);
       }
     } else {
       for (int e = threadIdx.x; e < TOT; e += NTHR) {
         const int c = e % C, j = e / C;
-        dst[((long)z * NY + j) * g.Kxp + i0 + c] = buf[e];
+        dst[mix_at<R>(j, i0 + c, g.Kxp)] = buf[e];
       }
     }
     __syncthreads();
@@ -173,9 +175,8 @@ __global__ void __launch_bounds__(NTHR) k_row_jac_ct(RowJacArgs a, Geom g, const
       const int e = threadIdx.x + b * NTHR;
       const int r = e / N2, k = e - r * N2;
       const bool ok = (ROWS * N2) % NTHR == 0 || e < ROWS * N2;
-      const int rel = r * (int)Kxp;
-      const int oa = (ok && k < Kx) ? (rel + k) * (int)sizeof(Cx) : kRowOob;
-      const int ob = (ok && N2 - k < Kx) ? (rel + N2 - k) * (int)sizeof(Cx) : kRowOob;
+      const int oa = (ok && k < Kx) ? mix_at<R>(r, k, (int)Kxp) * (int)sizeof(Cx) : kRowOob;
+      const int ob = (ok && N2 - k < Kx) ? mix_at<R>(r, N2 - k, (int)Kxp) * (int)sizeof(Cx) : kRowOob;
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
         const __amdgpu_buffer_rsrc_t rs = row_rsrc(srcs[s] + row0 * Kxp, ROWS * (int)Kxp * (int)sizeof(Cx));
@@ -224,18 +225,99 @@ __global__ void __launch_bounds__(NTHR) k_row_jac_ct(RowJacArgs a, Geom g, const
     Cx* jo = reinterpret_cast<Cx*>(a.J) + (long)row0 * Kxp;
     for (int e = threadIdx.x; e < ROWS * Kx; e += NTHR) {
       const int r = e / Kx, k = e - r * Kx;
-      jo[(long)r * Kxp + k] = r2c_post_pad<R, PS>(buf, 4 * r * N2, k, N2, tr);
+      jo[mix_at<R>(r, k, (int)Kxp)] = r2c_post_pad<R, PS>(buf, 4 * r * N2, k, N2, tr);
     }
     if (bottom) {
       Cx* du = reinterpret_cast<Cx*>(a.DU) + (long)((z / g.nzm) * g.ny + j0) * Kxp;
       Cx* dv = reinterpret_cast<Cx*>(a.DV) + (long)((z / g.nzm) * g.ny + j0) * Kxp;
       for (int e = threadIdx.x; e < ROWS * Kx; e += NTHR) {
         const int r = e / Kx, k = e - r * Kx;
-        du[(long)r * Kxp + k] = r2c_post_pad<R, PS>(buf, (4 * r + 1) * N2, k, N2, tr);
-        dv[(long)r * Kxp + k] = r2c_post_pad<R, PS>(buf, (4 * r + 2) * N2, k, N2, tr);
+        du[mix_at<R>(r, k, (int)Kxp)] = r2c_post_pad<R, PS>(buf, (4 * r + 1) * N2, k, N2, tr);
+        dv[mix_at<R>(r, k, (int)Kxp)] = r2c_post_pad<R, PS>(buf, (4 * r + 2) * N2, k, N2, tr);
       }
     }
     __syncthreads();   // buf is refilled by the next group
+  }
+}
+
+// Radix-RAD Stockham pass over the row sequences of the fused row pass:
+// ROWS rows of N2 values, row r's element i at padded<PS>(r N2 + i).  Every
+// LDS address is a per-thread base plus a compile-time offset: with
+// 2^PS | d, padded(x + d) = padded(x) + d + d / 2^PS, which covers the reads
+// (stride N2/RAD) and the writes of passes with 2^PS | P; a write stride of
+// P = 8 under PS = 4 adds the x-dependent carry (x/8 & 1 + m) / 2, which
+// takes two bases.  Same arithmetic as ct_pass on the same layout.
+template <int PS> constexpr int pad_step(int d) { return d + (d >> PS); }
+
+template <typename R, int N2, int RAD, int P, int NTHR, int PS, int OFF>
+__device__ __forceinline__ void row_pass(cplx<R>* buf, const cplx<R>* __restrict__ tw, int sign) {
+  using C = cplx<R>;
+  constexpr int NB = N2 / RAD;
+  constexpr int G = NB * (NTHR / (N2 / 8));   // butterflies: NB per row, NTHR / (N2/8) rows
+  constexpr int MAXB = G / NTHR;
+  static_assert(G % NTHR == 0 && NB % (1 << PS) == 0, "row pass shape");
+  const int tid = threadIdx.x;
+  C v[MAXB][RAD];
+  int wbase[MAXB];
+#pragma unroll
+  for (int b = 0; b < MAXB; ++b) {
+    const int gg = tid + b * NTHR;
+    const int t = gg % NB, s = gg / NB;
+    const int k = t % P;
+    const C* rp = buf + padded<PS>(s * N2 + t);
+#pragma unroll
+    for (int j = 0; j < RAD; ++j) v[b][j] = rp[j * pad_step<PS>(NB)];
+    if constexpr (P > 1) {
+      const C* tp = tw + OFF + k;
+#pragma unroll
+      for (int j = 1; j < RAD; ++j) {
+        C w = tp[(j - 1) * P];
+        if (sign > 0) w.y = -w.y;
+        v[b][j] = cmul(v[b][j], w);
+      }
+    }
+    Dft<C, RAD>::run(v[b], sign);
+    wbase[b] = s * N2 + (t - k) * RAD + k;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < MAXB; ++b) {
+    const int x = wbase[b];
+    if constexpr (P % (1 << PS) == 0) {
+      C* wp = buf + padded<PS>(x);
+#pragma unroll
+      for (int m = 0; m < RAD; ++m) wp[m * pad_step<PS>(P)] = v[b][m];
+    } else if constexpr (P == 8 && PS == 4) {
+      // padded(x + 8m) = padded(x) + 8m + (((x >> 3) & 1) + m) / 2
+      C* w0 = buf + padded<PS>(x);
+      C* w1 = w0 + ((x >> 3) & 1);
+#pragma unroll
+      for (int m = 0; m < RAD; ++m) {
+        if (m & 1) w1[8 * m + (m >> 1)] = v[b][m];
+        else w0[8 * m + (m >> 1)] = v[b][m];
+      }
+    } else {
+#pragma unroll
+      for (int m = 0; m < RAD; ++m) buf[padded<PS>(x + m * P)] = v[b][m];
+    }
+  }
+  __syncthreads();
+}
+
+// Passes [PASS, END) of the length-N2 plan on the row layout; passes whose
+// butterflies do not divide evenly over the threads (radix 3, 5) or whose
+// read stride is not a multiple of 2^PS take ct_pass.
+template <typename R, int N2, int PASS, int END, int P, int NTHR, int PS>
+__device__ __forceinline__ void row_passes_range(cplx<R>* buf, const cplx<R>* __restrict__ tw, int sign) {
+  constexpr CtPlan PL = make_ct_plan(N2);
+  if constexpr (PASS < END) {
+    constexpr int RAD = PL.rad[PASS];
+    constexpr int NB = N2 / RAD, ROWS = NTHR / (N2 / 8);
+    if constexpr ((NB * ROWS) % NTHR == 0 && NB % (1 << PS) == 0)
+      row_pass<R, N2, RAD, P, NTHR, PS, ct_tw_offset(N2, PASS)>(buf, tw, sign);
+    else
+      ct_pass<R, N2, RAD, P, CtLayout<ROWS, N2, 1, false, PS>, NTHR, ct_tw_offset(N2, PASS)>(buf, tw, sign);
+    row_passes_range<R, N2, PASS + 1, END, P * RAD, NTHR, PS>(buf, tw, sign);
   }
 }
 
@@ -247,7 +329,8 @@ __global__ void __launch_bounds__(NTHR) k_row_jac_ct(RowJacArgs a, Geom g, const
 // takes them, so the products skip an LDS round trip.  u and v are
 // transformed first (the bottom layer's |u| u and |u| v go out right away),
 // then q_x and q_y are folded into J = u q_x + v q_y one at a time, so at
-// most two fields and the accumulator are live.
+// most two fields and the accumulator are live.  LDS addresses are bases
+// plus compile-time offsets throughout (row_pass).
 template <typename R, int N2, int ROWS>
 __global__ void __launch_bounds__(ROWS * (N2 / 8)) k_row_jac_fused2(RowJacArgs a, Geom g,
                                                                    const cplx<R>* __restrict__ twx,
@@ -260,7 +343,10 @@ __global__ void __launch_bounds__(ROWS * (N2 / 8)) k_row_jac_fused2(RowJacArgs a
   constexpr int PS = row_pad_shift<R>();
   constexpr int LAST = PL.np - 1;
   constexpr int OFF_LAST = ct_tw_offset(N2, LAST);
-  using L1 = CtLayout<ROWS, N2, 1, false, PS>;
+  static_assert(N2 % (1 << PS) == 0, "row layout");
+  // the last pass's reads (stride NB) and the output loop's (stride NB) as
+  // base + offset when 2^PS | NB, else in full
+  constexpr bool AFF = NB % (1 << PS) == 0;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   Cx* buf = reinterpret_cast<Cx*>(lds_raw);
   Cx* ts = buf + row_lds_slots<R>(ROWS * N2);
@@ -276,32 +362,36 @@ __global__ void __launch_bounds__(ROWS * (N2 / 8)) k_row_jac_fused2(RowJacArgs a
   const int j0 = row0 - z * g.ny;
   const bool bottom = a.drag && (z % g.nzm == g.nzm - 1);
   const int Kx = g.Kx;
-  const long Kxp = g.Kxp;
-  const long rel = (long)r * Kxp;   // this thread's row within the group
+  const int Kxp = g.Kxp;
+  // this thread's slots: the first pass's outputs 8t .. 8t+7 of row r, and
+  // the last pass's inputs t + j NB
+  Cx* const w8p = buf + padded<PS>(r * N2 + 8 * t);   // padded(r N2 + 8t + m) = this + m, m < 8
+  Cx* const lp = buf + padded<PS>(r * N2 + t);
+  const Cx* const twl = ts + OFF_LAST + t;
   // x-c2r of one mixed row into registers: o[j] holds x[2m], x[2m+1] at m = t + j NB,
   // unnormalised (every product below is of two such fields: s2 = scale^2
   // normalises it once)
   const R s2 = scale * scale;
   auto inverse = [&](const void* src, Cx(&o)[8]) {
     const __amdgpu_buffer_rsrc_t rs =
-        row_rsrc(reinterpret_cast<const Cx*>(src) + (long)row0 * Kxp, ROWS * (int)Kxp * (int)sizeof(Cx));
+        row_rsrc(reinterpret_cast<const Cx*>(src) + (long)row0 * Kxp, ROWS * Kxp * (int)sizeof(Cx));
     Cx v[8];
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       const int k = t + j * NB;
-      v[j] = c2r_pre_rs<R>(rs, rel, Kx, k, N2, tr, (R)1);
+      v[j] = c2r_pre_rs<R>(rs, r, Kxp, Kx, k, N2, tr, (R)1);
     }
     Dft<Cx, 8>::run(v, +1);
     __syncthreads();   // earlier readers of buf are done
 #pragma unroll
-    for (int m = 0; m < 8; ++m) buf[padded<PS>(r * N2 + 8 * t + m)] = v[m];
+    for (int m = 0; m < 8; ++m) w8p[m] = v[m];
     __syncthreads();
-    ct_passes_range<R, N2, 1, LAST, 8, L1, NTHR>(buf, ts, +1);
+    row_passes_range<R, N2, 1, LAST, 8, NTHR, PS>(buf, ts, +1);
 #pragma unroll
-    for (int j = 0; j < 8; ++j) o[j] = buf[padded<PS>(r * N2 + t + j * NB)];
+    for (int j = 0; j < 8; ++j) o[j] = AFF ? lp[j * pad_step<PS>(NB)] : buf[padded<PS>(r * N2 + t + j * NB)];
 #pragma unroll
     for (int j = 1; j < 8; ++j) {
-      Cx w = ts[OFF_LAST + (j - 1) * NB + t];
+      Cx w = twl[(j - 1) * NB];
       w.y = -w.y;
       o[j] = cmul(o[j], w);
     }
@@ -312,12 +402,29 @@ __global__ void __launch_bounds__(ROWS * (N2 / 8)) k_row_jac_fused2(RowJacArgs a
     Dft<Cx, 8>::run(w8, -1);
     __syncthreads();
 #pragma unroll
-    for (int m = 0; m < 8; ++m) buf[padded<PS>(r * N2 + 8 * t + m)] = w8[m];
+    for (int m = 0; m < 8; ++m) w8p[m] = w8[m];
     __syncthreads();
-    ct_passes_range<R, N2, 1, PL.np, 8, L1, NTHR>(buf, ts, -1);
-    for (int e = threadIdx.x; e < ROWS * Kx; e += NTHR) {
-      const int rr = e / Kx, k = e - rr * Kx;
-      out[(long)rr * Kxp + k] = r2c_post_pad<R, PS>(buf, rr * N2, k, N2, tr);
+    row_passes_range<R, N2, 1, PL.np, 8, NTHR, PS>(buf, ts, -1);
+    // rows fastest: with row pairs interleaved, neighbouring lanes store
+    // neighbouring 8-byte words.  Thread e = tid + i NTHR takes bin
+    // k = e / ROWS of row e % ROWS; its LDS reads step by pad_step(NTHR / ROWS)
+    constexpr int KSTEP = NTHR / ROWS;
+    constexpr int NIT = (ROWS * (2 * N2 / 3 + 1) + NTHR - 1) / NTHR;   // Kx <= 2 N2 / 3 + 1
+    const int rr = threadIdx.x % ROWS, k0 = threadIdx.x / ROWS;
+    const Cx* za = buf + padded<PS>(rr * N2 + k0);            // Z[k], k = k0 + i KSTEP
+    const Cx* zb = buf + padded<PS>(rr * N2 + N2 - k0);       // Z[N2 - k]
+    const Cx* z0 = buf + padded<PS>(rr * N2);                 // Z[0] (for k = 0)
+#pragma unroll
+    for (int i = 0; i < NIT; ++i) {
+      const int k = k0 + i * KSTEP;
+      if (k < Kx) {
+        const Cx aa = AFF ? za[i * pad_step<PS>(KSTEP)] : buf[padded<PS>(rr * N2 + k)];
+        const Cx bb = k == 0 ? *z0 : (AFF ? zb[-i * pad_step<PS>(KSTEP)] : buf[padded<PS>(rr * N2 + N2 - k)]);
+        const Cx e = cmk<Cx>((R)0.5 * (aa.x + bb.x), (R)0.5 * (aa.y - bb.y));
+        const Cx dd = cmk<Cx>((R)0.5 * (aa.x - bb.x), (R)0.5 * (aa.y + bb.y));
+        const Cx o = cmk<Cx>(dd.y, -dd.x);
+        out[mix_at<R>(rr, k, Kxp)] = cadd(e, cmul(tr[k], o));
+      }
     }
   };
   __syncthreads();
@@ -400,19 +507,23 @@ __device__ __forceinline__ void rhs_tile(int b, int total, int ntile, int nzm, i
 // rows of `stride` complex values; each thread's loads are all issued before
 // its LDS stores.
 template <typename R, int C, int NTHR, int TOT>
-__device__ __forceinline__ void load_cols(cplx<R>* buf, const cplx<R>* __restrict__ src, long stride, int i0) {
+__device__ __forceinline__ void load_cols(cplx<R>* buf, const cplx<R>* __restrict__ src, int Kxp, int i0) {
   constexpr int EPT = (TOT + NTHR - 1) / NTHR;
   constexpr bool EXACT = EPT * NTHR == TOT;
-  if constexpr (sizeof(cplx<R>) == 8 && C % 2 == 0 && EPT % 2 == 0 && EXACT) {
-    // FP32: two adjacent columns per 16-byte load and LDS store
+  if constexpr (mix_pairs<R>() && EPT % 2 == 0 && EXACT) {
+    // FP32: rows 2p and 2p+1 of column c per 16-byte load
     float4 v[EPT / 2];
 #pragma unroll
     for (int k = 0; k < EPT / 2; ++k) {
-      const int e = 2 * ((int)threadIdx.x + k * NTHR);
-      v[k] = *reinterpret_cast<const float4*>(src + (long)(e / C) * stride + i0 + e % C);
+      const int e2 = (int)threadIdx.x + k * NTHR, c = e2 % C, p = e2 / C;
+      v[k] = *reinterpret_cast<const float4*>(src + mix_at<R>(2 * p, i0 + c, Kxp));
     }
 #pragma unroll
-    for (int k = 0; k < EPT / 2; ++k) reinterpret_cast<float4*>(buf)[threadIdx.x + k * NTHR] = v[k];
+    for (int k = 0; k < EPT / 2; ++k) {
+      const int e2 = (int)threadIdx.x + k * NTHR, c = e2 % C, p = e2 / C;
+      buf[2 * p * C + c] = cmk<cplx<R>>(v[k].x, v[k].y);
+      buf[(2 * p + 1) * C + c] = cmk<cplx<R>>(v[k].z, v[k].w);
+    }
     return;
   }
   cplx<R> v[EPT];
@@ -420,7 +531,7 @@ __device__ __forceinline__ void load_cols(cplx<R>* buf, const cplx<R>* __restric
   for (int u = 0; u < EPT; ++u) {
     const int e = threadIdx.x + u * NTHR;
     const int ee = (EXACT || e < TOT) ? e : TOT - 1;
-    v[u] = src[(long)(ee / C) * stride + i0 + ee % C];
+    v[u] = src[mix_at<R>(ee / C, i0 + ee % C, Kxp)];
   }
 #pragma unroll
   for (int u = 0; u < EPT; ++u) {
@@ -432,22 +543,24 @@ __device__ __forceinline__ void load_cols(cplx<R>* buf, const cplx<R>* __restric
 // The same tile of a - b.
 template <typename R, int C, int NTHR, int TOT>
 __device__ __forceinline__ void load_cols_sub(cplx<R>* buf, const cplx<R>* __restrict__ a,
-                                              const cplx<R>* __restrict__ b, long stride, int i0) {
+                                              const cplx<R>* __restrict__ b, int Kxp, int i0) {
   constexpr int EPT = (TOT + NTHR - 1) / NTHR;
   constexpr bool EXACT = EPT * NTHR == TOT;
-  if constexpr (sizeof(cplx<R>) == 8 && C % 2 == 0 && EPT % 2 == 0 && EXACT) {
+  if constexpr (mix_pairs<R>() && EPT % 2 == 0 && EXACT) {
     float4 v[EPT / 2], w[EPT / 2];
 #pragma unroll
     for (int k = 0; k < EPT / 2; ++k) {
-      const int e = 2 * ((int)threadIdx.x + k * NTHR);
-      const long o = (long)(e / C) * stride + i0 + e % C;
+      const int e2 = (int)threadIdx.x + k * NTHR, c = e2 % C, p = e2 / C;
+      const int o = mix_at<R>(2 * p, i0 + c, Kxp);
       v[k] = *reinterpret_cast<const float4*>(a + o);
       w[k] = *reinterpret_cast<const float4*>(b + o);
     }
 #pragma unroll
-    for (int k = 0; k < EPT / 2; ++k)
-      reinterpret_cast<float4*>(buf)[threadIdx.x + k * NTHR] =
-          make_float4(v[k].x - w[k].x, v[k].y - w[k].y, v[k].z - w[k].z, v[k].w - w[k].w);
+    for (int k = 0; k < EPT / 2; ++k) {
+      const int e2 = (int)threadIdx.x + k * NTHR, c = e2 % C, p = e2 / C;
+      buf[2 * p * C + c] = cmk<cplx<R>>(v[k].x - w[k].x, v[k].y - w[k].y);
+      buf[(2 * p + 1) * C + c] = cmk<cplx<R>>(v[k].z - w[k].z, v[k].w - w[k].w);
+    }
     return;
   }
   cplx<R> v[EPT], w[EPT];
@@ -455,7 +568,7 @@ __device__ __forceinline__ void load_cols_sub(cplx<R>* buf, const cplx<R>* __res
   for (int u = 0; u < EPT; ++u) {
     const int e = threadIdx.x + u * NTHR;
     const int ee = (EXACT || e < TOT) ? e : TOT - 1;
-    const long o = (long)(ee / C) * stride + i0 + ee % C;
+    const int o = mix_at<R>(ee / C, i0 + ee % C, Kxp);
     v[u] = a[o];
     w[u] = b[o];
   }
@@ -649,8 +762,8 @@ __global__ void __launch_bounds__(NTHR) k_row_maxabs_ct(const cplx<R>* __restric
     const int e = threadIdx.x + u * NTHR;
     const bool ok = EXACT || e < TOT;
     const int r = e / N2, k = e - r * N2, kk = N2 - k;
-    va[u] = rsrc_ld<Cx>(rs, ok && k < Kin ? (r * stride + k) * SZ : kRowOob);
-    vb[u] = rsrc_ld<Cx>(rs, ok && kk < Kin ? (r * stride + kk) * SZ : kRowOob);
+    va[u] = rsrc_ld<Cx>(rs, ok && k < Kin ? mix_at<R>(r, k, stride) * SZ : kRowOob);
+    vb[u] = rsrc_ld<Cx>(rs, ok && kk < Kin ? mix_at<R>(r, kk, stride) * SZ : kRowOob);
     w[u] = rsrc_ld<Cx>(rt, ok ? k * SZ : kRowOob);
   }
   load_table<R>(ts, twx, ct_tw_size(N2));
@@ -714,8 +827,9 @@ __global__ void __launch_bounds__(NTHR) k_closure_rows_in(const cplx<R>* __restr
   const bool isq = c0 < g.nzm;
   const int l0 = m * g.nzm + (isq ? c0 : c0 - g.nzm);   // the group's first layer in the batch
   const int stride = g.ny * g.Kxp;                       // complex values from one layer's row to the next's
+  constexpr int KS = mix_kstep<R>();
   const __amdgpu_buffer_rsrc_t rs =
-      row_rsrc((isq ? mq : mp) + ((long)l0 * g.ny + y) * g.Kxp, G * stride * SZ);
+      row_rsrc((isq ? mq : mp) + (long)l0 * stride + mix_at<R>(y, 0, g.Kxp), G * stride * SZ);
   const __amdgpu_buffer_rsrc_t rt = row_rsrc(twr, (N2 + 1) * SZ);
   const int Kin = g.Kx;
   Cx va[EPT], vb[EPT], w[EPT];
@@ -724,8 +838,8 @@ __global__ void __launch_bounds__(NTHR) k_closure_rows_in(const cplx<R>* __restr
     const int e = threadIdx.x + u * NTHR;
     const bool ok = EXACT || e < TOT;
     const int r = e / N2, k = e - r * N2, kk = N2 - k;
-    va[u] = rsrc_ld<Cx>(rs, ok && k < Kin ? (r * stride + k) * SZ : kRowOob);
-    vb[u] = rsrc_ld<Cx>(rs, ok && kk < Kin ? (r * stride + kk) * SZ : kRowOob);
+    va[u] = rsrc_ld<Cx>(rs, ok && k < Kin ? (r * stride + k * KS) * SZ : kRowOob);
+    vb[u] = rsrc_ld<Cx>(rs, ok && kk < Kin ? (r * stride + kk * KS) * SZ : kRowOob);
     w[u] = rsrc_ld<Cx>(rt, ok ? k * SZ : kRowOob);
   }
   load_table<R>(ts, twx, ct_tw_size(N2));
@@ -798,7 +912,8 @@ __global__ void __launch_bounds__(NTHR) k_closure_rows_out(const float* __restri
   const int Kx = g.Kx;
   for (int e = threadIdx.x; e < G * Kx; e += NTHR) {
     const int r = e / Kx, k = e - r * Kx;
-    mixed[((long)(m * g.nzm + c0 + r) * g.ny + y) * g.Kxp + k] = r2c_post_pad<R, PS>(buf, r * N2, k, N2, tr);
+    mixed[(long)(m * g.nzm + c0 + r) * g.ny * g.Kxp + mix_at<R>(y, k, g.Kxp)] =
+        r2c_post_pad<R, PS>(buf, r * N2, k, N2, tr);
   }
 }
 

@@ -558,23 +558,31 @@ int with_ns(int ns, F&& f) {
 // Mixed precision with nz <= 32: the stage + split-BF16 MFMA projection.
 int launch_stage_proj_mfma(qgn_ctx* c, const StageArgs& a, int kernel_id) {
   const long plane = (long)c->g.Ky * c->g.Kxp;
-  // nz <= 16: groups of ProjTiles<NS>::T tiles of 32 points, walked by as
-  // many workgroups as fit on the GPU at once; nz <= 32: workgroups walk
-  // tiles of 16 points, 8 per CU over all members, so the 4x larger split
-  // matrices are built once per workgroup
+  // nz <= 16: the projection walks groups of 4 tiles of 32 points with as
+  // many workgroups as fit on the GPU at once; nz <= 32: the fused kernel's
+  // workgroups walk tiles of 16 points, 8 per CU over all members, so the
+  // 4x larger split matrices are built once per workgroup
   return with_ns(a.ns, [&](auto nsc) {
     constexpr int NS = decltype(nsc)::value;
-    constexpr int T = ProjTiles<NS>::T;
     if (c->g.nzm <= 16) {
-      auto kern = k_stage_proj_mfma16<float, NS, T>;
-      const long groups = (plane + 32 * T - 1) / (32 * T);
+      // the stage sums as a streaming pass (no LDS, no barriers), then the
+      // MFMA projection of the FP32 stage value it wrote
+      const int bx = c->g.Kxp / 2;
+      const int sr = std::max(1, std::min(8, 768 / bx));
+      const dim3 eg((c->g.Ky + sr - 1) / sr, c->g.nz);
+      CHK(timed(c, kernel_id, [&] {
+        hipLaunchKernelGGL(k_stage_ew<NS>, eg, dim3(bx, sr), 0, c->stream, a, c->g);
+      }));
+      constexpr int PT = 4;
+      auto pk = k_proj32_mfma16<PT>;
+      const long groups = (plane + 32 * PT - 1) / (32 * PT);
       int per_cu = 0;
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pk, 256, 0) != hipSuccess || per_cu < 1) per_cu = 1;
       const long fit = std::max(1L, ((long)per_cu * c->ncu + c->nm - 1) / c->nm);
       const int blocks = (int)std::min(groups, fit);
       return timed(c, kernel_id, [&] {
-        hipLaunchKernelGGL(kern, dim3(blocks, c->nm), dim3(256), 0, c->stream, a, c->g, c->HM, c->MD, c->evals,
-                           1.0 / c->p.Htot, reinterpret_cast<float2*>(c->psi));
+        hipLaunchKernelGGL(pk, dim3(blocks, c->nm), dim3(256), 0, c->stream, (const float2*)a.out32, c->g, c->HM,
+                           c->MD, c->evals, 1.0 / c->p.Htot, reinterpret_cast<float2*>(c->psi));
       });
     }
     const long tiles = (plane + 15) / 16;
@@ -792,12 +800,9 @@ int rhs_impl(qgn_ctx* c, const double2* qin, void* out, bool psi_ready) {
     ca.src[4] = qsrc; ca.src_is_double[4] = !q32; ca.mult[4] = 0; ca.dst[4] = c->cl_mq;
     ca.src[5] = psi; ca.src_is_double[5] = isd; ca.mult[5] = 0; ca.dst[5] = c->cl_mp;
   }
-  CHK(launch_col_inv<R>(c, ca));
   RowJacArgs ra{};
   ra.U = c->Um; ra.V = c->Vm; ra.QX = c->QXm; ra.QY = c->QYm; ra.J = c->Jm; ra.DU = c->DUm; ra.DV = c->DVm;
   ra.drag = c->p.C_d != 0.0;
-  CHK(launch_row_jac<R>(c, ra));
-  if (c->net) CHK(closure_forcing<R>(c));
   ColRhsArgs ka{};
   ka.J = c->Jm; ka.M = c->net ? c->cl_mq : nullptr; ka.q = qin; ka.q32 = q32; ka.psi = psi; ka.out = out;
   ka.ubar = c->ubar; ka.qybar = c->qybar;
@@ -805,6 +810,9 @@ int rhs_impl(qgn_ctx* c, const double2* qin, void* out, bool psi_ready) {
   ka.ekman = c->ekm;
   ka.A2 = c->p.A2; ka.A8 = c->p.A8;
   ka.explicit_visc = c->p.is_explicit;
+  CHK(launch_col_inv<R>(c, ca));
+  CHK(launch_row_jac<R>(c, ra));
+  if (c->net) CHK(closure_forcing<R>(c));
   CHK(launch_col_rhs<R>(c, ka, ra.drag));
   c->st.rhs_evals++;
   return 0;
