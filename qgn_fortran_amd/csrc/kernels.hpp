@@ -544,6 +544,7 @@ __global__ void __launch_bounds__(256) k_proj32_mfma16(const float2* __restrict_
   const int tid = threadIdx.x;
   const long plane = (long)g.Ky * g.Kxp;
   const long ngroups = (plane + 32 * T - 1) / (32 * T);
+  const float kxp_inv = 1.0f / (float)g.Kxp;
   const int l = tid & 63;
   const int wv = tid >> 6;
   const int row = l & 15, kb = 4 * (l >> 4);
@@ -593,12 +594,17 @@ __global__ void __launch_bounds__(256) k_proj32_mfma16(const float2* __restrict_
       }
       qgn_float4 d = {0.f, 0.f, 0.f, 0.f};
       d = mfma3(a1h, a1l, bh, bl, d);
-      const long pt = p0 + 32 * t + 8 * wv + (row >> 1);
+      const int pt = (int)(p0 + 32 * t + 8 * wv + (row >> 1));
       bool zero_mode = pt >= plane;
       double k2 = 0.0;
       if (!zero_mode) {
-        const int i = (int)pt % g.Kxp;
-        const int j = compact_row_to_full((int)pt / g.Kxp, g);
+        // compact row and column of pt: a single-precision quotient
+        // (pt < 2^24), corrected to the exact one
+        int jc = (int)((float)pt * kxp_inv);
+        int i = pt - jc * g.Kxp;
+        if (i < 0) { --jc; i += g.Kxp; }
+        else if (i >= g.Kxp) { ++jc; i -= g.Kxp; }
+        const int j = compact_row_to_full(jc, g);
         const double kx = kx_of(i, g), ky = ky_of(j, g);
         k2 = kx * kx + ky * ky;
         zero_mode = (i == 0 && j == 0) || i >= g.Kx;
@@ -606,7 +612,10 @@ __global__ void __launch_bounds__(256) k_proj32_mfma16(const float2* __restrict_
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int mode = kb + r;
-        const float pm = (zero_mode || mode >= nz) ? 0.f : d[r] / (float)(-k2 + ev[r]);
+        // modal inversion by the hardware reciprocal (1 ulp): well inside
+        // the split-BF16 products' 2^-16
+        const float pm =
+            (zero_mode || mode >= nz) ? 0.f : d[r] * __builtin_amdgcn_rcpf((float)(-k2 + ev[r]));
         bh[r] = bf16_rne(pm);
         bl[r] = bf16_lo(pm, bh[r]);
       }
