@@ -28,8 +28,8 @@ sys.path.insert(0, ROOT)
 # ncmp = nz*Ky*Kx compact modes, nmix = nz*ny*Kx mixed values.
 def kernel_bytes(kernel, g, rs, qrs=None):
     """Bytes per launch (row_jac, col_inv, col_rhs, project, update) or per
-    attempt of all launches (stage: NS = 1..5; errnorm: the FP64 column and
-    row passes of the error field)."""
+    attempt of all launches (stage: the stage kernels of an attempt; errnorm:
+    the column and row passes of the error field)."""
     qrs = rs if qrs is None else qrs
     nz, ny, Kx, Ky = g["nz"], g["ny"], g["Kx"], g["Ky"]
     ncmp = nz * Ky * Kx
@@ -41,21 +41,26 @@ def kernel_bytes(kernel, g, rs, qrs=None):
         return (qrs + rs) * ncmp + 4 * nmix * rs
     if kernel == "col_rhs":     # read J, q, psi; write N (+ read |u|u, |u|v for layer nz)
         return nmix * rs + (qrs + 2 * rs) * ncmp + drag
-    if kernel == "project":     # read q (FP64); write psi and the stage value's FP32 copy (mixed)
-        return ncmp * (16 + rs + (rs if rs == 8 else 0))
-    if kernel == "stage":       # NS = 1..5: read q and NS stage RHS; write psi and the stage value
-        # (its FP32 copy on the mixed path, the FP64 value on the FP64 path)
-        return sum(ncmp * (16 + ns * rs + rs + (8 if rs == 8 else 16)) for ns in range(1, 6))
+    if kernel == "project":     # mixed: the MFMA projection of every GetRHS (read the FP32 stage
+        # value, write psi); FP64: the projection of q (read q, write psi)
+        return ncmp * 2 * rs if rs == 8 else ncmp * (16 + rs)
+    if kernel == "stage":       # mixed: NS = 0..5, read q and NS stage RHS, write the stage value
+        # rounded to FP32; FP64: NS = 1..5 fused with the projection, read q and NS stage RHS,
+        # write psi and the FP64 stage value
+        if rs == 8:
+            return sum(ncmp * (16 + ns * rs + rs) for ns in range(0, 6))
+        return sum(ncmp * (16 + ns * rs + rs + 16) for ns in range(1, 6))
     if kernel == "update":      # read q and 6 stage RHS; write the candidate and the error field (FP64)
         return ncmp * (16 + 6 * rs + 32)
-    if kernel == "errnorm":     # FP64 y-inverse (read compact, write mixed) and row max (read mixed)
-        return ncmp * 16 + 2 * nmix * 16
+    if kernel == "errnorm":     # y-inverse of E (read compact, write mixed) and row max (read mixed),
+        # in FP32 on the mixed path
+        return ncmp * rs + 2 * nmix * rs
     raise ValueError(kernel)
 
 
 def projection_flops(g):
-    """BF16 MFMA flops of one stage/projection launch (nz <= 16): per tile of
-    32 points, 4 waves x 2 products x 3 MFMAs of 16x16x16."""
+    """BF16 MFMA flops of one projection launch (k_proj32_mfma16, nz <= 16):
+    per tile of 32 points, 4 waves x 2 products x 3 MFMAs of 16x16x16."""
     plane = g["Ky"] * g["Kxp"]
     return -(-plane // 32) * 4 * 2 * 3 * 2 * 16 * 16 * 16
 
@@ -364,12 +369,12 @@ def main():
     # MFMA utilisation of the split-BF16 projection (mixed; nz <= 16): the
     # stage/projection kernels' BF16 flops over their summed time
     kp = mixed["kernels"]
-    proj_ms = kp["project"]["ms"] + kp["stage"]["ms"]
-    proj_launches = kp["project"]["launches"] + kp["stage"]["launches"]
+    proj_ms = kp["project"]["ms"]
+    proj_launches = kp["project"]["launches"]
     mfma = None
     if wl.params.nz <= 16 and proj_ms > 0:
         fl = projection_flops(g) * proj_launches
-        mfma = dict(kernel="k_stage_proj_mfma16", bf16_flops_per_launch=projection_flops(g),
+        mfma = dict(kernel="k_proj32_mfma16", bf16_flops_per_launch=projection_flops(g),
                     tflops=round(fl / (proj_ms * 1e-3) / 1e12, 2),
                     frac_bf16_dense_peak=round(fl / (proj_ms * 1e-3) / 2.5e15, 5))
     rs = 8  # mixed path: FP32 complex intermediates

@@ -198,6 +198,12 @@ int qgn_set_tuning(qgn_ctx* ctx, int col_inv, int col_rhs, int row);
  * qgn_prof_read returns the launch count and summed milliseconds since
  * enable, qgn_prof_read_all the same per kernel id (arrays of QGN_K_COUNT);
  * both synchronise the stream.  No counterpart in the reference. */
+/* Kernel ids: the three FFT passes of GetRHS; the projection (GetPsi: on
+ * the mixed path with nz <= 16 the MFMA projection of every GetRHS, else the
+ * projection of q); the stage kernels (the ARK stage sums, on the FP64 path
+ * fused with their projection; on the mixed path with nz <= 16 also the
+ * FP32 copy of q before the first GetRHS); the candidate update; the error
+ * norm's transform. */
 enum { QGN_K_ALL = -2, QGN_K_ROW_JAC = 0, QGN_K_COL_INV = 1, QGN_K_COL_RHS = 2, QGN_K_PROJECT = 3,
        QGN_K_STAGE = 4, QGN_K_UPDATE = 5, QGN_K_ERRNORM = 6, QGN_K_COUNT = 7 };
 int qgn_prof_enable(qgn_ctx* ctx, int kernel_id);

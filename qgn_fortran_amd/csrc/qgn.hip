@@ -570,7 +570,10 @@ int launch_stage_proj_mfma(qgn_ctx* c, const StageArgs& a, int kernel_id) {
       const int bx = c->g.Kxp / 2;
       const int sr = std::max(1, std::min(8, 768 / bx));
       const dim3 eg((c->g.Ky + sr - 1) / sr, c->g.nz);
-      CHK(timed(c, kernel_id, [&] {
+      // timed as the stage sums (QGN_K_STAGE) and the projection
+      // (QGN_K_PROJECT), whichever call this is
+      (void)kernel_id;
+      CHK(timed(c, QGN_K_STAGE, [&] {
         hipLaunchKernelGGL(k_stage_ew<NS>, eg, dim3(bx, sr), 0, c->stream, a, c->g);
       }));
       constexpr int PT = 4;
@@ -580,7 +583,7 @@ int launch_stage_proj_mfma(qgn_ctx* c, const StageArgs& a, int kernel_id) {
       if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pk, 256, 0) != hipSuccess || per_cu < 1) per_cu = 1;
       const long fit = std::max(1L, ((long)per_cu * c->ncu + c->nm - 1) / c->nm);
       const int blocks = (int)std::min(groups, fit);
-      return timed(c, kernel_id, [&] {
+      return timed(c, QGN_K_PROJECT, [&] {
         hipLaunchKernelGGL(pk, dim3(blocks, c->nm), dim3(256), 0, c->stream, (const float2*)a.out32, c->g, c->HM,
                            c->MD, c->evals, 1.0 / c->p.Htot, reinterpret_cast<float2*>(c->psi));
       });
