@@ -579,8 +579,13 @@ __device__ __forceinline__ void load_cols_sub(cplx<R>* buf, const cplx<R>* __res
   }
 }
 
+// FP32 power-of-two lengths with at most 512 threads: at least 6 waves per
+// SIMD (at most 85 VGPRs; these fit without spills, the radix-3 lengths do
+// not), so three workgroups of 8 waves share a CU within its LDS.
 template <typename R, typename S, int NY, int C, int NTHR>
-__global__ void __launch_bounds__(NTHR) k_col_rhs_ct(ColRhsArgs a, Geom g, const cplx<R>* __restrict__ twy) {
+__global__ void __launch_bounds__(NTHR)
+    __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 && (NY & (NY - 1)) == 0 && NTHR <= 512 ? 6 : 1)))
+    k_col_rhs_ct(ColRhsArgs a, Geom g, const cplx<R>* __restrict__ twy) {
   using Cx = cplx<R>;
   constexpr int MAXO = 6;   // kept modes per thread: C Ky <= 6 NTHR since Ky < 3 ny / 4
   extern __shared__ __align__(16) unsigned char lds_raw[];
