@@ -495,20 +495,24 @@ __device__ __forceinline__ qgn_float4 mfma3(const qgn_short4& ah, const qgn_shor
 }
 
 // The mixed path's stage values alone, as a streaming pass (the projection
-// follows in k_proj32_mfma16): block (Kxp/2, SR), thread (x, y) takes
-// columns 2x, 2x+1 of compact row blockIdx.x SR + y of layer blockIdx.y of
-// the batch, so the wavenumbers need no index division.  Writes the stage
+// follows in k_proj32_mfma16): grid (plane / 512, layers of the batch),
+// each thread two adjacent points of its layer.  Writes the stage
 // value rounded to FP32 (and the FP64 value where a.out is set); every
 // stage kernel forms the sums with stage_value_ij, so the values agree to
 // the bit.
 template <int NS>
-__global__ void __launch_bounds__(1024) k_stage_ew(StageArgs a, Geom g) {
+__global__ void __launch_bounds__(256) k_stage_ew(StageArgs a, Geom g) {
   const int z = blockIdx.y;
-  const int jc = blockIdx.x * blockDim.y + threadIdx.y;
-  if (jc >= g.Ky) return;
-  const int i = 2 * threadIdx.x;
+  const int pt = 2 * (int)(blockIdx.x * blockDim.x + threadIdx.x);   // Kxp is even: pt, pt + 1 share a row
+  if (pt >= g.Ky * g.Kxp) return;
+  // compact row and column: a single-precision quotient (pt < 2^24),
+  // corrected to the exact one
+  int jc = (int)((float)pt * (1.0f / (float)g.Kxp));
+  int i = pt - jc * g.Kxp;
+  if (i < 0) { --jc; i += g.Kxp; }
+  else if (i >= g.Kxp) { ++jc; i -= g.Kxp; }
   const int j = compact_row_to_full(jc, g);
-  const long e = ((long)z * g.Ky + jc) * g.Kxp + i;
+  const long e = (long)z * g.Ky * g.Kxp + pt;
   const double dt = a.dtv ? a.dtv[z / g.nzm] : a.dt;
   const double2 q0 = stage_value_ij<float, NS>(a, g, e, i, j, z, dt);
   const double2 q1 = stage_value_ij<float, NS>(a, g, e + 1, i + 1, j, z, dt);

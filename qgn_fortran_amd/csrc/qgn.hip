@@ -567,14 +567,12 @@ int launch_stage_proj_mfma(qgn_ctx* c, const StageArgs& a, int kernel_id) {
     if (c->g.nzm <= 16) {
       // the stage sums as a streaming pass (no LDS, no barriers), then the
       // MFMA projection of the FP32 stage value it wrote
-      const int bx = c->g.Kxp / 2;
-      const int sr = std::max(1, std::min(8, 768 / bx));
-      const dim3 eg((c->g.Ky + sr - 1) / sr, c->g.nz);
+      const dim3 eg((unsigned)((plane / 2 + 255) / 256), c->g.nz);
       // timed as the stage sums (QGN_K_STAGE) and the projection
       // (QGN_K_PROJECT), whichever call this is
       (void)kernel_id;
       CHK(timed(c, QGN_K_STAGE, [&] {
-        hipLaunchKernelGGL(k_stage_ew<NS>, eg, dim3(bx, sr), 0, c->stream, a, c->g);
+        hipLaunchKernelGGL(k_stage_ew<NS>, eg, dim3(256), 0, c->stream, a, c->g);
       }));
       constexpr int PT = 4;
       auto pk = k_proj32_mfma16<PT>;
