@@ -50,8 +50,9 @@ def kernel_bytes(kernel, g, rs, qrs=None):
         if rs == 8:
             return sum(ncmp * (16 + ns * rs + rs) for ns in range(0, 6))
         return sum(ncmp * (16 + ns * rs + rs + 16) for ns in range(1, 6))
-    if kernel == "update":      # read q and 6 stage RHS; write the candidate and the error field (FP64)
-        return ncmp * (16 + 6 * rs + 32)
+    if kernel == "update":      # read q and 6 stage RHS; write the candidate (FP64) and the error field
+        # (float2 on the mixed path)
+        return ncmp * (16 + 6 * rs + 16 + rs)
     if kernel == "errnorm":     # y-inverse of E (read compact, write mixed) and row max (read mixed),
         # in FP32 on the mixed path
         return ncmp * rs + 2 * nmix * rs
