@@ -340,7 +340,8 @@ struct qgn_ens {
   std::vector<qgn_cnn*> nets;   // closure networks its graph captured (held until destroy)
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
-  std::vector<hipEvent_t> ready;
+  hipEvent_t fork = nullptr;        // capture edges: stream -> each batch's stream -> stream
+  std::vector<hipEvent_t> ready, join;
   // per variant v of the batches' attempt graphs (each batch's state in its
   // gbuf[v]): one attempt of every batch, and kChain attempts of every batch
   // in one launch
@@ -2487,57 +2488,88 @@ int qgn_ensemble_destroy(qgn_ens* e) {
     if (e->chain[v]) (void)hipGraphDestroy(e->chain[v]);
   }
   for (hipEvent_t x : e->ready) (void)hipEventDestroy(x);
+  for (hipEvent_t x : e->join) (void)hipEventDestroy(x);
+  if (e->fork) (void)hipEventDestroy(e->fork);
   if (e->done) (void)hipEventDestroy(e->done);
   if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
   return 0;
 }
 
-// Per variant v of the batches' attempt graphs, two graphs: one attempt of
-// every batch (its attempt graph as a child), and kChain attempts of every
-// batch, each batch a chain of kChain children (the variants alternating)
-// that depend on one another and on nothing of the other batches.
-static int ensemble_create_impl(qgn_ens* e, qgn_ctx* const* ctxs, int n) {
-  for (int v = 0; v < 2; ++v) {
-    HIPCHK(hipGraphCreate(&e->graph[v], 0));
-    HIPCHK(hipGraphCreate(&e->chain[v], 0));
+// Capture n_att attempts of every batch from variant v on (the variants
+// alternating) into *out.  Each batch's attempts are recorded on that
+// batch's own stream, forked from e->stream and joined back to it, so the
+// graph holds one chain of kernel nodes per batch with no edge between the
+// chains: the graph executor may run the sizes' kernels side by side (child
+// graph nodes would run one batch after another).
+static int ensemble_capture(qgn_ens* e, int v, int n_att, hipGraph_t* out) {
+  *out = nullptr;
+  HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeRelaxed));
+  int rc = 0;
+  hipError_t st = hipEventRecord(e->fork, e->stream);
+  for (size_t i = 0; i < e->ctx.size() && !rc && st == hipSuccess; ++i) {
+    qgn_ctx* c = e->ctx[i];
+    st = hipStreamWaitEvent(c->stream, e->fork, 0);
+    if (st != hipSuccess) break;
+    double2* const q0 = c->q;
+    double2* const qn0 = c->qn;
+    const int prof = c->prof_id;
+    c->prof_id = -1;   // no event records inside the graph
+    for (int k = 0; k < n_att && !rc; ++k) {
+      const int w = (v + k) & 1;
+      c->q = c->gbuf[w];
+      c->qn = c->gbuf[1 - w];
+      rc = batch_attempt(c);
+    }
+    c->q = q0;
+    c->qn = qn0;
+    c->prof_id = prof;
+    if (!rc) st = hipEventRecord(e->join[i], c->stream);
+    if (!rc && st == hipSuccess) st = hipStreamWaitEvent(e->stream, e->join[i], 0);
   }
+  hipGraph_t gr = nullptr;
+  const hipError_t se = hipStreamEndCapture(e->stream, &gr);
+  if (!rc && st != hipSuccess) rc = fail(QGN_EHIP, std::string("ensemble capture: ") + hipGetErrorString(st));
+  if (!rc && se != hipSuccess) rc = fail(QGN_EHIP, std::string("ensemble capture: ") + hipGetErrorString(se));
+  if (rc) {
+    if (gr) (void)hipGraphDestroy(gr);
+    return rc;
+  }
+  *out = gr;
+  return 0;
+}
+
+static int ensemble_create_impl(qgn_ens* e, qgn_ctx* const* ctxs, int n) {
   for (int i = 0; i < n; ++i) {
     if (!ctxs[i]) return fail(QGN_EINVAL, "null context");
     for (int k = 0; k < i; ++k) {
       if (ctxs[k] == ctxs[i]) return fail(QGN_EINVAL, "a context appears twice in the ensemble");
-      // the batches' graphs run concurrently; one network's activation buffers cannot serve two
+      // the batches' kernels run concurrently; one network's activation buffers cannot serve two
       if (ctxs[i]->net && ctxs[k]->net == ctxs[i]->net)
         return fail(QGN_EINVAL, "two batches of an ensemble share one closure network");
     }
-    CHK(batch_check(ctxs[i]));
+    CHK(batch_check(ctxs[i]));   // gbuf[0], gbuf[1] fixed, launches checked by one capture
     e->ctx.push_back(ctxs[i]);
     ctxs[i]->ens_refs++;
-    if (ctxs[i]->net) {   // the child graph keeps the network's buffers after a detach
+    if (ctxs[i]->net) {   // the graph keeps the network's buffers after a detach
       qcnn::hold(ctxs[i]->net, +1);
       e->nets.push_back(ctxs[i]->net);
     }
-    for (int v = 0; v < 2; ++v) {
-      hipGraphNode_t node;
-      HIPCHK(hipGraphAddChildGraphNode(&node, e->graph[v], nullptr, 0, ctxs[i]->graph[v]));
-      hipGraphNode_t prev = nullptr;
-      for (int k = 0; k < kChain; ++k) {
-        hipGraphNode_t nk;
-        HIPCHK(hipGraphAddChildGraphNode(&nk, e->chain[v], prev ? &prev : nullptr, prev ? 1 : 0,
-                                         ctxs[i]->graph[(v + k) & 1]));
-        prev = nk;
-      }
+    for (std::vector<hipEvent_t>* evs : {&e->ready, &e->join}) {
+      hipEvent_t ev;
+      HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+      evs->push_back(ev);
     }
-    hipEvent_t ev;
-    HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-    e->ready.push_back(ev);
-  }
-  for (int v = 0; v < 2; ++v) {
-    HIPCHK(hipGraphInstantiate(&e->exec[v], e->graph[v], nullptr, nullptr, 0));
-    HIPCHK(hipGraphInstantiate(&e->chain_exec[v], e->chain[v], nullptr, nullptr, 0));
   }
   HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
   HIPCHK(hipEventCreateWithFlags(&e->done, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&e->fork, hipEventDisableTiming));
+  for (int v = 0; v < 2; ++v) {
+    CHK(ensemble_capture(e, v, 1, &e->graph[v]));
+    CHK(ensemble_capture(e, v, kChain, &e->chain[v]));
+    HIPCHK(hipGraphInstantiate(&e->exec[v], e->graph[v], nullptr, nullptr, 0));
+    HIPCHK(hipGraphInstantiate(&e->chain_exec[v], e->chain[v], nullptr, nullptr, 0));
+  }
   return 0;
 }
 
