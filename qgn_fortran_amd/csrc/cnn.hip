@@ -21,6 +21,7 @@
 // transposed (ci <-> co) and the taps mirrored.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -291,86 +292,135 @@ __global__ void __launch_bounds__(256) k_conv3x3_tile(const u16* __restrict__ in
 // then CO biases); k_wgrad_reduce adds the slices in a fixed order, so a
 // training step is reproducible.  X: (n, H, W, CI) bf16 (layer input), dY:
 // (n, H, W, COP) bf16 (gradient of the layer's pre-activation, channels >=
-// CO zero).  Block: `rows` image rows of image blockIdx.y; per row the three
-// input rows and the gradient row are staged in LDS, and each wave
-// accumulates its share of the 9 (COP/16) (CI/16) 16x16 tiles over 16-pixel
-// chunks (each weight entry has one owning lane).
+// CO zero).  Block: `rows` image rows of image blockIdx.y.
+//
+// The reduction runs over pixels, so both MFMA operands are staged in LDS
+// channel-major (each channel's pixels contiguous): a lane's four pixels of
+// A (dY, 16 output channels x 16 pixels) are one 8-byte read, and those of
+// B (X, 16 pixels x 16 input channels) for the three horizontal taps of a
+// row come from one 8-byte and one 4-byte read (pixels p0-1 .. p0+4; the
+// middle tap takes its words across a 2-byte boundary).  The X rows carry
+// their periodic neighbours (pixel -1 and W) and live in a 3-row ring, so
+// each image row is staged once; row strides of W + 4 values keep the 16
+// channels of a read in distinct LDS banks.  Each wave owns one 16 x 16
+// (output, input) channel tile for all 9 taps (9 independent accumulation
+// chains); with 2 such tiles, two waves split the 16-pixel chunks of each
+// row and add their sums in wave order at the end.
 template <int CI, int CO>
 __global__ void __launch_bounds__(256) k_conv_wgrad(const u16* __restrict__ x, const u16* __restrict__ dy,
                                                     float* __restrict__ part, int H, int W, int rows) {
   constexpr int COP = pad16(CO);
-  constexpr int NMT = COP / 16, NNT = CI / 16, NT = 9 * NMT * NNT, TPW = (NT + 3) / 4;
+  constexpr int NMT = COP / 16, NNT = CI / 16, NP = NMT * NNT;
+  static_assert(NP == 2 || NP == 4, "channel tiles");
+  constexpr int WPP = 4 / NP;            // waves per channel tile
   extern __shared__ __align__(16) u16 lds[];
-  u16* xs = lds;                       // [3][W][CI]
-  u16* ds = lds + 3 * W * CI;          // [W][COP]
-  __shared__ float bpart[256];
+  const int S = W + 4;                   // row stride (values) of the channel-major rows
+  u16* xs = lds;                         // [3 ring slots][CI][S]: pixel p at p + 1, p = -1 .. W
+  u16* ds = lds + 3 * CI * S;            // [COP][S]: pixel p at p
+  __shared__ float bpart[8 * 256];
   const int tid = threadIdx.x;
   const int lane = tid & 63, w = tid >> 6;
   const int j = lane & 15, g = lane >> 4;
   const int n = blockIdx.y;
   const int y0 = blockIdx.x * rows;
   const int y1 = y0 + rows < H ? y0 + rows : H;
-  f4v acc[TPW];
+  const int pair = w / WPP, sub = w % WPP;
+  const int mt = pair / NNT, nt = pair % NNT;
+  f4v acc[9];
 #pragma unroll
-  for (int t = 0; t < TPW; ++t) acc[t] = f4v{0.f, 0.f, 0.f, 0.f};
-  float bacc = 0.f;   // channel tid % COP (256 is a multiple of COP)
+  for (int t = 0; t < 9; ++t) acc[t] = f4v{0.f, 0.f, 0.f, 0.f};
+  // bias partial sums: thread tid's staging loads always hold channels
+  // 8 (tid % (COP / 8)) .. + 7 (256 is a multiple of COP / 8)
+  float bacc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   part += (long)(blockIdx.y * gridDim.x + blockIdx.x) * (CO * 9 * CI + CO);
-  for (int y = y0; y < y1; ++y) {
-    __syncthreads();
-    for (int e = tid * 8; e < 3 * W * CI; e += 256 * 8) {
-      const int r = e / (W * CI), rem = e - r * (W * CI);
-      int yy = y + r - 1;
-      yy += yy < 0 ? H : 0;
-      yy -= yy >= H ? H : 0;
-      *reinterpret_cast<uint4*>(&xs[e]) =
-          *reinterpret_cast<const uint4*>(&x[((long)n * H + yy) * W * CI + rem]);
+  // image row yy (periodic) of X into ring slot yy mod 3, channel-major with neighbours
+  auto stage_x = [&](int yy) {
+    int yr = yy;
+    yr += yr < 0 ? H : 0;
+    yr -= yr >= H ? H : 0;
+    u16* dst = xs + ((yy + 3) % 3) * CI * S;
+    const u16* src = x + ((long)n * H + yr) * W * CI;
+    for (int e = tid; e < (W + 2) * (CI / 8); e += 256) {
+      const int pp = e / (CI / 8), c8 = 8 * (e - pp * (CI / 8));   // padded pixel pp = p + 1
+      int px = pp - 1;
+      px += px < 0 ? W : 0;
+      px -= px >= W ? W : 0;
+      const uint4 v = *reinterpret_cast<const uint4*>(src + (long)px * CI + c8);
+      const u16* h = reinterpret_cast<const u16*>(&v);
+#pragma unroll
+      for (int c = 0; c < 8; ++c) dst[(c8 + c) * S + pp] = h[c];
     }
-    for (int e = tid * 8; e < W * COP; e += 256 * 8)
-      *reinterpret_cast<uint4*>(&ds[e]) = *reinterpret_cast<const uint4*>(&dy[((long)n * H + y) * W * COP + e]);
-    __syncthreads();
-    for (int e = tid; e < W * COP; e += 256) bacc += bf2f(ds[e]);
-    for (int ks = 0; ks < W / 16; ++ks) {
+  };
+  for (int yy = y0 - 1; yy <= y0; ++yy) stage_x(yy);
+  for (int y = y0; y < y1; ++y) {
+    stage_x(y + 1);   // slot (y + 1) mod 3 held row y - 2, which the previous row's chunks no longer read
+    {
+      const u16* src = dy + ((long)n * H + y) * W * COP;
+      for (int e = tid; e < W * (COP / 8); e += 256) {
+        const int px = e / (COP / 8), c8 = 8 * (e - px * (COP / 8));
+        const uint4 v = *reinterpret_cast<const uint4*>(src + (long)px * COP + c8);
+        const u16* h = reinterpret_cast<const u16*>(&v);
 #pragma unroll
-      for (int tt = 0; tt < TPW; ++tt) {
-        const int t = w + 4 * tt;
-        if (t >= NT) break;
-        const int tap = t / (NMT * NNT);
-        const int rem = t - tap * (NMT * NNT);
-        const int mt = rem / NNT, nt = rem - mt * NNT;
-        const int ddy = tap / 3 - 1, ddx = tap % 3 - 1;
-        s4v a, b;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int px = ks * 16 + 4 * g + r;
-          a[r] = (short)ds[px * COP + mt * 16 + j];                 // A[co][pixel]
-          int xx = px + ddx;
-          xx += xx < 0 ? W : 0;
-          xx -= xx >= W ? W : 0;
-          b[r] = (short)xs[((ddy + 1) * W + xx) * CI + nt * 16 + j];  // B[pixel][ci]
+        for (int c = 0; c < 8; ++c) {
+          ds[(c8 + c) * S + px] = h[c];
+          bacc[c] += bf2f(h[c]);
         }
-        acc[tt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[tt], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+    const u16* arow = ds + (mt * 16 + j) * S + 4 * g;
+    for (int ks = sub; ks < W / 16; ks += WPP) {
+      const s4v a = *reinterpret_cast<const s4v*>(arow + 16 * ks);
+#pragma unroll
+      for (int ddy = 0; ddy < 3; ++ddy) {
+        const u16* brow = xs + ((y + ddy - 1 + 3) % 3) * CI * S + (nt * 16 + j) * S + 16 * ks + 4 * g;
+        const uint2 w01 = *reinterpret_cast<const uint2*>(brow);       // pixels p0-1 .. p0+2
+        const unsigned w2 = *reinterpret_cast<const unsigned*>(brow + 4);   // pixels p0+3, p0+4
+        uint2 m;
+        m.x = __builtin_amdgcn_alignbyte(w01.y, w01.x, 2);               // pixels p0, p0+1
+        m.y = __builtin_amdgcn_alignbyte(w2, w01.y, 2);                  // pixels p0+2, p0+3
+        const uint2 hi = make_uint2(w01.y, w2);                          // pixels p0+1 .. p0+4
+        s4v b0, b1, b2;
+        __builtin_memcpy(&b0, &w01, 8);
+        __builtin_memcpy(&b1, &m, 8);
+        __builtin_memcpy(&b2, &hi, 8);
+        acc[3 * ddy + 0] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b0, acc[3 * ddy + 0], 0, 0, 0);
+        acc[3 * ddy + 1] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b1, acc[3 * ddy + 1], 0, 0, 0);
+        acc[3 * ddy + 2] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b2, acc[3 * ddy + 2], 0, 0, 0);
+      }
+    }
+    __syncthreads();   // ds and the ring slot of row y - 1 are rewritten next
+  }
+  // the staging buffers are free: combine the waves of one channel tile in
+  // wave order, then the bias partial sums in thread order
+  float* red = reinterpret_cast<float*>(lds);
+  if (WPP == 2 && sub == 1) {
+#pragma unroll
+    for (int t = 0; t < 9; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) red[((pair * 9 + t) * 4 + r) * 64 + lane] = acc[t][r];
+  }
+  __syncthreads();
+  if (sub == 0) {
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        float v = acc[t][r];
+        if (WPP == 2) v += red[((pair * 9 + t) * 4 + r) * 64 + lane];
+        const int co = mt * 16 + 4 * g + r;
+        if (co < CO) part[(co * 9 + t) * CI + nt * 16 + j] = v;
       }
     }
   }
 #pragma unroll
-  for (int tt = 0; tt < TPW; ++tt) {
-    const int t = w + 4 * tt;
-    if (t >= NT) break;
-    const int tap = t / (NMT * NNT);
-    const int rem = t - tap * (NMT * NNT);
-    const int mt = rem / NNT, nt = rem - mt * NNT;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int co = mt * 16 + 4 * g + r;
-      if (co < CO) part[(co * 9 + tap) * CI + nt * 16 + j] = acc[tt][r];
-    }
-  }
-  // bias: thread tid holds channel tid % COP; summed per channel in thread order
-  bpart[tid] = bacc;
+  for (int c = 0; c < 8; ++c) bpart[c * 256 + tid] = bacc[c];
   __syncthreads();
   if (tid < CO) {
+    // channel tid = 8 c8 + c: threads k with k % (COP / 8) == c8, in order
+    const int c8 = tid / 8, c = tid % 8;
     float sb = 0.f;
-    for (int k = tid; k < 256; k += COP) sb += bpart[k];
+    for (int k = c8; k < 256; k += COP / 8) sb += bpart[c * 256 + k];
     part[CO * 9 * CI + tid] = sb;
   }
 }
@@ -561,7 +611,9 @@ int conv(const u16* in, const u16* w, const float* b, const u16* mask, void* out
   return 0;
 }
 
-constexpr int kWgradRows = 16;   // image rows per weight-gradient workgroup
+// image rows per weight-gradient workgroup: 640 workgroups for 10 images of
+// 256^2 (2.5 per CU), 9248 floats of partial sums each
+constexpr int kWgradRows = 4;
 
 // workgroup slices of the weight-gradient scratch for a batch of n images of
 // H rows, and floats per slice (the largest layer: 32 x 9 x 32 + 32)
@@ -573,7 +625,7 @@ int wgrad(const u16* x, const u16* dy, float* dw, float* db, int n, int H, int W
   constexpr int COP = pad16(CO);
   constexpr int PSZ = CO * 9 * CI + CO;
   static_assert(PSZ <= kWgradSlice, "slice");
-  const size_t lds = (size_t)(3 * W * CI + W * COP) * sizeof(u16);
+  const size_t lds = std::max((size_t)(3 * CI + COP) * (W + 4) * sizeof(u16), (size_t)9 * 4 * 64 * 4 * 2);
   auto kern = k_conv_wgrad<CI, CO>;
   if (allow_lds((const void*)kern, lds)) return cfail(QGN_EINVAL, "wgrad: grid row too wide for LDS");
   const dim3 grid((H + kWgradRows - 1) / kWgradRows, n);
@@ -588,7 +640,8 @@ int wgrad(const u16* x, const u16* dy, float* dw, float* db, int n, int H, int W
 int check_shape(const qgn_cnn* h, int n, int H, int W) {
   if (n < 1 || H < 3 || W < 16 || W % 16 || ((long)n * H * W) % 64)
     return cfail(QGN_EINVAL, "cnn: needs W % 16 == 0, n*H*W % 64 == 0, H >= 3");
-  if (3L * W * 32 * 2 + (long)W * 32 * 2 > 160 * 1024) return cfail(QGN_EINVAL, "cnn: W too large");
+  // the weight gradient's LDS rows: 3 input rows and one gradient row of 32 channels, W + 4 values each
+  if ((3L * 32 + 32) * (W + 4) * 2 > 160 * 1024) return cfail(QGN_EINVAL, "cnn: W too large");
   return 0;
 }
 

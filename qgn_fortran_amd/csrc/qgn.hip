@@ -578,8 +578,13 @@ int launch_stage_proj_mfma(qgn_ctx* c, const StageArgs& a, int kernel_id) {
       constexpr int PT = 4;
       auto pk = k_proj32_mfma16<PT>;
       const long groups = (plane + 32 * PT - 1) / (32 * PT);
+      // at most 3 workgroups per CU, so each walks about 2.4 groups at
+      // 1024^2 x 16 and the next group's loads overlap the current one's
+      // MFMAs (21.2 us per launch against 23.2 at the 5 that fit, 24.6 with
+      // one group per workgroup)
       int per_cu = 0;
       if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pk, 256, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+      per_cu = std::min(per_cu, 3);
       const long fit = std::max(1L, ((long)per_cu * c->ncu + c->nm - 1) / c->nm);
       const int blocks = (int)std::min(groups, fit);
       return timed(c, QGN_K_PROJECT, [&] {
