@@ -375,6 +375,7 @@ def main(argv=None):
     ap.add_argument("--filter-cells", type=float, default=2.0)
     ap.add_argument("--seeds", default="11,12,13,14,15")
     ap.add_argument("--holdout", type=int, default=2)
+    ap.add_argument("--init-seed", type=int, default=5, help="seed of the initial weights and of the batch order")
     ap.add_argument("--out", default=None)
     a = ap.parse_args(argv)
     import torch
@@ -382,7 +383,7 @@ def main(argv=None):
         raise SystemExit("no GPU")
     st = Setup(n_hi=a.n_hi, n_lo=a.n_lo, snapshots=a.snapshots, every=a.every, spinup=a.spinup, epochs=a.epochs,
                lr=a.lr, filter_cells=a.filter_cells, seeds=tuple(int(x) for x in a.seeds.split(",")),
-               holdout=a.holdout)
+               holdout=a.holdout, seed=a.init_seed)
     res = run(st, log=lambda s: print(s, file=sys.stderr, flush=True))
     line = json.dumps(res)
     print(line, flush=True)
