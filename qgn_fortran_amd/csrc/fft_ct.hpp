@@ -173,4 +173,27 @@ __device__ __forceinline__ void load_table(cplx<Rt>* dst, const cplx<Rt>* __rest
   for (; i < count; i += step) dst[i] = src[i];
 }
 
+// The same copy in two halves for a compile-time count and thread count:
+// table_fetch() issues the thread's loads into registers, table_store()
+// writes them to LDS.  A kernel fetches the table, then issues its own data
+// loads, so both sets are in flight together before either is stored.
+template <int COUNT, int NTHR> constexpr int table_per_thread() { return (COUNT + NTHR - 1) / NTHR; }
+template <typename Rt, int COUNT, int NTHR, int PER>
+__device__ __forceinline__ void table_fetch(cplx<Rt> (&v)[PER], const cplx<Rt>* __restrict__ src) {
+  static_assert(PER == table_per_thread<COUNT, NTHR>(), "table share");
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int i = (int)threadIdx.x + u * NTHR;
+    v[u] = src[PER * NTHR == COUNT || i < COUNT ? i : 0];
+  }
+}
+template <typename Rt, int COUNT, int NTHR, int PER>
+__device__ __forceinline__ void table_store(const cplx<Rt> (&v)[PER], cplx<Rt>* dst) {
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int i = (int)threadIdx.x + u * NTHR;
+    if (PER * NTHR == COUNT || i < COUNT) dst[i] = v[u];
+  }
+}
+
 }  // namespace qgn

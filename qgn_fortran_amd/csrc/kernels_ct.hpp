@@ -45,7 +45,12 @@ __global__ void __launch_bounds__(NTHR) k_col_inv_ct(ColInvArgs a, Geom g, const
   const int f = w % a.nf;
   const int i0 = ((w / a.nf) % ntile) * C;
   const int z = w / (a.nf * ntile);
-  load_table<R>(tw, twy, ct_tw_size(NY));
+  // FP32: the twiddle table's loads in flight with the tile's loads below
+  // (FP64 tiles would spill the extra registers at 2048 rows)
+  constexpr bool TAB_OVERLAP = sizeof(R) == 4;
+  cplx<R> tab[TAB_OVERLAP ? table_per_thread<ct_tw_size(NY), NTHR>() : 1];
+  if constexpr (TAB_OVERLAP) table_fetch<R, ct_tw_size(NY), NTHR>(tab, twy);
+  else load_table<R>(tw, twy, ct_tw_size(NY));
   {
     const int mult = a.mult[f];
     const bool isd = a.src_is_double[f];
@@ -105,6 +110,7 @@ __global__ void __launch_bounds__(NTHR) k_col_inv_ct(ColInvArgs a, Geom g, const
       }
       buf[e] = x;
     }
+    if constexpr (TAB_OVERLAP) table_store<R, ct_tw_size(NY), NTHR>(tab, tw);
     __syncthreads();
     ct_fft<R, NY, L, NTHR>(buf, tw, +1);
     Cx* dst = reinterpret_cast<Cx*>(a.dst[f]) + (long)z * NY * g.Kxp;
@@ -597,12 +603,26 @@ __global__ void __launch_bounds__(NTHR)
   int z, i0;
   rhs_tile(blockIdx.x, gridDim.x, ntile, g.nzm, a.drag, &z, &i0);   // grid = ntile * nz
   i0 *= C;
-  load_table<R>(tw, twy, ct_tw_size(NY));
-  if (a.M)   // the forcing enters as -FFT_y(J - M)
-    load_cols_sub<R, C, NTHR, TOT>(buf, reinterpret_cast<const Cx*>(a.J) + (long)z * NY * g.Kxp,
-                                   reinterpret_cast<const Cx*>(a.M) + (long)z * NY * g.Kxp, g.Kxp, i0);
-  else
-    load_cols<R, C, NTHR, TOT>(buf, reinterpret_cast<const Cx*>(a.J) + (long)z * NY * g.Kxp, g.Kxp, i0);
+  // the tile of J (of J - M with the closure network's forcing M) into LDS
+  auto load_tile = [&]() {
+    if (a.M)   // the forcing enters as -FFT_y(J - M)
+      load_cols_sub<R, C, NTHR, TOT>(buf, reinterpret_cast<const Cx*>(a.J) + (long)z * NY * g.Kxp,
+                                     reinterpret_cast<const Cx*>(a.M) + (long)z * NY * g.Kxp, g.Kxp, i0);
+    else
+      load_cols<R, C, NTHR, TOT>(buf, reinterpret_cast<const Cx*>(a.J) + (long)z * NY * g.Kxp, g.Kxp, i0);
+  };
+  if constexpr (sizeof(R) == 4 && NTHR <= 512) {
+    // FP32 with up to 512 threads: the twiddle table's loads in flight with
+    // the tile's (FP64 tiles, and the 1024-thread tiles of 2048 rows, would
+    // lose occupancy to the extra registers)
+    cplx<R> tab[table_per_thread<ct_tw_size(NY), NTHR>()];
+    table_fetch<R, ct_tw_size(NY), NTHR>(tab, twy);
+    load_tile();
+    table_store<R, ct_tw_size(NY), NTHR>(tab, tw);
+  } else {
+    load_table<R>(tw, twy, ct_tw_size(NY));
+    load_tile();
+  }
   __syncthreads();
   ct_fft<R, NY, L, NTHR>(buf, tw, -1);
   cplx<S>* out = reinterpret_cast<cplx<S>*>(a.out);
