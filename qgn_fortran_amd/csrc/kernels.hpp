@@ -486,6 +486,24 @@ __device__ __forceinline__ short bf16_rne(float x) {
 }
 __device__ __forceinline__ float bf16_to_f(short h) { return __uint_as_float(((unsigned)(unsigned short)h) << 16); }
 __device__ __forceinline__ short bf16_lo(float x, short hi) { return bf16_rne(x - bf16_to_f(hi)); }
+// The split of four floats into an MFMA operand pair (hi = bf16(x), lo =
+// bf16(x - hi)) with gfx950's packed conversion (v_cvt_pk_bf16_f32, round
+// to nearest even like bf16_rne): two conversions per pair of values.
+__device__ __forceinline__ void bf16_split4(const float (&x)[4], qgn_short4& hi, qgn_short4& lo) {
+  typedef __bf16 bf2v __attribute__((ext_vector_type(2)));
+  typedef float f2v __attribute__((ext_vector_type(2)));
+  unsigned h[2], l[2];
+#pragma unroll
+  for (int p = 0; p < 2; ++p) {
+    const bf2v hv = __builtin_convertvector((f2v){x[2 * p], x[2 * p + 1]}, bf2v);
+    __builtin_memcpy(&h[p], &hv, 4);
+    const float h0 = __uint_as_float(h[p] << 16), h1 = __uint_as_float(h[p] & 0xffff0000u);
+    const bf2v lv = __builtin_convertvector((f2v){x[2 * p] - h0, x[2 * p + 1] - h1}, bf2v);
+    __builtin_memcpy(&l[p], &lv, 4);
+  }
+  __builtin_memcpy(&hi, h, 8);
+  __builtin_memcpy(&lo, l, 8);
+}
 __device__ __forceinline__ qgn_float4 mfma3(const qgn_short4& ah, const qgn_short4& al, const qgn_short4& bh,
                                             const qgn_short4& bl, qgn_float4 c) {
   c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, c, 0, 0, 0);
@@ -589,13 +607,13 @@ __global__ void __launch_bounds__(256) k_proj32_mfma16(const float2* __restrict_
     qgn_float4 pv[T];
 #pragma unroll
     for (int t = 0; t < T; ++t) {
+      float xs[4];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const float2 v = sq[t][kb + r][8 * wv + (row >> 1)];
-        const float x = (row & 1) ? v.y : v.x;
-        bh[r] = bf16_rne(x);
-        bl[r] = bf16_lo(x, bh[r]);
+        xs[r] = (row & 1) ? v.y : v.x;
       }
+      bf16_split4(xs, bh, bl);
       qgn_float4 d = {0.f, 0.f, 0.f, 0.f};
       d = mfma3(a1h, a1l, bh, bl, d);
       const int pt = (int)(p0 + 32 * t + 8 * wv + (row >> 1));
@@ -613,16 +631,15 @@ __global__ void __launch_bounds__(256) k_proj32_mfma16(const float2* __restrict_
         k2 = kx * kx + ky * ky;
         zero_mode = (i == 0 && j == 0) || i >= g.Kx;
       }
+      float pms[4];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int mode = kb + r;
         // modal inversion by the hardware reciprocal (1 ulp): well inside
         // the split-BF16 products' 2^-16
-        const float pm =
-            (zero_mode || mode >= nz) ? 0.f : d[r] * __builtin_amdgcn_rcpf((float)(-k2 + ev[r]));
-        bh[r] = bf16_rne(pm);
-        bl[r] = bf16_lo(pm, bh[r]);
+        pms[r] = (zero_mode || mode >= nz) ? 0.f : d[r] * __builtin_amdgcn_rcpf((float)(-k2 + ev[r]));
       }
+      bf16_split4(pms, bh, bl);
       qgn_float4 pz = {0.f, 0.f, 0.f, 0.f};
       pv[t] = mfma3(a2h, a2l, bh, bl, pz);
     }
@@ -727,13 +744,13 @@ __global__ void __launch_bounds__(256) k_stage_proj_mfma(StageArgs a, Geom g, co
 #pragma unroll
     for (int kt = 0; kt < NT; ++kt) {
       qgn_short4 bh, bl;
+      float xs[4];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const double2 v = sq[half][16 * kt + kb + r][8 * wv + (row >> 1)];
-        const float x = (float)((row & 1) ? v.y : v.x);
-        bh[r] = bf16_rne(x);
-        bl[r] = bf16_lo(x, bh[r]);
+        xs[r] = (float)((row & 1) ? v.y : v.x);
       }
+      bf16_split4(xs, bh, bl);
 #pragma unroll
       for (int mi = 0; mi < NT; ++mi) d[mi] = mfma3(a1h[mi][kt], a1l[mi][kt], bh, bl, d[mi]);
     }
@@ -750,14 +767,15 @@ __global__ void __launch_bounds__(256) k_stage_proj_mfma(StageArgs a, Geom g, co
     // modal Helmholtz inversion; D1 as it stands is the B operand of the second product
     qgn_short4 ph[NT], pl[NT];
 #pragma unroll
-    for (int mi = 0; mi < NT; ++mi)
+    for (int mi = 0; mi < NT; ++mi) {
+      float pms[4];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int mode = 16 * mi + kb + r;
-        const float pm = (zero_mode || mode >= nz) ? 0.f : d[mi][r] / (float)(-k2 + ev[mi][r]);
-        ph[mi][r] = bf16_rne(pm);
-        pl[mi][r] = bf16_lo(pm, ph[mi][r]);
+        pms[r] = (zero_mode || mode >= nz) ? 0.f : d[mi][r] / (float)(-k2 + ev[mi][r]);
       }
+      bf16_split4(pms, ph[mi], pl[mi]);
+    }
     // second product: psi[layer][column] = Modes[layer][mode] pm[mode][column]
 #pragma unroll
     for (int li = 0; li < NT; ++li) {
