@@ -289,7 +289,7 @@ __global__ void __launch_bounds__(256) k_conv3x3_tile(const u16* __restrict__ in
 // Weight and bias gradients of one layer, per workgroup:
 //   dW[co][tap][ci] = sum_p dY[p][co] X[p + off(tap)][ci],  db[co] = sum_p dY[p][co]
 // over the workgroup's pixels, into its slice of `part` (CO 9 CI weights,
-// then CO biases); k_wgrad_reduce adds the slices in a fixed order, so a
+// then CO biases); k_wgrad_reduce_chunks/_final add the slices in a fixed order, so a
 // training step is reproducible.  X: (n, H, W, CI) bf16 (layer input), dY:
 // (n, H, W, COP) bf16 (gradient of the layer's pre-activation, channels >=
 // CO zero).  Block: `rows` image rows of image blockIdx.y.
@@ -426,24 +426,38 @@ __global__ void __launch_bounds__(256) k_conv_wgrad(const u16* __restrict__ x, c
 }
 
 // grad[i] += sum over the nblk workgroup slices of part[b * psz + i]; i <
-// CO 9 CI are weights (dw), the rest biases (db).  A workgroup of 256 takes
-// 64 entries: wave q sums slices q, q + 4, q + 8, ... in order, then the four
-// wave sums are added in wave order (a fixed order throughout).
-__global__ void __launch_bounds__(256) k_wgrad_reduce(const float* __restrict__ part, int nblk, int psz, int nw,
-                                                      float* __restrict__ dw, float* __restrict__ db) {
-  __shared__ float red[4][64];
+// CO 9 CI are weights (dw), the rest biases (db).  Two passes, each in a
+// fixed order: k_wgrad_reduce_chunks sums chunk c of the slices ([c per,
+// (c + 1) per)) for 64 entries per workgroup (wave q takes slices q, q + 4,
+// ... of the chunk in order, then the four wave sums are added in wave
+// order) into red[c psz + i]; k_wgrad_reduce_final adds the chunk sums in
+// chunk order.  Chunks spread the slices over many workgroups (a training
+// batch of 10 images of 256^2 has 640 slices).
+constexpr int kWgradChunks = 16;
+__global__ void __launch_bounds__(256) k_wgrad_reduce_chunks(const float* __restrict__ part, int nblk, int psz,
+                                                             float* __restrict__ red) {
+  __shared__ float ws[4][64];
   const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
   const int i = blockIdx.x * 64 + lane;
+  const int c = blockIdx.y;
+  const int per = (nblk + kWgradChunks - 1) / kWgradChunks;
+  const int b0 = c * per, b1 = b0 + per < nblk ? b0 + per : nblk;
   float sacc = 0.f;
   if (i < psz)
-    for (int b = q; b < nblk; b += 4) sacc += part[(long)b * psz + i];
-  red[q][lane] = sacc;
+    for (int b = b0 + q; b < b1; b += 4) sacc += part[(long)b * psz + i];
+  ws[q][lane] = sacc;
   __syncthreads();
-  if (q == 0 && i < psz) {
-    const float t = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
-    if (i < nw) dw[i] += t;
-    else db[i - nw] += t;
-  }
+  if (q == 0 && i < psz) red[(long)c * psz + i] = ((ws[0][lane] + ws[1][lane]) + ws[2][lane]) + ws[3][lane];
+}
+__global__ void __launch_bounds__(256) k_wgrad_reduce_final(const float* __restrict__ red, int psz, int nw,
+                                                            float* __restrict__ dw, float* __restrict__ db) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= psz) return;
+  float t = 0.f;
+#pragma unroll
+  for (int c = 0; c < kWgradChunks; ++c) t += red[(long)c * psz + i];
+  if (i < nw) dw[i] += t;
+  else db[i - nw] += t;
 }
 
 // Mean-squared error of pred against target over n_pix * CO values; writes
@@ -631,8 +645,13 @@ int wgrad(const u16* x, const u16* dy, float* dw, float* db, int n, int H, int W
   const dim3 grid((H + kWgradRows - 1) / kWgradRows, n);
   hipLaunchKernelGGL(kern, grid, dim3(256), lds, s, x, dy, part, H, W, kWgradRows);
   CCHK(hipGetLastError());
-  hipLaunchKernelGGL(k_wgrad_reduce, dim3((PSZ + 63) / 64), dim3(256), 0, s, part, (int)(grid.x * grid.y), PSZ,
-                     CO * 9 * CI, dw, db);
+  // the chunk sums follow the slices in the same scratch buffer
+  const int nblk = (int)(grid.x * grid.y);
+  float* red = part + (long)nblk * kWgradSlice;
+  hipLaunchKernelGGL(k_wgrad_reduce_chunks, dim3((PSZ + 63) / 64, kWgradChunks), dim3(256), 0, s, part, nblk, PSZ,
+                     red);
+  CCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_wgrad_reduce_final, dim3((PSZ + 255) / 256), dim3(256), 0, s, red, PSZ, CO * 9 * CI, dw, db);
   CCHK(hipGetLastError());
   return 0;
 }
@@ -808,7 +827,7 @@ static int backward_impl(qgn_cnn* h, const float* target, double* loss) {
   const int n = h->n, H = h->H, W = h->W;
   const long npix = (long)n * H * W;
   const Layer& L4 = h->L[kLayers - 1];
-  const long need = wgrad_slices(n, H) * kWgradSlice;
+  const long need = (wgrad_slices(n, H) + kWgradChunks) * kWgradSlice;   // slices, then chunk sums
   if (need > h->part_cap) {
     CCHK(hipStreamSynchronize(s));
     if (h->part) (void)hipFree(h->part);
