@@ -461,16 +461,20 @@ __global__ void __launch_bounds__(256) k_wgrad_reduce_final(const float* __restr
 }
 
 // Mean-squared error of pred against target over n_pix * CO values; writes
-// dL/dpred as bf16 with COP channels (the padding channels zero) and adds
-// the sum of squares to *sq.
-__global__ void k_mse(const float* __restrict__ pred, const float* __restrict__ tgt, long npix, int CO, int COP,
-                      float scale, u16* __restrict__ dpred, double* __restrict__ sq) {
+// dL/dpred as bf16 with COP channels (the padding channels zero, COP a power
+// of two) and each workgroup's sum of squares to sq[blockIdx.x] (the host
+// adds the kMseBlocks partial sums in order, so the loss is reproducible).
+constexpr int kMseBlocks = 1024;
+__global__ void __launch_bounds__(256) k_mse(const float* __restrict__ pred, const float* __restrict__ tgt,
+                                             long npix, int CO, int COP, float scale, u16* __restrict__ dpred,
+                                             double* __restrict__ sq) {
   __shared__ double red[4];
   double s = 0.0;
   const long n = npix * COP;
+  const int lg = __ffs(COP) - 1;
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
-    const long p = e / COP;
-    const int c = (int)(e - p * COP);
+    const long p = e >> lg;
+    const int c = (int)e & (COP - 1);
     float d = 0.f;
     if (c < CO) {
       d = pred[p * CO + c] - tgt[p * CO + c];
@@ -481,7 +485,7 @@ __global__ void k_mse(const float* __restrict__ pred, const float* __restrict__ 
   for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
   __syncthreads();
-  if (threadIdx.x == 0) atomicAdd(sq, red[0] + red[1] + red[2] + red[3]);
+  if (threadIdx.x == 0) sq[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
 }
 
 // Adam on the FP32 parameter vector; c1 = 1/(1 - b1^t), c2 = 1/(1 - b2^t).
@@ -764,8 +768,8 @@ static int cnn_create_impl(qgn_cnn* h, int cin, int cout, const float* params) {
     CCHK(hipMalloc((void**)&h->wb[l], nb));
     CCHK(hipMemset(h->wb[l], 0, nb));
   }
-  CCHK(hipMalloc((void**)&h->sq, sizeof(double)));
-  CCHK(hipHostMalloc((void**)&h->h_sq, sizeof(double), hipHostMallocDefault));
+  CCHK(hipMalloc((void**)&h->sq, kMseBlocks * sizeof(double)));
+  CCHK(hipHostMalloc((void**)&h->h_sq, kMseBlocks * sizeof(double), hipHostMallocDefault));
   CRET(repack(h, h->stream));
   CCHK(hipStreamSynchronize(h->stream));
   return 0;
@@ -838,10 +842,9 @@ static int backward_impl(qgn_cnn* h, const float* target, double* loss) {
   }
   float* part = h->part;
   CCHK(hipMemsetAsync(h->g, 0, (size_t)h->np * sizeof(float), s));
-  CCHK(hipMemsetAsync(h->sq, 0, sizeof(double), s));
   const float scale = (float)(2.0 / ((double)npix * L4.co));
-  hipLaunchKernelGGL(k_mse, dim3(1024), dim3(256), 0, s, h->pred, target, npix, L4.co, L4.cop, scale, h->dpred,
-                     h->sq);
+  hipLaunchKernelGGL(k_mse, dim3(kMseBlocks), dim3(256), 0, s, h->pred, target, npix, L4.co, L4.cop, scale,
+                     h->dpred, h->sq);
   CCHK(hipGetLastError());
   // layer 4 (no activation): dY = dpred
   float* g = h->g;
@@ -863,9 +866,11 @@ static int backward_impl(qgn_cnn* h, const float* target, double* loss) {
     else CRET((wgrad<32, 32>(h->act[l - 1], dx, g + Lp.woff, g + Lp.boff, n, H, W, part, s)));
     dy = dx;
   }
-  CCHK(hipMemcpyAsync(h->h_sq, h->sq, sizeof(double), hipMemcpyDeviceToHost, s));
+  CCHK(hipMemcpyAsync(h->h_sq, h->sq, kMseBlocks * sizeof(double), hipMemcpyDeviceToHost, s));
   CCHK(hipStreamSynchronize(s));
-  if (loss) *loss = *h->h_sq / ((double)npix * L4.co);
+  double sum = 0.0;
+  for (int b = 0; b < kMseBlocks; ++b) sum += h->h_sq[b];
+  if (loss) *loss = sum / ((double)npix * L4.co);
   return 0;
 }
 
