@@ -374,24 +374,35 @@ __global__ void __launch_bounds__(ROWS * (N2 / 8)) k_row_jac_fused2(RowJacArgs a
   Cx* const w8p = buf + padded<PS>(r * N2 + 8 * t);   // padded(r N2 + 8t + m) = this + m, m < 8
   Cx* const lp = buf + padded<PS>(r * N2 + t);
   const Cx* const twl = ts + OFF_LAST + t;
-  // x-c2r of one mixed row into registers: o[j] holds x[2m], x[2m+1] at m = t + j NB,
-  // unnormalised (every product below is of two such fields: s2 = scale^2
-  // normalises it once)
+  // x-c2r of one mixed row in two halves.  fetch: the kept bins X[k] and
+  // X[N2 - k] of this thread's 8 first-pass inputs (out-of-range bins read
+  // zero).  head: the c2r pre-step and the first radix-8 pass, into LDS
+  // (the fetched bins are dead after it, so the next field's fetch can
+  // follow).  tail: the middle passes and the last pass, which leaves
+  // o[j] = x[2m], x[2m+1] at m = t + j NB, unnormalised (every product
+  // below is of two such fields: s2 = scale^2 normalises it once)
   const R s2 = scale * scale;
-  auto inverse = [&](const void* src, Cx(&o)[8]) {
+  auto fetch = [&](const void* src, Cx(&xa)[8], Cx(&xb)[8]) {
     const __amdgpu_buffer_rsrc_t rs =
         row_rsrc(reinterpret_cast<const Cx*>(src) + (long)row0 * Kxp, ROWS * Kxp * (int)sizeof(Cx));
-    Cx v[8];
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      const int k = t + j * NB;
-      v[j] = c2r_pre_rs<R>(rs, r, Kxp, Kx, k, N2, tr, (R)1);
+      const int k = t + j * NB, kk = N2 - k;
+      xa[j] = rsrc_ld<Cx>(rs, k < Kx ? mix_at<R>(r, k, Kxp) * (int)sizeof(Cx) : kRowOob);
+      xb[j] = rsrc_ld<Cx>(rs, kk < Kx ? mix_at<R>(r, kk, Kxp) * (int)sizeof(Cx) : kRowOob);
     }
+  };
+  auto head = [&](const Cx(&xa)[8], const Cx(&xb)[8]) {
+    Cx v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = c2r_pre_ab<R>(xa[j], xb[j], t + j * NB, tr, (R)1);
     Dft<Cx, 8>::run(v, +1);
     __syncthreads();   // earlier readers of buf are done
 #pragma unroll
     for (int m = 0; m < 8; ++m) w8p[m] = v[m];
     __syncthreads();
+  };
+  auto tail = [&](Cx(&o)[8]) {
     row_passes_range<R, N2, 1, LAST, 8, NTHR, PS>(buf, ts, +1);
 #pragma unroll
     for (int j = 0; j < 8; ++j) o[j] = AFF ? lp[j * pad_step<PS>(NB)] : buf[padded<PS>(r * N2 + t + j * NB)];
@@ -433,35 +444,48 @@ __global__ void __launch_bounds__(ROWS * (N2 / 8)) k_row_jac_fused2(RowJacArgs a
       }
     }
   };
-  __syncthreads();
-  Cx fu[8], fv[8];
-  inverse(a.U, fu);
-  inverse(a.V, fv);
-  if (bottom) {   // URMS = sqrt(u^2 + v^2) (QGN_Module.f90:285-289)
-    const long drow = (long)((z / g.nzm) * g.ny + j0) * Kxp;   // bottom layer of member z / nzm
-    Cx w8[8];
+  // each field's bins load while the previous field runs its LDS passes;
+  // on bottom-layer rows q_x loads after the drag products' transforms
+  Cx fu[8], fv[8], acc[8];
+  {
+    Cx xa[8], xb[8];
+    fetch(a.U, xa, xb);   // in flight with the twiddle tables' loads
+    __syncthreads();
+    head(xa, xb);
+    fetch(a.V, xa, xb);
+    tail(fu);
+    head(xa, xb);
+    if (!bottom) fetch(a.QX, xa, xb);
+    tail(fv);
+    if (bottom) {   // URMS = sqrt(u^2 + v^2) (QGN_Module.f90:285-289)
+      const long drow = (long)((z / g.nzm) * g.ny + j0) * Kxp;   // bottom layer of member z / nzm
+      Cx w8[8];
 #pragma unroll
-    for (int m = 0; m < 8; ++m) {
-      const R r0 = s2 * sqrt(fu[m].x * fu[m].x + fv[m].x * fv[m].x);
-      const R r1 = s2 * sqrt(fu[m].y * fu[m].y + fv[m].y * fv[m].y);
-      w8[m] = cmk<Cx>(r0 * fu[m].x, r1 * fu[m].y);
-    }
-    forward(w8, reinterpret_cast<Cx*>(a.DU) + drow);
+      for (int m = 0; m < 8; ++m) {
+        const R r0 = s2 * sqrt(fu[m].x * fu[m].x + fv[m].x * fv[m].x);
+        const R r1 = s2 * sqrt(fu[m].y * fu[m].y + fv[m].y * fv[m].y);
+        w8[m] = cmk<Cx>(r0 * fu[m].x, r1 * fu[m].y);
+      }
+      forward(w8, reinterpret_cast<Cx*>(a.DU) + drow);
 #pragma unroll
-    for (int m = 0; m < 8; ++m) {
-      const R r0 = s2 * sqrt(fu[m].x * fu[m].x + fv[m].x * fv[m].x);
-      const R r1 = s2 * sqrt(fu[m].y * fu[m].y + fv[m].y * fv[m].y);
-      w8[m] = cmk<Cx>(r0 * fv[m].x, r1 * fv[m].y);
+      for (int m = 0; m < 8; ++m) {
+        const R r0 = s2 * sqrt(fu[m].x * fu[m].x + fv[m].x * fv[m].x);
+        const R r1 = s2 * sqrt(fu[m].y * fu[m].y + fv[m].y * fv[m].y);
+        w8[m] = cmk<Cx>(r0 * fv[m].x, r1 * fv[m].y);
+      }
+      forward(w8, reinterpret_cast<Cx*>(a.DV) + drow);
+      fetch(a.QX, xa, xb);
     }
-    forward(w8, reinterpret_cast<Cx*>(a.DV) + drow);
+    head(xa, xb);
+    fetch(a.QY, xa, xb);
+    tail(acc);
+#pragma unroll
+    for (int m = 0; m < 8; ++m) acc[m] = cmk<Cx>(fu[m].x * acc[m].x, fu[m].y * acc[m].y);
+    head(xa, xb);
   }
-  Cx acc[8];
-  inverse(a.QX, acc);
-#pragma unroll
-  for (int m = 0; m < 8; ++m) acc[m] = cmk<Cx>(fu[m].x * acc[m].x, fu[m].y * acc[m].y);
   {
     Cx qy[8];
-    inverse(a.QY, qy);
+    tail(qy);
 #pragma unroll
     for (int m = 0; m < 8; ++m)
       acc[m] = cmk<Cx>(s2 * (acc[m].x + fv[m].x * qy[m].x), s2 * (acc[m].y + fv[m].y * qy[m].y));
