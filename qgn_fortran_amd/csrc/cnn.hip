@@ -170,6 +170,9 @@ __global__ void __launch_bounds__(256) k_conv3x3_tile(const u16* __restrict__ in
                                                       void* __restrict__ out, int H, int W) {
   constexpr int COP = pad16(CO);
   constexpr int NW = COP * 9 * CI;
+  // LDS weight rows of 9 CI values padded by 8 (16 bytes): the 16 output
+  // channels of an A operand read fall in distinct banks
+  constexpr int WS = 9 * CI + 8;
   constexpr int TH = 64 * TP / TW;
   constexpr int WW = TW + 2, NR = TH + 2;
   constexpr int PS = CI + 4;            // bf16 values per staged pixel
@@ -177,7 +180,7 @@ __global__ void __launch_bounds__(256) k_conv3x3_tile(const u16* __restrict__ in
   constexpr int NCH = NR * WW * CH;
   constexpr int MAXC = (NCH + 255) / 256;
   static_assert(TW % 16 == 0 && (64 * TP) % TW == 0, "tile of whole 16-pixel rows");
-  __shared__ __align__(16) u16 ws[NW];
+  __shared__ __align__(16) u16 ws[COP * WS];
   __shared__ __align__(16) u16 xs[NR * WW * PS];
   const int tx_n = W / TW, ty_n = H / TH;
   const int img = blockIdx.x / (tx_n * ty_n);
@@ -203,8 +206,10 @@ __global__ void __launch_bounds__(256) k_conv3x3_tile(const u16* __restrict__ in
     const int off = c < NCH ? ((yy * W + xx) * CI + sub * 8) * 2 : 0x7ffffff0;
     v[u] = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
   }
-  for (int e = threadIdx.x * 8; e < NW; e += 256 * 8)
-    *reinterpret_cast<uint4*>(&ws[e]) = *reinterpret_cast<const uint4*>(&wpk[e]);
+  for (int e = threadIdx.x * 8; e < NW; e += 256 * 8) {
+    const int co = e / (9 * CI);   // 9 CI is a multiple of 8: a chunk stays in its row
+    *reinterpret_cast<uint4*>(&ws[co * WS + e - co * 9 * CI]) = *reinterpret_cast<const uint4*>(&wpk[e]);
+  }
 #pragma unroll
   for (int u = 0; u < MAXC; ++u) {
     const int c = threadIdx.x + u * 256;
@@ -241,7 +246,7 @@ __global__ void __launch_bounds__(256) k_conv3x3_tile(const u16* __restrict__ in
       for (int u = 0; u < TP; ++u) b[u] = *reinterpret_cast<const s4v*>(base[u] + toff + kc * 16);
 #pragma unroll
       for (int mt = 0; mt < COP / 16; ++mt) {
-        const s4v a = *reinterpret_cast<const s4v*>(&ws[((mt * 16 + j) * 9 + tap) * CI + kc * 16 + 4 * g]);
+        const s4v a = *reinterpret_cast<const s4v*>(&ws[(mt * 16 + j) * WS + tap * CI + kc * 16 + 4 * g]);
 #pragma unroll
         for (int u = 0; u < TP; ++u) acc[u][mt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[u], acc[u][mt], 0, 0, 0);
       }
@@ -275,12 +280,15 @@ __global__ void __launch_bounds__(256) k_conv3x3_tile(const u16* __restrict__ in
           f[2] = bf2f(m.z) > 0.f ? acc[u][mt][2] : 0.f;
           f[3] = bf2f(m.w) > 0.f ? acc[u][mt][3] : 0.f;
         }
-        ushort4 o;
-        o.x = f2bf(f[0]);
-        o.y = f2bf(f[1]);
-        o.z = f2bf(f[2]);
-        o.w = f2bf(f[3]);
-        *reinterpret_cast<ushort4*>(reinterpret_cast<u16*>(out) + p * CO + co) = o;
+        // two packed conversions (round to nearest even, as f2bf for finite values)
+        typedef __bf16 bf2v __attribute__((ext_vector_type(2)));
+        typedef float f2v __attribute__((ext_vector_type(2)));
+        const bf2v lo = __builtin_convertvector((f2v){f[0], f[1]}, bf2v);
+        const bf2v hi = __builtin_convertvector((f2v){f[2], f[3]}, bf2v);
+        uint2 o;
+        __builtin_memcpy(&o.x, &lo, 4);
+        __builtin_memcpy(&o.y, &hi, 4);
+        *reinterpret_cast<uint2*>(reinterpret_cast<u16*>(out) + p * CO + co) = o;
       }
     }
   }
