@@ -60,6 +60,8 @@ template <typename C> __device__ __forceinline__ C ld_or_zero(const C* p, int k,
   return ok ? v : cmk<C>(0, 0);
 }
 template <typename C> __device__ __forceinline__ C cscale(C a, decltype(C::x) s) { return cmk<C>(a.x * s, a.y * s); }
+__device__ __forceinline__ float fma_r(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_r(double a, double b, double c) { return __builtin_fma(a, b, c); }
 // multiply by sign*i  (sign = +1 or -1)
 template <typename C> __device__ __forceinline__ C cmul_si(C a, int sign) {
   return sign > 0 ? cmk<C>(-a.y, a.x) : cmk<C>(a.y, -a.x);
@@ -96,18 +98,82 @@ template <typename C> struct Dft<C, 8> {
     C o[4] = {a[1], a[3], a[5], a[7]};
     Dft<C, 4>::run(e, s);
     Dft<C, 4>::run(o, s);
-    // o[m] *= w8^m, w8 = exp(s*i*pi/4) = (1 + s i)/sqrt(2)
-    C o1 = cmk<C>(h * (o[1].x - s * o[1].y), h * (o[1].y + s * o[1].x));
-    C o2 = cmul_si(o[2], s);
-    C o3 = cmk<C>(h * (-o[3].x - s * o[3].y), h * (-o[3].y + s * o[3].x));
+    // o[m] *= w8^m, w8 = exp(s*i*pi/4) = (1 + s i)/sqrt(2): the factor h
+    // of m = 1, 3 goes into the FMAs of the last butterflies
+    const C d1 = cmk<C>(o[1].x - s * o[1].y, o[1].y + s * o[1].x);
+    const C o2 = cmul_si(o[2], s);
+    const C d3 = cmk<C>(-o[3].x - s * o[3].y, -o[3].y + s * o[3].x);
     a[0] = cadd(e[0], o[0]);
     a[4] = csub(e[0], o[0]);
-    a[1] = cadd(e[1], o1);
-    a[5] = csub(e[1], o1);
+    a[1] = cmk<C>(fma_r(h, d1.x, e[1].x), fma_r(h, d1.y, e[1].y));
+    a[5] = cmk<C>(fma_r(-h, d1.x, e[1].x), fma_r(-h, d1.y, e[1].y));
     a[2] = cadd(e[2], o2);
     a[6] = csub(e[2], o2);
-    a[3] = cadd(e[3], o3);
-    a[7] = csub(e[3], o3);
+    a[3] = cmk<C>(fma_r(h, d3.x, e[3].x), fma_r(h, d3.y, e[3].y));
+    a[7] = cmk<C>(fma_r(-h, d3.x, e[3].x), fma_r(-h, d3.y, e[3].y));
+  }
+};
+
+// ---- the same DFTs of twiddled inputs, X[m] = sum_j a[j] W^j w^(j m) --------
+// The twiddle products are folded into the butterflies: a radix-2 step
+// a' = a + t b, b' = a - t b takes four FMAs for a' and one each for
+// b' = 2a - a', six instructions against eight for a complex product and
+// two complex additions.  Radix 8 splits into two radix-4 steps over the
+// even and odd inputs (base W^2) and a last radix-2 step with W w^m; it
+// needs the table values W, W^2 and W^4 only.
+template <typename C> __device__ __forceinline__ void bfly_tw(C& a, C& b, const C& t) {
+  const auto px = fma_r(t.x, b.x, fma_r(-t.y, b.y, a.x));
+  const auto py = fma_r(t.x, b.y, fma_r(t.y, b.x, a.y));
+  b = cmk<C>(fma_r(decltype(px)(2), a.x, -px), fma_r(decltype(px)(2), a.y, -py));
+  a = cmk<C>(px, py);
+}
+// radix 4: y[r] W^r with W = w1, W^2 = w2; results in y[0..3]
+template <typename C> __device__ __forceinline__ void dft4_tw(C& y0, C& y1, C& y2, C& y3, const C& w1, const C& w2,
+                                                              int s) {
+  bfly_tw(y0, y2, w2);   // y0 +- W^2 y2
+  bfly_tw(y1, y3, w2);   // y1 +- W^2 y3
+  bfly_tw(y0, y1, w1);   // X0, X2
+  bfly_tw(y2, y3, cmul_si(w1, s));   // X1, X3
+  const C x1 = y2;
+  y2 = y1;
+  y1 = x1;
+}
+template <typename C, int RAD> struct DftTw {
+  // generic radix: the complex products, then the plain DFT
+  template <typename TW> static __device__ __forceinline__ void run(C* a, TW twj, int s) {
+#pragma unroll
+    for (int j = 1; j < RAD; ++j) a[j] = cmul(a[j], twj(j));
+    Dft<C, RAD>::run(a, s);
+  }
+};
+template <typename C> struct DftTw<C, 2> {
+  template <typename TW> static __device__ __forceinline__ void run(C* a, TW twj, int) { bfly_tw(a[0], a[1], twj(1)); }
+};
+template <typename C> struct DftTw<C, 4> {
+  template <typename TW> static __device__ __forceinline__ void run(C* a, TW twj, int s) {
+    dft4_tw(a[0], a[1], a[2], a[3], twj(1), twj(2), s);
+  }
+};
+template <typename C> struct DftTw<C, 8> {
+  template <typename TW> static __device__ __forceinline__ void run(C* a, TW twj, int s) {
+    using R = decltype(C::x);
+    const R h = (R)0.70710678118654752440084436210484903928;
+    const C w1 = twj(1), w2 = twj(2), w4 = twj(4);
+    C e[4] = {a[0], a[2], a[4], a[6]};
+    C o[4] = {a[1], a[3], a[5], a[7]};
+    dft4_tw(e[0], e[1], e[2], e[3], w2, w4, s);
+    dft4_tw(o[0], o[1], o[2], o[3], w2, w4, s);
+    // X[m] = E[m] + W w8^m O[m], X[m+4] = E[m] - W w8^m O[m]
+    const C t1 = cmk<C>(h * (w1.x - s * w1.y), h * (w1.y + s * w1.x));
+    bfly_tw(e[0], o[0], w1);
+    bfly_tw(e[1], o[1], t1);
+    bfly_tw(e[2], o[2], cmul_si(w1, s));
+    bfly_tw(e[3], o[3], cmul_si(t1, s));
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      a[m] = e[m];
+      a[m + 4] = o[m];
+    }
   }
 };
 

@@ -275,14 +275,14 @@ __device__ __forceinline__ void row_pass(cplx<R>* buf, const cplx<R>* __restrict
     for (int j = 0; j < RAD; ++j) v[b][j] = rp[j * pad_step<PS>(NB)];
     if constexpr (P > 1) {
       const C* tp = tw + OFF + k;
-#pragma unroll
-      for (int j = 1; j < RAD; ++j) {
+      DftTw<C, RAD>::run(v[b], [&](int j) {
         C w = tp[(j - 1) * P];
         if (sign > 0) w.y = -w.y;
-        v[b][j] = cmul(v[b][j], w);
-      }
+        return w;
+      }, sign);
+    } else {
+      Dft<C, RAD>::run(v[b], sign);
     }
-    Dft<C, RAD>::run(v[b], sign);
     wbase[b] = s * N2 + (t - k) * RAD + k;
   }
   __syncthreads();
@@ -406,13 +406,11 @@ __global__ void __launch_bounds__(ROWS * (N2 / 8)) k_row_jac_fused2(RowJacArgs a
     row_passes_range<R, N2, 1, LAST, 8, NTHR, PS>(buf, ts, +1);
 #pragma unroll
     for (int j = 0; j < 8; ++j) o[j] = AFF ? lp[j * pad_step<PS>(NB)] : buf[padded<PS>(r * N2 + t + j * NB)];
-#pragma unroll
-    for (int j = 1; j < 8; ++j) {
+    DftTw<Cx, 8>::run(o, [&](int j) {
       Cx w = twl[(j - 1) * NB];
       w.y = -w.y;
-      o[j] = cmul(o[j], w);
-    }
-    Dft<Cx, 8>::run(o, +1);
+      return w;
+    }, +1);
   };
   // x-r2c of the products in w8 (same positions) to the Kx kept bins of ROWS rows at out
   auto forward = [&](Cx(&w8)[8], Cx* out) {
