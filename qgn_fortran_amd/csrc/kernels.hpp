@@ -233,6 +233,27 @@ __device__ __forceinline__ cplx<R> c2r_pre_w(cplx<R> a, cplx<R> b, int k, cplx<R
   const Cx wd = cmul(w, d);
   return cmk<Cx>((s.x - wd.y) * scale, (s.y + wd.x) * scale);  // s + i*wd
 }
+// c2r_pre_w with scale 1 and the twiddle product folded into FMAs:
+// s + i conj(w) d as four FMAs instead of a complex product and two adds
+template <typename R>
+__device__ __forceinline__ cplx<R> c2r_pre_fma(cplx<R> a, cplx<R> b, int k, cplx<R> w) {
+  using Cx = cplx<R>;
+  if (k == 0) { a.y = (R)0; b.y = (R)0; }
+  const Cx s = cmk<Cx>(a.x + b.x, a.y - b.y);   // X[k] + conj X[N2-k]
+  const Cx d = cmk<Cx>(a.x - b.x, a.y + b.y);   // X[k] - conj X[N2-k]
+  // conj(w) d = (w.x d.x + w.y d.y, w.x d.y - w.y d.x); s + i conj(w) d
+  return cmk<Cx>(fma_r(-w.x, d.y, fma_r(w.y, d.x, s.x)), fma_r(w.x, d.x, fma_r(w.y, d.y, s.y)));
+}
+// r2c_post's bin from Z[k] = aa and Z[N2-k] = bb with the twiddle w:
+// (S + w O) / 2 with S = aa + conj bb, O = -i (aa - conj bb), the product
+// folded into FMAs and the halving applied once
+template <typename R>
+__device__ __forceinline__ cplx<R> r2c_post_fma(cplx<R> aa, cplx<R> bb, cplx<R> w) {
+  using Cx = cplx<R>;
+  const R sx = aa.x + bb.x, sy = aa.y - bb.y;
+  const R ox = aa.y + bb.y, oy = bb.x - aa.x;   // -i (aa - conj bb)
+  return cmk<Cx>((R)0.5 * fma_r(w.x, ox, fma_r(-w.y, oy, sx)), (R)0.5 * fma_r(w.x, oy, fma_r(w.y, ox, sy)));
+}
 template <typename R>
 __device__ __forceinline__ cplx<R> c2r_pre_ab(cplx<R> a, cplx<R> b, int k, const cplx<R>* __restrict__ twr,
                                               R scale) {

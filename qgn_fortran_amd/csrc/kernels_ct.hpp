@@ -395,7 +395,7 @@ __global__ void __launch_bounds__(ROWS * (N2 / 8)) k_row_jac_fused2(RowJacArgs a
   auto head = [&](const Cx(&xa)[8], const Cx(&xb)[8]) {
     Cx v[8];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) v[j] = c2r_pre_ab<R>(xa[j], xb[j], t + j * NB, tr, (R)1);
+    for (int j = 0; j < 8; ++j) v[j] = c2r_pre_fma<R>(xa[j], xb[j], t + j * NB, tr[t + j * NB]);
     Dft<Cx, 8>::run(v, +1);
     __syncthreads();   // earlier readers of buf are done
 #pragma unroll
@@ -435,10 +435,7 @@ __global__ void __launch_bounds__(ROWS * (N2 / 8)) k_row_jac_fused2(RowJacArgs a
       if (k < Kx) {
         const Cx aa = AFF ? za[i * pad_step<PS>(KSTEP)] : buf[padded<PS>(rr * N2 + k)];
         const Cx bb = k == 0 ? *z0 : (AFF ? zb[-i * pad_step<PS>(KSTEP)] : buf[padded<PS>(rr * N2 + N2 - k)]);
-        const Cx e = cmk<Cx>((R)0.5 * (aa.x + bb.x), (R)0.5 * (aa.y - bb.y));
-        const Cx dd = cmk<Cx>((R)0.5 * (aa.x - bb.x), (R)0.5 * (aa.y + bb.y));
-        const Cx o = cmk<Cx>(dd.y, -dd.x);
-        out[mix_at<R>(rr, k, Kxp)] = cadd(e, cmul(tr[k], o));
+        out[mix_at<R>(rr, k, Kxp)] = r2c_post_fma<R>(aa, bb, tr[k]);
       }
     }
   };
